@@ -1,0 +1,784 @@
+/*
+ * fhe_oracle.c -- CPU oracle for the FHEW/TFHE gate-bootstrapping path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker for the HIP library, never the product).
+ *
+ * Each function restates the reference routine it cites.  Arithmetic is the
+ * reference's own: canonical results from unsigned/signed __int128 '%'
+ * (FHE_ACC_runtime/include/nbt.h:59-101).
+ *
+ * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ *   orc_first_prime/previous_prime/root_of_unity/inverse_mod
+ *        -> param_1k.txt / param_2k.txt known values, brute-force minimum root
+ *           on small primes, oracle/_ref (reference nbt.cpp compiled here).
+ *   orc_ntt / orc_intt -> RLWExRGSW_output.txt (N=1024, 27-bit) and
+ *           RLWESUBS_output.txt (N=2048, 54-bit) time/NTT pairs; schoolbook
+ *           negacyclic convolution; NTT(X^k) closed form; oracle/_ref.
+ *   orc_digit_decompose / orc_rlwe_mult_rgsw -> RLWExRGSW_output.txt per-digit
+ *           polys, running sums and output; oracle/_ref at other (BG, d, Q).
+ *   orc_acc_init -> RLWExRGSW_output.txt input (b=0, NAND bounds, q=512);
+ *           oracle/_ref for every gate and b.
+ *   orc_poly_substitute / orc_rlwe_substitute / orc_rlwe_keyswitch ->
+ *           RLWESUBS_output.txt; expansion invariant (test.cpp:1889-1912).
+ *   orc_lwe_evaluate / orc_lwe_keyswitch / orc_scale_round / orc_lwe_modswitch /
+ *   orc_blind_rotate / orc_eval_bootstrap -> oracle/_ref (reference
+ *           LWE.cpp/RLWE.cpp compiled here, explicit keys), scale_round closed
+ *           form, transpose invariant, decrypted truth tables.
+ *   orc_rng_word / orc_gauss / keygen -> this build's own sampler (the
+ *           reference seeds from std::random_device, RNG.cpp:5-6, so it has no
+ *           reproducible keys).  Pinned by decryption of fresh encryptions and
+ *           by the device key generator agreeing word for word.
+ */
+#include "fhe_oracle.h"
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+typedef unsigned __int128 u128;
+typedef __int128 s128;
+
+/* nbt.h:59-101: canonical reductions */
+static inline uint64_t modu(u128 a, uint64_t m) { return (uint64_t)(a % (u128)m); }
+static inline uint64_t mods(s128 a, uint64_t m) {
+    return (uint64_t)(((a % (s128)m) + (s128)m) % (s128)m);
+}
+
+/* ------------------------------------------------------------------ */
+/* number theory                                                       */
+/* ------------------------------------------------------------------ */
+
+/* nbt.cpp:7-26 */
+uint64_t orc_reverse_bit(uint64_t order, uint64_t input) {
+    int bits = 0;
+    while ((1ULL << bits) < order) bits++;
+    uint64_t r = 0;
+    for (int i = 0; i < bits; i++) r |= ((input >> i) & 1ULL) << (bits - 1 - i);
+    return r;
+}
+
+/* nbt.cpp:28-47 */
+uint64_t orc_power_mod(uint64_t base, uint64_t exp, uint64_t mod) {
+    uint64_t b = base % mod;
+    u128 r = 1;
+    while (exp) {
+        if (exp & 1) r = (r * b) % mod;
+        b = modu((u128)b * b, mod);
+        exp >>= 1;
+    }
+    return (uint64_t)r;
+}
+
+/* nbt.cpp:73-90 (Fermat inverse, prime modulus) */
+uint64_t orc_inverse_mod(uint64_t x, uint64_t mod) { return orc_power_mod(x, mod - 2, mod); }
+
+/* nbt.cpp:152-207 restated as a deterministic Miller-Rabin (these bases are
+ * exact for all 64-bit inputs, so the answer equals the reference's 20/40-round
+ * randomized test on every prime it accepts). */
+int orc_is_prime(uint64_t x) {
+    if (x < 2) return 0;
+    static const uint64_t small[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    for (unsigned i = 0; i < sizeof small / sizeof small[0]; i++) {
+        if (x == small[i]) return 1;
+        if (x % small[i] == 0) return 0;
+    }
+    uint64_t d = x - 1;
+    int s = 0;
+    while ((d & 1) == 0) { d >>= 1; s++; }
+    static const uint64_t bases[] = {2, 325, 9375, 28178, 450775, 9780504, 1795265022};
+    for (unsigned i = 0; i < sizeof bases / sizeof bases[0]; i++) {
+        uint64_t a = bases[i] % x;
+        if (a == 0) continue;
+        uint64_t y = orc_power_mod(a, d, x);
+        if (y == 1 || y == x - 1) continue;
+        int comp = 1;
+        for (int r = 1; r < s; r++) {
+            y = modu((u128)y * y, x);
+            if (y == x - 1) { comp = 0; break; }
+        }
+        if (comp) return 0;
+    }
+    return 1;
+}
+
+/* nbt.cpp:209-245: first candidate is 2^nbit + m - (2^nbit mod m) + 1, step m */
+uint64_t orc_first_prime(int nbit, int order) {
+    uint64_t m = 2ULL * (uint64_t)order;
+    uint64_t base = 1ULL << nbit;
+    uint64_t cand = base + m - (base % m) + 1;
+    while (!orc_is_prime(cand)) cand += m;
+    return cand;
+}
+
+/* nbt.cpp:247-266 */
+uint64_t orc_previous_prime(uint64_t prime, int order) {
+    uint64_t m = 2ULL * (uint64_t)order;
+    uint64_t cand = prime - m;
+    while (!orc_is_prime(cand)) cand -= m;
+    return cand;
+}
+
+/* nbt.cpp:300-346: the smallest primitive 2N-th root of unity.  Any primitive
+ * root r gives all of them as r^k for odd k < 2N; take the minimum. */
+uint64_t orc_root_of_unity(int order, uint64_t mod) {
+    uint64_t m = 2ULL * (uint64_t)order;
+    if ((mod - 1) % m) return 0;
+    uint64_t r = 0;
+    for (uint64_t g = 2; g < mod; g++) {
+        uint64_t c = orc_power_mod(g, (mod - 1) / m, mod);
+        if (orc_power_mod(c, (uint64_t)order, mod) == mod - 1) { r = c; break; }
+    }
+    if (!r) return 0;
+    uint64_t r2 = modu((u128)r * r, mod);
+    uint64_t cur = r, best = r;
+    for (uint64_t k = 1; k < m; k += 2) {
+        if (cur < best && cur != 1) best = cur;
+        cur = modu((u128)cur * r2, mod);
+    }
+    return best;
+}
+
+/* params.cpp:149-175 */
+void orc_rou_tables(uint64_t rou, uint64_t mod, int order, uint64_t *tbl, uint64_t *itbl) {
+    uint64_t irou = orc_inverse_mod(rou, mod);
+    tbl[0] = 1;
+    itbl[0] = 1;
+    uint64_t t = 1, it = 1;
+    for (uint64_t i = 1; i < (uint64_t)order; i++) {
+        t = modu((u128)t * rou, mod);
+        it = modu((u128)it * irou, mod);
+        uint64_t r = orc_reverse_bit((uint64_t)order, i);
+        tbl[r] = t;
+        itbl[r] = it;
+    }
+}
+
+static int cdt_build(double sigma, uint64_t *cdt, int cap);
+
+/* digit counts exactly as the apps compute them (fhe_wo_fpga.cpp:122,128,133,138) */
+static uint32_t digits_of(double lg_num, double base) {
+    return (uint32_t)ceil(lg_num / log2(base));
+}
+
+int orc_params_init(orc_params *p, int n, uint64_t q, int N, uint64_t Q, int nbit,
+                    uint32_t BGbits, uint64_t Br, uint64_t Bks, uint32_t Bks_R_bits,
+                    double stddev, uint64_t E_rlwe) {
+    memset(p, 0, sizeof *p);
+    if (N < 4 || (N & (N - 1)) || q < 4 || (q & (q - 1)) || (2ULL * N) % q) return -1;
+    if (Q == 0) Q = orc_previous_prime(orc_first_prime(nbit, N), N);
+    if (Q < 3 || (Q - 1) % (2ULL * N) || Q >= (1ULL << 62)) return -1;
+    if (BGbits < 1 || BGbits > 32 || Br < 2 || Bks < 2 || Bks_R_bits < 1) return -1;
+    p->n = n; p->q = q; p->N = N; p->Q = Q;
+    p->iN = orc_inverse_mod((uint64_t)N, Q);
+    p->ROU = orc_root_of_unity(N, Q);
+    if (!p->ROU) return -1;
+    p->BG = 1ULL << BGbits; p->BGbits = BGbits;
+    p->digitG = digits_of(log2((double)Q), (double)p->BG);
+    p->Br = Br; p->digitR = digits_of(log2((double)q), (double)Br);
+    p->Bks = Bks; p->digitKS = digits_of(log2((double)Q), (double)Bks);
+    p->Bks_R = 1ULL << Bks_R_bits; p->Bks_R_bits = Bks_R_bits;
+    p->digitKS_R = digits_of(log2((double)Q), (double)p->Bks_R);
+    p->E = q / 16;
+    p->E_rlwe = E_rlwe;
+    p->stddev = stddev;
+    p->cdt_len = cdt_build(stddev, p->cdt, 64);
+    /* fhe_wo_fpga.cpp:141-146 */
+    static const uint64_t gc[6] = {5, 7, 1, 3, 5, 1};
+    for (int g = 0; g < 6; g++) p->gate_constant[g] = gc[g] * (q >> 3);
+    p->rou = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    p->irou = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    if (!p->rou || !p->irou) { orc_params_free(p); return -1; }
+    orc_rou_tables(p->ROU, Q, N, p->rou, p->irou);
+    return 0;
+}
+
+void orc_params_free(orc_params *p) {
+    free(p->rou);
+    free(p->irou);
+    p->rou = p->irou = NULL;
+}
+
+/* ------------------------------------------------------------------ */
+/* NTT (nbt.cpp:423-491)                                               */
+/* ------------------------------------------------------------------ */
+
+void orc_ntt(uint64_t *a, int N, uint64_t Q, const uint64_t *rou) {
+    int t = N;
+    for (int m = 1; m < N; m *= 2) {
+        t /= 2;
+        for (int i = 0; i < m; i++) {
+            int j1 = 2 * i * t, j2 = j1 + t;
+            uint64_t S = rou[m + i];
+            for (int j = j1; j < j2; j++) {
+                uint64_t U = a[j];
+                uint64_t V = modu((u128)a[j + t] * S, Q);
+                a[j] = modu((u128)U + V, Q);
+                a[j + t] = mods((s128)U - (s128)V, Q);
+            }
+        }
+    }
+}
+
+void orc_intt(uint64_t *a, int N, uint64_t Q, const uint64_t *irou, uint64_t iN) {
+    int t = 1;
+    for (int m = N / 2; m >= 1; m /= 2) {
+        int j1 = 0;
+        for (int i = 0; i < m; i++) {
+            int j2 = j1 + t;
+            uint64_t S = irou[m + i];
+            for (int j = j1; j < j2; j++) {
+                uint64_t U = a[j], V = a[j + t];
+                a[j] = modu((u128)U + V, Q);
+                a[j + t] = mods(((s128)U - (s128)V) * (s128)S, Q);
+            }
+            j1 += 2 * t;
+        }
+        t *= 2;
+    }
+    for (int i = 0; i < N; i++) a[i] = modu((u128)a[i] * iN, Q);
+}
+
+void orc_ntt_batch(const orc_params *p, uint64_t *a, int64_t count) {
+    for (int64_t c = 0; c < count; c++) orc_ntt(a + c * p->N, p->N, p->Q, p->rou);
+}
+
+void orc_intt_batch(const orc_params *p, uint64_t *a, int64_t count) {
+    for (int64_t c = 0; c < count; c++) orc_intt(a + c * p->N, p->N, p->Q, p->irou, p->iN);
+}
+
+/* ------------------------------------------------------------------ */
+/* RLWE / RGSW                                                          */
+/* ------------------------------------------------------------------ */
+
+/* RLWE.cpp:23-37 */
+void orc_poly_substitute(const uint64_t *in, int sub_factor, int N, uint64_t Q, uint64_t *out) {
+    int32_t power = (N >> sub_factor) + 1;
+    for (int32_t i = 0; i < N; i++) {
+        int32_t io = (i * power) % N;
+        if (((i * power / N) % 2) == 1) out[io] = (Q - in[i]) % Q;
+        else out[io] = in[i];
+    }
+}
+
+/* RLWE.cpp:623-640: time-domain unsigned digits, low digit first, then NTT */
+void orc_digit_decompose(const orc_params *p, const uint64_t *rlwe, uint32_t d, uint32_t bits,
+                         uint64_t *out) {
+    int N = p->N;
+    uint64_t mask = (bits >= 64) ? ~0ULL : ((1ULL << bits) - 1);
+    uint64_t *t = (uint64_t *)malloc(sizeof(uint64_t) * 2 * N);
+    memcpy(t, rlwe, sizeof(uint64_t) * 2 * N);
+    orc_intt(t, N, p->Q, p->irou, p->iN);
+    orc_intt(t + N, N, p->Q, p->irou, p->iN);
+    for (int i = 0; i < N; i++) {
+        uint64_t xa = t[i], xb = t[N + i];
+        for (uint32_t j = 0; j < d; j++) {
+            out[(size_t)j * 2 * N + i] = xa & mask;
+            xa >>= bits;
+            out[(size_t)j * 2 * N + N + i] = xb & mask;
+            xb >>= bits;
+        }
+    }
+    for (uint32_t j = 0; j < d; j++) {
+        orc_ntt(out + (size_t)j * 2 * N, N, p->Q, p->rou);
+        orc_ntt(out + (size_t)j * 2 * N + N, N, p->Q, p->rou);
+    }
+    free(t);
+}
+
+/* RLWE.cpp:642-704: out = sum_h rgsw[h][0] * dec_h(a) + rgsw[h][1] * dec_h(b) */
+void orc_rlwe_mult_rgsw(const orc_params *p, const uint64_t *acc, const uint64_t *rgsw, uint64_t *out) {
+    int N = p->N;
+    uint32_t d = p->digitG;
+    uint64_t Q = p->Q;
+    uint64_t *dec = (uint64_t *)malloc(sizeof(uint64_t) * 2 * N * d);
+    orc_digit_decompose(p, acc, d, p->BGbits, dec);
+    uint64_t *oa = out, *ob = out + N;
+    for (int i = 0; i < N; i++) oa[i] = ob[i] = 0;
+    for (uint32_t h = 0; h < d; h++) {
+        const uint64_t *da = dec + (size_t)h * 2 * N, *db = da + N;
+        const uint64_t *r0 = rgsw + (size_t)h * 4 * N;      /* [h][0]: a, b */
+        const uint64_t *r1 = r0 + 2 * N;                     /* [h][1]: a, b */
+        for (int i = 0; i < N; i++) {
+            /* RLWE_mult_NTT_poly then RLWE_addition (RLWE.cpp:432-442, 377-387) */
+            oa[i] = modu((u128)modu((u128)r0[i] * da[i], Q) + oa[i], Q);
+            ob[i] = modu((u128)modu((u128)r0[N + i] * da[i], Q) + ob[i], Q);
+            oa[i] = modu((u128)modu((u128)r1[i] * db[i], Q) + oa[i], Q);
+            ob[i] = modu((u128)modu((u128)r1[N + i] * db[i], Q) + ob[i], Q);
+        }
+    }
+    free(dec);
+}
+
+void orc_rlwe_mult_rgsw_batch(const orc_params *p, const uint64_t *acc, const uint64_t *rgsw,
+                              int64_t rgsw_stride, uint64_t *out, int64_t count) {
+    size_t rl = 2 * (size_t)p->N;
+    for (int64_t c = 0; c < count; c++)
+        orc_rlwe_mult_rgsw(p, acc + c * rl, rgsw + c * rgsw_stride, out + c * rl);
+}
+
+/* RLWE.cpp:706-724 (embed_factor = 2N/q, RLWE.cpp:744) */
+void orc_acc_init(const orc_params *p, uint64_t bound1, uint64_t bound2, uint64_t value, uint64_t *acc) {
+    int N = p->N;
+    uint64_t q = p->q, Q = p->Q;
+    uint64_t Q8 = Q / 8 + 1, negQ8 = Q - Q8;
+    uint64_t ef = 2ULL * (uint64_t)N / q;
+    memset(acc, 0, sizeof(uint64_t) * 2 * N);
+    uint64_t *b = acc + N;
+    for (uint64_t i = 0; i < q / 2; i++) {
+        uint64_t tmp = (uint64_t)((int64_t)value - (int64_t)i) % q;
+        if (bound1 < bound2) b[i * ef] = (tmp >= bound1 && tmp < bound2) ? negQ8 : Q8;
+        else b[i * ef] = (tmp >= bound2 && tmp < bound1) ? Q8 : negQ8;
+    }
+    orc_ntt(acc, N, Q, p->rou);
+    orc_ntt(b, N, Q, p->rou);
+}
+
+/* RLWE.cpp:444-462 (input in NTT domain) */
+void orc_rlwe_substitute(const orc_params *p, const uint64_t *in, int sub_factor, uint64_t *out) {
+    int N = p->N;
+    uint64_t *t = (uint64_t *)malloc(sizeof(uint64_t) * 2 * N);
+    memcpy(t, in, sizeof(uint64_t) * 2 * N);
+    orc_intt(t, N, p->Q, p->irou, p->iN);
+    orc_intt(t + N, N, p->Q, p->irou, p->iN);
+    orc_poly_substitute(t, sub_factor, N, p->Q, out);
+    orc_poly_substitute(t + N, sub_factor, N, p->Q, out + N);
+    orc_ntt(out, N, p->Q, p->rou);
+    orc_ntt(out + N, N, p->Q, p->rou);
+    free(t);
+}
+
+/* RLWE.cpp:464-510: b' = b - sum ks_i.b * dec_i(a); a' = -sum ks_i.a * dec_i(a) */
+void orc_rlwe_keyswitch(const orc_params *p, const uint64_t *in, const uint64_t *ks, uint64_t *out) {
+    int N = p->N;
+    uint64_t Q = p->Q;
+    uint32_t d = p->digitKS_R;
+    uint64_t *dec = (uint64_t *)malloc(sizeof(uint64_t) * 2 * N * d);
+    orc_digit_decompose(p, in, d, p->Bks_R_bits, dec);
+    uint64_t *oa = out, *ob = out + N;
+    for (int i = 0; i < N; i++) { oa[i] = 0; ob[i] = in[N + i]; }
+    for (uint32_t l = 0; l < d; l++) {
+        const uint64_t *da = dec + (size_t)l * 2 * N;
+        const uint64_t *ka = ks + (size_t)l * 2 * N, *kb = ka + N;
+        for (int i = 0; i < N; i++) {
+            oa[i] = mods((s128)oa[i] - (s128)modu((u128)ka[i] * da[i], Q), Q);
+            ob[i] = mods((s128)ob[i] - (s128)modu((u128)kb[i] * da[i], Q), Q);
+        }
+    }
+    free(dec);
+}
+
+/* ------------------------------------------------------------------ */
+/* LWE                                                                 */
+/* ------------------------------------------------------------------ */
+
+/* LWE.cpp:203-222 (and the c1 == c2 rejection of RLWE.cpp:727-730) */
+int orc_lwe_evaluate(const orc_params *p, const uint64_t *c1, const uint64_t *c2, int gate, uint64_t *out) {
+    int n = p->n;
+    uint64_t q = p->q;
+    if (memcmp(c1, c2, sizeof(uint64_t) * (n + 1)) == 0) return -1;
+    for (int i = 0; i <= n; i++) {
+        if (gate == ORC_XOR || gate == ORC_XNOR) out[i] = (2 * (c1[i] - c2[i])) % q;
+        else out[i] = (c1[i] + c2[i]) % q;
+    }
+    return 0;
+}
+
+/* LWE.cpp:224-231 */
+void orc_lwe_eval_not(const orc_params *p, const uint64_t *c, uint64_t *out) {
+    uint64_t q = p->q;
+    for (int i = 0; i < p->n; i++) out[i] = (q - c[i]) % q;
+    out[p->n] = ((q / 4) - c[p->n]) % q;
+}
+
+/* LWE.cpp:233-253 with the NTT-domain transpose of RLWE.cpp:218-226 */
+void orc_lwe_keyswitch(const orc_params *p, const uint64_t *acc, const uint64_t *ksk, uint64_t *out) {
+    int N = p->N, n = p->n;
+    uint64_t Q = p->Q, Bks = p->Bks;
+    uint32_t dks = p->digitKS;
+    uint64_t *t = (uint64_t *)malloc(sizeof(uint64_t) * 2 * N);
+    for (int i = 0; i < N; i++) t[i] = acc[N - 1 - i];
+    memcpy(t + N, acc + N, sizeof(uint64_t) * N);
+    orc_intt(t, N, Q, p->irou, p->iN);
+    orc_intt(t + N, N, Q, p->irou, p->iN);
+    uint64_t Q8 = Q / 8 + 1;
+    for (int k = 0; k < n; k++) out[k] = 0;
+    out[n] = modu((u128)t[N] + Q8, Q);
+    size_t row = (size_t)n + 1;
+    for (int i = 0; i < N; i++) {
+        uint64_t at = t[i];
+        for (uint32_t j = 0; j < dks; j++) {
+            uint64_t dg = at % Bks;
+            const uint64_t *r = ksk + (((size_t)dg * dks + j) * N + i) * row;
+            for (int k = 0; k <= n; k++) out[k] = mods((s128)(int64_t)out[k] - (s128)(int64_t)r[k], Q);
+            at /= Bks;
+        }
+    }
+    free(t);
+}
+
+/* LWE.cpp:255-260: floor((Q + 2 x q) / 2Q), not reduced mod q */
+uint64_t orc_scale_round(uint64_t x, uint64_t new_mod, uint64_t old_mod) {
+    return (uint64_t)(((u128)old_mod + 2 * (u128)x * (u128)new_mod) / (2 * (u128)old_mod));
+}
+
+/* LWE.cpp:262-269 */
+void orc_lwe_modswitch(const orc_params *p, const uint64_t *in, uint64_t *out) {
+    for (int i = 0; i <= p->n; i++) out[i] = orc_scale_round(in[i], p->q, p->Q);
+}
+
+/* LWE.cpp:148-190 (t = 4, E = q/16) */
+uint64_t orc_lwe_decrypt(const orc_params *p, const uint64_t *sk, const uint64_t *c) {
+    uint64_t q = p->q, m = c[p->n];
+    for (int i = 0; i < p->n; i++) m = (m - c[i] * sk[i]) % q;
+    m = (m + p->E * 2) % q;
+    return m * 4 / q;
+}
+
+/* ------------------------------------------------------------------ */
+/* gate bootstrap                                                      */
+/* ------------------------------------------------------------------ */
+
+static size_t rgsw_words(const orc_params *p) { return (size_t)p->digitG * 4 * p->N; }
+
+/* RLWE.cpp:754-767: i outer, j inner, zero residues skipped */
+int64_t orc_blind_rotate(const orc_params *p, const uint64_t *lwe, const uint64_t *bsk, uint64_t *acc) {
+    int N = p->N, n = p->n;
+    uint64_t q = p->q, Br = p->Br;
+    size_t rw = rgsw_words(p);
+    uint64_t *tmp = (uint64_t *)malloc(sizeof(uint64_t) * 2 * N);
+    int64_t applied = 0;
+    for (int i = 0; i < n; i++) {
+        uint64_t nega = (q - lwe[i]) % q;
+        for (uint32_t j = 0; j < p->digitR; j++) {
+            uint64_t r = nega % Br;
+            if (r > 0) {
+                const uint64_t *key = bsk + (((size_t)r * p->digitR + j) * n + i) * rw;
+                orc_rlwe_mult_rgsw(p, acc, key, tmp);
+                memcpy(acc, tmp, sizeof(uint64_t) * 2 * N);
+                applied++;
+            }
+            nega /= Br;
+        }
+    }
+    free(tmp);
+    return applied;
+}
+
+/* RLWE.cpp:726-786 */
+int orc_eval_bootstrap(const orc_params *p, const uint64_t *c1, const uint64_t *c2, int gate,
+                       const uint64_t *bsk, const uint64_t *ksk, uint64_t *out,
+                       uint64_t *cap_acc, uint64_t *cap_ks) {
+    int N = p->N, n = p->n;
+    uint64_t *c = (uint64_t *)malloc(sizeof(uint64_t) * (n + 1));
+    uint64_t *acc = (uint64_t *)malloc(sizeof(uint64_t) * 2 * N);
+    uint64_t *ks = (uint64_t *)malloc(sizeof(uint64_t) * (n + 1));
+    if (orc_lwe_evaluate(p, c1, c2, gate, c) != 0) { free(c); free(acc); free(ks); return -1; }
+    uint64_t b1 = p->gate_constant[gate] % p->q;
+    uint64_t b2 = (b1 + p->q / 2) % p->q;
+    orc_acc_init(p, b1, b2, c[n], acc);
+    orc_blind_rotate(p, c, bsk, acc);
+    if (cap_acc) memcpy(cap_acc, acc, sizeof(uint64_t) * 2 * N);
+    orc_lwe_keyswitch(p, acc, ksk, ks);
+    if (cap_ks) memcpy(cap_ks, ks, sizeof(uint64_t) * (n + 1));
+    orc_lwe_modswitch(p, ks, out);
+    free(c); free(acc); free(ks);
+    return 0;
+}
+
+typedef struct {
+    const orc_params *p;
+    const uint64_t *c1, *c2, *bsk, *ksk;
+    const int32_t *gates;
+    uint64_t *out, *cap_acc, *cap_ks;
+    int64_t begin, end;
+    int rc;
+} bs_job;
+
+static void *bs_worker(void *arg) {
+    bs_job *j = (bs_job *)arg;
+    const orc_params *p = j->p;
+    size_t lw = (size_t)p->n + 1, rl = 2 * (size_t)p->N;
+    for (int64_t c = j->begin; c < j->end; c++) {
+        int r = orc_eval_bootstrap(p, j->c1 + c * lw, j->c2 + c * lw, j->gates[c], j->bsk, j->ksk,
+                                   j->out + c * lw, j->cap_acc ? j->cap_acc + c * rl : NULL,
+                                   j->cap_ks ? j->cap_ks + c * lw : NULL);
+        if (r) j->rc = r;
+    }
+    return NULL;
+}
+
+static int resolve_threads(int threads, int64_t work) {
+    if (threads <= 0) {
+        long c = sysconf(_SC_NPROCESSORS_ONLN);
+        threads = c > 0 ? (int)c : 1;
+    }
+    if (threads > 256) threads = 256;
+    if (work < threads) threads = (int)(work > 0 ? work : 1);
+    return threads;
+}
+
+int orc_eval_bootstrap_batch(const orc_params *p, const uint64_t *c1, const uint64_t *c2,
+                             const int32_t *gates, int64_t count, const uint64_t *bsk,
+                             const uint64_t *ksk, uint64_t *out, uint64_t *cap_acc,
+                             uint64_t *cap_ks, int threads) {
+    threads = resolve_threads(threads, count);
+    pthread_t th[256];
+    bs_job jobs[256];
+    int rc = 0;
+    for (int t = 0; t < threads; t++) {
+        bs_job j = {p, c1, c2, bsk, ksk, gates, out, cap_acc, cap_ks,
+                    count * t / threads, count * (t + 1) / threads, 0};
+        jobs[t] = j;
+        pthread_create(&th[t], NULL, bs_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; t++) {
+        pthread_join(th[t], NULL);
+        if (jobs[t].rc) rc = jobs[t].rc;
+    }
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* seeded sampler and key generation (this build's own; the reference  */
+/* draws from std::random_device, RNG.cpp:5-6,20-21)                   */
+/* ------------------------------------------------------------------ */
+
+static inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+/* word = mix64(mix64(seed + stream * G) + (ctr + 1) * G), G the golden gamma */
+uint64_t orc_rng_word(uint64_t seed, uint64_t stream, uint64_t ctr) {
+    const uint64_t G = 0x9E3779B97F4A7C15ULL;
+    uint64_t key = mix64(seed + stream * G);
+    return mix64(key + (ctr + 1) * G);
+}
+
+static inline uint64_t uniform_mod(uint64_t w, uint64_t m) { return (uint64_t)(((u128)w * m) >> 64); }
+
+/* Gaussian: |X| from a cumulative table of trunc(N(0, sigma)) (RNG.cpp:65-74
+ * draws trunc(normal) then % bound); sign from the low bit. */
+static int cdt_build(double sigma, uint64_t *cdt, int cap) {
+    int k = 0;
+    for (; k < cap; k++) {
+        double c = erf((double)(k + 1) / (sigma * sqrt(2.0)));
+        cdt[k] = (c >= 1.0) ? (1ULL << 63) : (uint64_t)ldexp(c, 63);
+        if (cdt[k] == (1ULL << 63)) return k + 1;
+    }
+    cdt[cap - 1] = 1ULL << 63;
+    return cap;
+}
+
+int64_t orc_gauss(const orc_params *p, uint64_t word) {
+    uint64_t v = word >> 1;
+    int k = 0;
+    while (k < p->cdt_len - 1 && v >= p->cdt[k]) k++;
+    if (k == 0) return 0;
+    return (word & 1) ? -(int64_t)k : (int64_t)k;
+}
+
+void orc_lwe_secret(const orc_params *p, uint64_t seed, int binary, uint64_t *sk) {
+    for (int i = 0; i < p->n; i++) {
+        uint64_t w = orc_rng_word(seed, ORC_STREAM(ORC_TAG_LWE_SK, 0), (uint64_t)i);
+        if (binary) sk[i] = w >> 63;
+        else sk[i] = (uniform_mod(w, 3) + p->q - 1) % p->q; /* {-1,0,1} mod q */
+    }
+}
+
+void orc_rlwe_secret(const orc_params *p, uint64_t seed, uint64_t *sk) {
+    for (int i = 0; i < p->N; i++) {
+        uint64_t w = orc_rng_word(seed, ORC_STREAM(ORC_TAG_RLWE_SK, 0), (uint64_t)i);
+        sk[i] = (uniform_mod(w, 3) + p->Q - 1) % p->Q;
+    }
+}
+
+/* RLWE.cpp:324-344 with a zero message: a uniform (time), e Gaussian mod
+ * E_rlwe, both to NTT, b = a*s + e. */
+void orc_rlwe_encrypt_zero(const orc_params *p, const uint64_t *sk_ntt, uint64_t seed, uint64_t stream,
+                           uint64_t *out) {
+    int N = p->N;
+    uint64_t Q = p->Q;
+    uint64_t *a = out, *b = out + N;
+    for (int k = 0; k < N; k++) {
+        a[k] = uniform_mod(orc_rng_word(seed, stream, (uint64_t)k), Q);
+        int64_t e = orc_gauss(p, orc_rng_word(seed, stream, (uint64_t)(N + k))) % (int64_t)p->E_rlwe;
+        b[k] = mods((s128)e, Q);
+    }
+    orc_ntt(a, N, Q, p->rou);
+    orc_ntt(b, N, Q, p->rou);
+    for (int k = 0; k < N; k++) b[k] = modu((u128)a[k] * sk_ntt[k] + (u128)b[k], Q);
+}
+
+/* RLWE.cpp:512-526 (plaintext X^mm with sign) and 561-577 (rows) */
+void orc_rgsw_encrypt_monomial(const orc_params *p, const uint64_t *sk_ntt, uint64_t m_in,
+                               uint64_t seed, uint64_t stream_base, uint64_t *out) {
+    int N = p->N;
+    uint64_t Q = p->Q, q = p->q;
+    uint64_t mm = (m_in % q) * (2ULL * N / q);
+    uint64_t sign = 1;
+    if (mm >= (uint64_t)N) { mm -= N; sign = Q - 1; }
+    uint64_t *m = (uint64_t *)calloc(N, sizeof(uint64_t));
+    m[mm] = sign;
+    orc_ntt(m, N, Q, p->rou);
+    for (uint32_t h = 0; h < p->digitG; h++) {
+        for (int g = 0; g < 2; g++) {
+            uint64_t *r = out + ((size_t)h * 2 + g) * 2 * N;
+            orc_rlwe_encrypt_zero(p, sk_ntt, seed, stream_base + (uint64_t)(h * 2 + g), r);
+            uint64_t *dst = r + (size_t)g * N; /* g=0 -> a, g=1 -> b */
+            for (int k = 0; k < N; k++) dst[k] = modu((u128)dst[k] + m[k], Q);
+        }
+        for (int k = 0; k < N; k++) m[k] = modu((u128)p->BG * m[k], Q);
+    }
+    free(m);
+}
+
+typedef struct {
+    const orc_params *p;
+    const uint64_t *lwe_sk, *sk_ntt;
+    uint64_t seed, *bsk;
+    int64_t begin, end;
+} bsk_job;
+
+static void *bsk_worker(void *arg) {
+    bsk_job *j = (bsk_job *)arg;
+    const orc_params *p = j->p;
+    size_t rw = rgsw_words(p);
+    int n = p->n;
+    for (int64_t e = j->begin; e < j->end; e++) {
+        /* e = (r * digitR + jj) * n + k */
+        int64_t k = e % n, rj = e / n;
+        int64_t jj = rj % p->digitR, r = rj / p->digitR;
+        uint64_t powbr = 1;
+        for (int64_t t = 0; t < jj; t++) powbr = (powbr * p->Br) % p->q;
+        /* RLWE.cpp:589: ((r * Br^j mod q) * s_k) mod q */
+        uint64_t m = (((uint64_t)r * powbr) % p->q * j->lwe_sk[k]) % p->q;
+        orc_rgsw_encrypt_monomial(p, j->sk_ntt, m, j->seed,
+                                  ORC_STREAM(ORC_TAG_BSK, (uint64_t)e * 2 * p->digitG),
+                                  j->bsk + (size_t)e * rw);
+    }
+    return NULL;
+}
+
+/* RLWE.cpp:579-606 */
+void orc_bsk_gen(const orc_params *p, const uint64_t *lwe_sk, const uint64_t *rlwe_sk_time,
+                 uint64_t seed, uint64_t *bsk, int threads) {
+    int N = p->N;
+    uint64_t *sk_ntt = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    memcpy(sk_ntt, rlwe_sk_time, sizeof(uint64_t) * N);
+    orc_ntt(sk_ntt, N, p->Q, p->rou);
+    int64_t total = (int64_t)p->Br * p->digitR * p->n;
+    threads = resolve_threads(threads, total);
+    pthread_t th[256];
+    bsk_job jobs[256];
+    for (int t = 0; t < threads; t++) {
+        bsk_job j = {p, lwe_sk, sk_ntt, seed, bsk, total * t / threads, total * (t + 1) / threads};
+        jobs[t] = j;
+        pthread_create(&th[t], NULL, bsk_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(sk_ntt);
+}
+
+typedef struct {
+    const orc_params *p;
+    const uint64_t *z, *s;
+    uint64_t seed, *ksk;
+    int64_t begin, end;
+} ksk_job;
+
+static void *ksk_worker(void *arg) {
+    ksk_job *jb = (ksk_job *)arg;
+    const orc_params *p = jb->p;
+    int N = p->N, n = p->n;
+    uint64_t Q = p->Q;
+    size_t row = (size_t)n + 1;
+    for (int64_t e = jb->begin; e < jb->end; e++) {
+        /* e = (i * digitKS + j) * N + k */
+        int64_t k = e % N, ij = e / N;
+        int64_t j = ij % p->digitKS, i = ij / p->digitKS;
+        uint64_t powks = 1;
+        for (int64_t t = 0; t < j; t++) powks = modu((u128)powks * p->Bks, Q);
+        uint64_t stream = ORC_STREAM(ORC_TAG_KSK, (uint64_t)e);
+        uint64_t mult = modu((u128)modu((u128)(uint64_t)i * powks, Q) * jb->z[k], Q);
+        int64_t err = orc_gauss(p, orc_rng_word(jb->seed, stream, (uint64_t)n)) % (int64_t)p->E_rlwe;
+        uint64_t *r = jb->ksk + (size_t)e * row;
+        uint64_t b = mods((s128)err + (s128)mult, Q);
+        for (int h = 0; h < n; h++) {
+            r[h] = uniform_mod(orc_rng_word(jb->seed, stream, (uint64_t)h), Q);
+            b = modu((u128)b + modu((u128)r[h] * jb->s[h], Q), Q);
+        }
+        r[n] = b;
+    }
+    return NULL;
+}
+
+/* LWE.cpp:55-106: s mapped into Z_Q (s >= q/2 -> s + Q - q) */
+void orc_ksk_gen(const orc_params *p, const uint64_t *rlwe_sk_time, const uint64_t *lwe_sk,
+                 uint64_t seed, uint64_t *ksk, int threads) {
+    int n = p->n;
+    uint64_t *s = (uint64_t *)malloc(sizeof(uint64_t) * n);
+    for (int h = 0; h < n; h++) s[h] = (lwe_sk[h] >= p->q / 2) ? lwe_sk[h] + (p->Q - p->q) : lwe_sk[h];
+    int64_t total = (int64_t)p->Bks * p->digitKS * p->N;
+    threads = resolve_threads(threads, total);
+    pthread_t th[256];
+    ksk_job jobs[256];
+    for (int t = 0; t < threads; t++) {
+        ksk_job j = {p, rlwe_sk_time, s, seed, ksk, total * t / threads, total * (t + 1) / threads};
+        jobs[t] = j;
+        pthread_create(&th[t], NULL, ksk_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(s);
+}
+
+/* RLWE.cpp:293-322: ks[l] = (a_l, a_l*s_new + e + s_old*Bks_R^l), NTT domain */
+void orc_rlwe_ksk_gen(const orc_params *p, const uint64_t *old_sk_ntt, const uint64_t *new_sk_ntt,
+                      uint64_t seed, uint64_t stream_base, uint64_t *ks) {
+    int N = p->N;
+    uint64_t Q = p->Q;
+    uint64_t *t = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    memcpy(t, old_sk_ntt, sizeof(uint64_t) * N);
+    for (uint32_t l = 0; l < p->digitKS_R; l++) {
+        uint64_t *r = ks + (size_t)l * 2 * N;
+        orc_rlwe_encrypt_zero(p, new_sk_ntt, seed, stream_base + l, r);
+        for (int k = 0; k < N; k++) r[N + k] = modu((u128)r[N + k] + t[k], Q);
+        for (int k = 0; k < N; k++) t[k] = modu((u128)p->Bks_R * t[k], Q);
+    }
+    free(t);
+}
+
+/* app.cpp:4-24: key i switches s(X^(N/2^i + 1)) back to s(X), i < log2 N */
+void orc_subs_key_gen(const orc_params *p, const uint64_t *rlwe_sk_time, uint64_t seed, uint64_t *subs) {
+    int N = p->N, logN = 0;
+    while ((1 << logN) < N) logN++;
+    uint64_t *sk_ntt = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    uint64_t *sub = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    memcpy(sk_ntt, rlwe_sk_time, sizeof(uint64_t) * N);
+    orc_ntt(sk_ntt, N, p->Q, p->rou);
+    size_t per = (size_t)p->digitKS_R * 2 * N;
+    for (int i = 0; i < logN; i++) {
+        orc_poly_substitute(rlwe_sk_time, i, N, p->Q, sub);
+        orc_ntt(sub, N, p->Q, p->rou);
+        orc_rlwe_ksk_gen(p, sub, sk_ntt, seed, ORC_STREAM(ORC_TAG_SUBS, (uint64_t)i * 64), subs + i * per);
+    }
+    free(sk_ntt);
+    free(sub);
+}
+
+/* LWE.cpp:108-146 (t = 4): b = e + m*q/4 + <a, s> mod q */
+void orc_lwe_encrypt(const orc_params *p, const uint64_t *sk, uint64_t m, uint64_t seed, uint64_t index,
+                     uint64_t *out) {
+    int n = p->n;
+    uint64_t q = p->q;
+    uint64_t stream = ORC_STREAM(ORC_TAG_LWE_ENC, index);
+    int64_t e = orc_gauss(p, orc_rng_word(seed, stream, (uint64_t)n)) % (int64_t)p->E;
+    uint64_t b = (uint64_t)(e + (int64_t)(m * (q / 4))) % q;
+    for (int i = 0; i < n; i++) {
+        out[i] = uniform_mod(orc_rng_word(seed, stream, (uint64_t)i), q);
+        b = (b + (out[i] * sk[i]) % q) % q;
+    }
+    out[n] = b;
+}

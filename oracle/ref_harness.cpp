@@ -1,0 +1,224 @@
+// ref_harness.cpp -- explicit-input driver around the reference's own CPU path.
+//
+// TEST INFRASTRUCTURE ONLY.  Compiled (oracle/Makefile, target "ref") together
+// with the reference sources FHE_ACC_runtime/src/{nbt,params,RNG,LWE,RLWE,app}.cpp
+// where they lie, into oracle/_ref/libfheref.so.  It sets the reference's
+// global parameter block (params.h:13-134) the way the apps do
+// (apps/fhe_wo_fpga.cpp:35-162), fills the reference's key and ciphertext
+// objects from flat arrays, and calls the reference functions unchanged.  Key
+// generation never goes through the reference (its RNG is seeded from
+// std::random_device, RNG.cpp:5-6); keys come from the caller.
+//
+// Flat layouts match oracle/fhe_oracle.h.
+#include "LWE.h"
+#include "RLWE.h"
+#include "app.h"
+#include "nbt.h"
+#include "params.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace {
+
+void fill_rlwe(RLWE::RLWE_ciphertext &c, const uint64_t *src) {
+    c.a.assign(src, src + N);
+    c.b.assign(src + N, src + 2 * N);
+    c.NTT_form = true;
+}
+
+void dump_rlwe(const RLWE::RLWE_ciphertext &c, uint64_t *dst) {
+    std::memcpy(dst, c.a.data(), sizeof(uint64_t) * N);
+    std::memcpy(dst + N, c.b.data(), sizeof(uint64_t) * N);
+}
+
+RLWE::RGSW_ciphertext make_rgsw(const uint64_t *src) {
+    std::vector<std::vector<RLWE::RLWE_ciphertext>> rows(digitG);
+    for (uint32_t h = 0; h < digitG; h++) {
+        rows[h].resize(2);
+        for (int g = 0; g < 2; g++) fill_rlwe(rows[h][g], src + ((size_t)h * 2 + g) * 2 * N);
+    }
+    return RLWE::RGSW_ciphertext(rows);
+}
+
+void fill_lwe(LWE::LWE_ciphertext &c, const uint64_t *src) {
+    c.a.assign(src, src + n);
+    c.b = src[n];
+}
+
+void dump_lwe(const LWE::LWE_ciphertext &c, uint64_t *dst) {
+    std::memcpy(dst, c.a.data(), sizeof(uint64_t) * n);
+    dst[n] = c.b;
+}
+
+std::unique_ptr<RLWE::bootstrap_key> g_bsk;
+std::unique_ptr<LWE::keyswitch_key> g_ksk;
+
+}  // namespace
+
+extern "C" {
+
+// Parameter block, as apps/fhe_wo_fpga.cpp:35-162 assigns it.
+int ref_set_params(int n_, uint64_t q_, int N_, uint64_t Q_, uint32_t BGbits_, uint64_t Br_,
+                   uint64_t Bks_, uint32_t Bks_R_bits_) {
+    n = n_;
+    t = 4;
+    q = q_;
+    q2 = q / 2;
+    E = q / 16;
+    N = N_;
+    N2 = N / 2 + 1;
+    digitN = (uint16_t)std::log2((double)N);
+    E_rlwe = 18;
+    Q = Q_;
+    iN = inverse_mod(N, Q);
+    ROU = root_of_unity(N, Q);
+    free(ROU_table);
+    free(iROU_table);
+    ROU_table = (ciphertext_t *)malloc(N * sizeof(ciphertext_t));
+    iROU_table = (ciphertext_t *)malloc(N * sizeof(ciphertext_t));
+    root_of_unity_table(ROU, Q, N, ROU_table, iROU_table);
+    BG = 1ULL << BGbits_;
+    BGbits = (ciphertext_t)std::log2(BG);
+    BG_mask = BG - 1;
+    digitG = (ciphertext_t)std::ceil(std::log2((double)Q) / std::log2((double)BG));
+    digitG2 = 2 * digitG;
+    Br = Br_;
+    digitR = (ciphertext_t)std::ceil(std::log2((double)q) / std::log2((double)Br));
+    Bks = Bks_;
+    digitKS = (ciphertext_t)std::ceil(std::log2((double)Q) / std::log2((double)Bks));
+    Bks_R = 1ULL << Bks_R_bits_;
+    Bks_R_bits = (ciphertext_t)std::log2(Bks_R);
+    digitKS_R = (ciphertext_t)std::ceil(std::log2((double)Q) / std::log2((double)Bks_R));
+    Bks_R_mask = Bks_R - 1;
+    gate_constant[0] = 5 * (q >> 3);
+    gate_constant[1] = 7 * (q >> 3);
+    gate_constant[2] = 1 * (q >> 3);
+    gate_constant[3] = 3 * (q >> 3);
+    gate_constant[4] = 5 * (q >> 3);
+    gate_constant[5] = 1 * (q >> 3);
+    g_bsk.reset();
+    g_ksk.reset();
+    return 0;
+}
+
+void ref_get_params(uint64_t *out) {
+    // ROU, iN, digitG, digitR, digitKS, digitKS_R
+    out[0] = ROU;
+    out[1] = iN;
+    out[2] = digitG;
+    out[3] = digitR;
+    out[4] = digitKS;
+    out[5] = digitKS_R;
+}
+
+void ref_ntt(uint64_t *a, int64_t count) {
+    for (int64_t c = 0; c < count; c++) NTT(a + c * N, N, Q, ROU_table);
+}
+
+void ref_intt(uint64_t *a, int64_t count) {
+    for (int64_t c = 0; c < count; c++) iNTT(a + c * N, N, Q, iROU_table, iN);
+}
+
+void ref_mult_rgsw(const uint64_t *acc, const uint64_t *rgsw, uint64_t *out) {
+    RLWE::RLWE_ciphertext a;
+    fill_rlwe(a, acc);
+    RLWE::RLWE_ciphertext r = RLWE::RLWE_mult_RGSW(a, make_rgsw(rgsw));
+    dump_rlwe(r, out);
+}
+
+void ref_acc_init(uint64_t bound1, uint64_t bound2, uint64_t value, uint64_t *out) {
+    RLWE::RLWE_ciphertext acc = RLWE::acc_initialization(2 * N / q, bound1, bound2, value);
+    dump_rlwe(acc, out);
+}
+
+void ref_lwe_evaluate(const uint64_t *c1, const uint64_t *c2, int gate, uint64_t *out) {
+    LWE::LWE_ciphertext a, b;
+    fill_lwe(a, c1);
+    fill_lwe(b, c2);
+    dump_lwe(LWE::LWE_evaluate(a, b, (GATES)gate), out);
+}
+
+void ref_modswitch(const uint64_t *in, uint64_t *out) {
+    LWE::LWE_ciphertext a;
+    fill_lwe(a, in);
+    dump_lwe(LWE::LWE_modswitch(a), out);
+}
+
+uint64_t ref_scale_round(uint64_t x, uint64_t new_mod, uint64_t old_mod) {
+    return LWE::scale_round(x, new_mod, old_mod);
+}
+
+// Keys: flat bootstrap key [Br][digitR][n] RGSW and key-switch key
+// [Bks][digitKS][N] LWE(n).  Kept resident between calls.
+void ref_load_keys(const uint64_t *bsk, const uint64_t *ksk) {
+    size_t rw = (size_t)digitG * 4 * N;
+    g_bsk.reset(new RLWE::bootstrap_key());
+    g_bsk->btkey.resize(Br);
+    for (uint64_t r = 0; r < Br; r++) {
+        g_bsk->btkey[r].resize(digitR);
+        for (uint64_t j = 0; j < digitR; j++) {
+            g_bsk->btkey[r][j].reserve(n);
+            for (int k = 0; k < n; k++)
+                g_bsk->btkey[r][j].push_back(make_rgsw(bsk + ((r * digitR + j) * n + k) * rw));
+        }
+    }
+    g_ksk.reset(new LWE::keyswitch_key());
+    g_ksk->kskey.resize(Bks);
+    size_t row = (size_t)n + 1;
+    for (uint64_t d = 0; d < Bks; d++) {
+        g_ksk->kskey[d].resize(digitKS);
+        for (uint64_t j = 0; j < digitKS; j++) {
+            g_ksk->kskey[d][j].resize(N);
+            for (int i = 0; i < N; i++) fill_lwe(g_ksk->kskey[d][j][i], ksk + ((d * digitKS + j) * N + i) * row);
+        }
+    }
+}
+
+void ref_lwe_keyswitch(const uint64_t *acc, uint64_t *out) {
+    RLWE::RLWE_ciphertext a;
+    fill_rlwe(a, acc);
+    dump_lwe(LWE::LWE_keyswitch(a, *g_ksk), out);
+}
+
+// RLWE::eval_bootstrap on each pair (RLWE.cpp:726-786).  Returns the wall time
+// in seconds.  Equal operands make the reference exit the process, so the
+// caller must not pass them.
+double ref_eval_bootstrap(const uint64_t *c1, const uint64_t *c2, const int32_t *gates, int64_t count,
+                          uint64_t *out) {
+    size_t row = (size_t)n + 1;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int64_t c = 0; c < count; c++) {
+        LWE::LWE_ciphertext a, b;
+        fill_lwe(a, c1 + c * row);
+        fill_lwe(b, c2 + c * row);
+        LWE::LWE_ciphertext r = RLWE::eval_bootstrap(a, b, *g_bsk, *g_ksk, (GATES)gates[c]);
+        dump_lwe(r, out + c * row);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+void ref_substitute(const uint64_t *in, int f, uint64_t *out) {
+    RLWE::RLWE_ciphertext a;
+    fill_rlwe(a, in);
+    dump_rlwe(RLWE::RLWE_substitute(a, f), out);
+}
+
+void ref_rlwe_keyswitch(const uint64_t *in, const uint64_t *ks, uint64_t *out) {
+    RLWE::RLWE_ciphertext a;
+    fill_rlwe(a, in);
+    RLWE::keyswitch_key k;
+    k.RLWE_kskey.resize(digitKS_R);
+    for (uint64_t l = 0; l < digitKS_R; l++) fill_rlwe(k.RLWE_kskey[l], ks + l * 2 * N);
+    dump_rlwe(RLWE::RLWE_keyswitch(a, k), out);
+}
+
+uint64_t ref_root_of_unity(int order, uint64_t mod) { return root_of_unity(order, mod); }
+uint64_t ref_first_prime(int nbit, int order) { return first_prime(nbit, order); }
+uint64_t ref_previous_prime(uint64_t p, int order) { return previous_prime(p, order); }
+
+}  // extern "C"
