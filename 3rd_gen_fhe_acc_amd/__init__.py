@@ -1,0 +1,318 @@
+"""MI355X gate-bootstrapping library: Python binding of libfheacc.so.
+
+The compute path is the HIP library (lib/libfheacc.so, built by build_lib.py);
+this module is a thin ctypes layer over its C ABI (include/fheacc.h).  There is
+no CPU fallback: if the library is missing, loading fails with an error.
+Device buffers are torch tensors (torch is used for device memory and streams
+only).
+
+Names follow the reference host API (FHE_ACC_runtime/include/{LWE,RLWE}.h):
+gates OR, AND, NOR, NAND, XOR, XNOR (params.h:107); LWE ciphertexts are
+[n+1] words (a then b); RLWE ciphertexts [2][N] (a then b, NTT domain).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+try:
+    # torch bundles its own HIP runtime; loading it before libfheacc.so makes
+    # both resolve to that one runtime in this process.
+    import torch  # noqa: F401
+except ImportError:  # the C ABI works without torch (host-pointer calls)
+    torch = None
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfheacc.so")
+
+GATES = {"OR": 0, "AND": 1, "NOR": 2, "NAND": 3, "XOR": 4, "XNOR": 5}
+
+
+class FheAccError(RuntimeError):
+    pass
+
+
+class _Params(C.Structure):
+    _fields_ = [("n", C.c_int32), ("q", C.c_uint64), ("N", C.c_int32), ("Q", C.c_uint64),
+                ("nbit", C.c_int32), ("BGbits", C.c_uint32), ("Br", C.c_uint64), ("Bks", C.c_uint64),
+                ("Bks_R_bits", C.c_uint32), ("stddev", C.c_double), ("E_rlwe", C.c_uint64)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("Q", C.c_uint64), ("iN", C.c_uint64), ("ROU", C.c_uint64), ("digitG", C.c_uint32),
+                ("digitR", C.c_uint32), ("digitKS", C.c_uint32), ("digitKS_R", C.c_uint32),
+                ("narrow", C.c_uint32), ("gate_constant", C.c_uint64 * 6), ("bsk_words", C.c_uint64),
+                ("ksk_words", C.c_uint64), ("subs_words", C.c_uint64), ("bsk_device_bytes", C.c_uint64),
+                ("ksk_device_bytes", C.c_uint64), ("subs_device_bytes", C.c_uint64)]
+
+
+class _Capture(C.Structure):
+    _fields_ = [("acc", C.c_void_p), ("ks", C.c_void_p)]
+
+
+EXPORTS = {
+    "fheacc_ctx_create": [C.POINTER(_Params), C.c_int, C.POINTER(C.c_void_p)],
+    "fheacc_ctx_destroy": [C.c_void_p],
+    "fheacc_ctx_info": [C.c_void_p, C.POINTER(_Info)],
+    "fheacc_ctx_set_stream": [C.c_void_p, C.c_void_p],
+    "fheacc_sync": [C.c_void_p],
+    "fheacc_last_error": [],
+    "fheacc_ctx_profile": [C.c_void_p, C.c_int],
+    "fheacc_ctx_kernel_ms": [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
+    "fheacc_ctx_set_tuning": [C.c_void_p, C.c_int, C.c_int],
+    "fheacc_ctx_get_tuning": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "fheacc_autotune": [C.c_void_p, C.c_void_p, C.c_int64, C.c_char_p, C.c_size_t],
+    "fheacc_keys_create": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "fheacc_keys_destroy": [C.c_void_p],
+    "fheacc_keys_upload_bsk": [C.c_void_p, C.c_void_p],
+    "fheacc_keys_upload_ksk": [C.c_void_p, C.c_void_p],
+    "fheacc_keys_upload_subs": [C.c_void_p, C.c_void_p],
+    "fheacc_keys_download_bsk": [C.c_void_p, C.c_void_p],
+    "fheacc_keys_download_ksk": [C.c_void_p, C.c_void_p],
+    "fheacc_keys_download_subs": [C.c_void_p, C.c_void_p],
+    "fheacc_keys_generate": [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p],
+    "fheacc_keys_generate_subs": [C.c_void_p, C.c_uint64, C.c_void_p],
+    "fheacc_keys_digest": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "fheacc_lwe_secret": [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p],
+    "fheacc_rlwe_secret": [C.c_void_p, C.c_uint64, C.c_void_p],
+    "fheacc_lwe_encrypt": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.c_void_p],
+    "fheacc_lwe_decrypt": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
+    "fheacc_bootstrap_batch": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_int64, C.POINTER(_Capture)],
+    "fheacc_bootstrap_batch_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_int64, C.POINTER(_Capture)],
+    "fheacc_ntt_dev": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int],
+    "fheacc_external_product_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64],
+    "fheacc_subs_keyswitch_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64],
+}
+
+_lib = None
+
+
+def lib():
+    """Load libfheacc.so; raise if it was not built (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FheAccError("libfheacc.so not built: run __graft_entry__.build() or "
+                              "python 3rd_gen_fhe_acc_amd/build_lib.py")
+        L = C.CDLL(LIB_PATH)
+        for name, args in EXPORTS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_char_p if name == "fheacc_last_error" else C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise FheAccError("fheacc error %d: %s" % (rc, lib().fheacc_last_error().decode()))
+
+
+def _np(a):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a, a.ctypes.data_as(C.c_void_p)
+
+
+def _dptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Context:
+    """One parameter set on one GPU (the reference's global parameter block)."""
+
+    def __init__(self, n, q, N, Q=0, nbit=27, BGbits=7, Br=32, Bks=32, Bks_R_bits=9, stddev=3.19,
+                 E_rlwe=18, device=0, use_torch_stream=True):
+        self._p = _Params(n, q, N, Q, nbit, BGbits, Br, Bks, Bks_R_bits, stddev, E_rlwe)
+        h = C.c_void_p()
+        _check(lib().fheacc_ctx_create(C.byref(self._p), device, C.byref(h)))
+        self.h = h
+        self.device = device
+        info = _Info()
+        _check(lib().fheacc_ctx_info(self.h, C.byref(info)))
+        self.info = {f: getattr(info, f) for f, _ in _Info._fields_ if f != "gate_constant"}
+        self.info["gate_constant"] = list(info.gate_constant)
+        self.n, self.q, self.N, self.Q = n, q, N, info.Q
+        self.digitG, self.digitR, self.digitKS, self.digitKS_R = info.digitG, info.digitR, info.digitKS, info.digitKS_R
+        self.narrow = bool(info.narrow)
+        if use_torch_stream:
+            import torch
+            torch.cuda.set_device(device)
+            self.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fheacc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream):
+        _check(lib().fheacc_ctx_set_stream(self.h, C.c_void_p(stream) if stream else None))
+
+    def sync(self):
+        _check(lib().fheacc_sync(self.h))
+
+    KERNEL_CLASSES = ("blind_rotate", "lwe_keyswitch", "external_product", "ntt")
+
+    def profile(self, enable=True):
+        """Start (and clear) or stop per-kernel HIP-event timing."""
+        _check(lib().fheacc_ctx_profile(self.h, 1 if enable else 0))
+
+    def kernel_ms(self):
+        """{kernel class: (total ms, launches)} since profiling started."""
+        ms = (C.c_double * 4)()
+        n = (C.c_int64 * 4)()
+        _check(lib().fheacc_ctx_kernel_ms(self.h, ms, n))
+        return {k: (ms[i], n[i]) for i, k in enumerate(self.KERNEL_CLASSES)}
+
+    def set_tuning(self, threads, polys_per_batch=0):
+        _check(lib().fheacc_ctx_set_tuning(self.h, threads, polys_per_batch))
+
+    def get_tuning(self):
+        t, p = C.c_int(), C.c_int()
+        _check(lib().fheacc_ctx_get_tuning(self.h, C.byref(t), C.byref(p)))
+        return t.value, p.value
+
+    def autotune(self, keys, batch):
+        """Time every blind-rotation launch shape on `batch` synthetic
+        ciphertexts, keep the fastest; returns the tuner's report."""
+        import json
+
+        buf = C.create_string_buffer(16384)
+        _check(lib().fheacc_autotune(self.h, keys.h, batch, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    # host API -------------------------------------------------------------
+    def lwe_secret(self, seed, binary=True):
+        out = np.zeros(self.n, dtype=np.uint64)
+        _check(lib().fheacc_lwe_secret(self.h, seed, 1 if binary else 0, _np(out)[1]))
+        return out
+
+    def rlwe_secret(self, seed):
+        out = np.zeros(self.N, dtype=np.uint64)
+        _check(lib().fheacc_rlwe_secret(self.h, seed, _np(out)[1]))
+        return out
+
+    def lwe_encrypt(self, sk, bits, seed, first_index=0):
+        bits = np.ascontiguousarray(np.asarray(bits, dtype=np.uint8).reshape(-1))
+        out = np.zeros((bits.size, self.n + 1), dtype=np.uint64)
+        sk, psk = _np(sk)
+        _check(lib().fheacc_lwe_encrypt(self.h, psk, bits.ctypes.data_as(C.c_void_p), bits.size, seed,
+                                        first_index, _np(out)[1]))
+        return out
+
+    def lwe_decrypt(self, sk, c):
+        c = np.ascontiguousarray(c, dtype=np.uint64).reshape(-1, self.n + 1)
+        out = np.zeros(c.shape[0], dtype=np.uint8)
+        sk, psk = _np(sk)
+        _check(lib().fheacc_lwe_decrypt(self.h, psk, _np(c)[1], c.shape[0], out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def keys(self):
+        return Keys(self)
+
+    # gate path ------------------------------------------------------------
+    def bootstrap(self, keys, gates, c1, c2, capture=False):
+        """eval_bootstrap over host arrays; returns out (and acc, ks captures)."""
+        c1 = np.ascontiguousarray(c1, dtype=np.uint64).reshape(-1, self.n + 1)
+        c2 = np.ascontiguousarray(c2, dtype=np.uint64).reshape(-1, self.n + 1)
+        B = c1.shape[0]
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(gates, dtype=np.int32), (B,)))
+        out = np.zeros((B, self.n + 1), dtype=np.uint64)
+        cap = None
+        acc = ks = None
+        if capture:
+            acc = np.zeros((B, 2, self.N), dtype=np.uint64)
+            ks = np.zeros((B, self.n + 1), dtype=np.uint64)
+            cap = _Capture(acc.ctypes.data, ks.ctypes.data)
+        _check(lib().fheacc_bootstrap_batch(self.h, keys.h, g.ctypes.data_as(C.c_void_p), _np(c1)[1],
+                                            _np(c2)[1], _np(out)[1], B, C.byref(cap) if cap else None))
+        return (out, acc, ks) if capture else out
+
+    def bootstrap_dev(self, keys, gates, c1, c2, out, cap_acc=None, cap_ks=None):
+        """Asynchronous on the context stream; arguments are device tensors."""
+        B = c1.shape[0]
+        cap = _Capture(cap_acc.data_ptr() if cap_acc is not None else None,
+                       cap_ks.data_ptr() if cap_ks is not None else None)
+        _check(lib().fheacc_bootstrap_batch_dev(self.h, keys.h, _dptr(gates), _dptr(c1), _dptr(c2), _dptr(out),
+                                                B, C.byref(cap)))
+
+    def ntt_dev(self, polys, inverse=False):
+        _check(lib().fheacc_ntt_dev(self.h, _dptr(polys), polys.numel() // self.N, 1 if inverse else 0))
+
+    def external_product_dev(self, acc, rgsw, out, per_ciphertext_key=False):
+        count = acc.numel() // (2 * self.N)
+        stride = self.digitG * 4 * self.N if per_ciphertext_key else 0
+        _check(lib().fheacc_external_product_dev(self.h, _dptr(acc), _dptr(rgsw), stride, _dptr(out), count))
+
+    def subs_keyswitch_dev(self, keys, rlwe, sub_factor, out):
+        _check(lib().fheacc_subs_keyswitch_dev(self.h, keys.h, _dptr(rlwe), sub_factor, _dptr(out),
+                                               rlwe.numel() // (2 * self.N)))
+
+
+class Keys:
+    """A client's device-resident keys (bootstrap, LWE key switch, subs)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().fheacc_keys_create(ctx.h, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fheacc_keys_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_bsk(self, bsk):
+        bsk, p = _np(bsk)
+        _check(lib().fheacc_keys_upload_bsk(self.h, p))
+
+    def upload_ksk(self, ksk):
+        ksk, p = _np(ksk)
+        _check(lib().fheacc_keys_upload_ksk(self.h, p))
+
+    def upload_subs(self, subs):
+        subs, p = _np(subs)
+        _check(lib().fheacc_keys_upload_subs(self.h, p))
+
+    def download_bsk(self):
+        out = np.zeros(self.ctx.info["bsk_words"], dtype=np.uint64)
+        _check(lib().fheacc_keys_download_bsk(self.h, _np(out)[1]))
+        return out
+
+    def download_ksk(self):
+        out = np.zeros(self.ctx.info["ksk_words"], dtype=np.uint64)
+        _check(lib().fheacc_keys_download_ksk(self.h, _np(out)[1]))
+        return out
+
+    def download_subs(self):
+        out = np.zeros(self.ctx.info["subs_words"], dtype=np.uint64)
+        _check(lib().fheacc_keys_download_subs(self.h, _np(out)[1]))
+        return out
+
+    def generate(self, seed, lwe_sk, rlwe_sk):
+        lwe_sk, p1 = _np(lwe_sk)
+        rlwe_sk, p2 = _np(rlwe_sk)
+        _check(lib().fheacc_keys_generate(self.h, seed, p1, p2))
+
+    def generate_subs(self, seed, rlwe_sk):
+        rlwe_sk, p = _np(rlwe_sk)
+        _check(lib().fheacc_keys_generate_subs(self.h, seed, p))
+
+    def digest(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        _check(lib().fheacc_keys_digest(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value

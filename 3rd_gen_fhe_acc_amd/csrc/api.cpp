@@ -1,0 +1,968 @@
+// api.cpp -- the C ABI (include/fheacc.h) over the HIP kernels.
+#include <errno.h>
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "fheacc_internal.hpp"
+#include "kernels.hpp"
+
+using namespace fheacc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &m, int rc = FHEACC_ERR) {
+    g_err = m;
+    return rc;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            return fail(std::string(#expr) + ": " + hipGetErrorString(e_),                 \
+                        e_ == hipErrorOutOfMemory ? FHEACC_ENOMEM : FHEACC_ERR);            \
+        }                                                                                   \
+    } while (0)
+
+typedef unsigned __int128 u128;
+
+uint64_t shoup64h(uint64_t w, uint64_t Q) { return (uint64_t)(((u128)w << 64) / Q); }
+uint32_t shoup32h(uint32_t w, uint32_t Q) { return (uint32_t)(((uint64_t)w << 32) / Q); }
+uint64_t mulmodh(uint64_t a, uint64_t b, uint64_t Q) { return (uint64_t)((u128)a * b % Q); }
+
+ShoupGen make_sg(uint64_t Q) {
+    ShoupGen s;
+    s.Q = Q;
+    s.a = (uint64_t)(((u128)1 << 64) / Q);
+    s.b = (uint64_t)(((u128)1 << 64) % Q);
+    s.bp = shoup64h(s.b, Q);
+    return s;
+}
+
+template <typename W>
+Arith<W> make_arith(const HostParams &p) {
+    Arith<W> A;
+    A.Q = (W)p.Q;
+    A.Q2 = (W)(2 * p.Q);
+    A.mu = (uint64_t)(((u128)1 << 64) / p.Q);
+    return A;
+}
+
+template <typename W>
+Twiddles<W> make_tw(const fheacc_ctx *ctx) {
+    Twiddles<W> t;
+    t.fw = (const W *)ctx->t.fw;
+    t.fwp = (const W *)ctx->t.fwp;
+    t.iw = (const W *)ctx->t.iw;
+    t.iwp = (const W *)ctx->t.iwp;
+    t.iN = (W)ctx->t.iN;
+    t.iNp = (W)ctx->t.iNp;
+    t.onep = sizeof(W) == 4 ? (W)shoup32h(1, (uint32_t)ctx->p.Q) : (W)shoup64h(1, ctx->p.Q);
+    return t;
+}
+
+size_t wsize(const fheacc_ctx *c) { return c->p.narrow ? 4 : 8; }
+
+// W elements of one device RGSW: narrow d*N*4, wide d*N*8
+size_t rgsw_elems(const HostParams &p) { return (size_t)p.digitG * p.N * (p.narrow ? 4 : 8); }
+size_t rgsw_bytes(const HostParams &p) { return rgsw_elems(p) * (p.narrow ? 4 : 8); }
+int n_pad(const HostParams &p) { return (p.n + 1 + 7) / 8 * 8; }
+size_t bsk_dev_bytes(const HostParams &p) { return (size_t)p.n * p.digitR * (p.Br - 1) * rgsw_bytes(p); }
+size_t ksk_dev_bytes(const HostParams &p) {
+    return (size_t)p.N * p.digitKS * p.Bks * n_pad(p) * (p.narrow ? 4 : 8);
+}
+size_t subs_elems(const HostParams &p) { return (size_t)p.logN * p.digitKS_R * p.N * (p.narrow ? 2 : 4); }
+size_t subs_dev_bytes(const HostParams &p) { return subs_elems(p) * (p.narrow ? 4 : 8); }
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+int ensure_scratch(fheacc_ctx *ctx, size_t bytes) {
+    if (ctx->scratch_bytes >= bytes) return 0;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->scratch) HIP_TRY(hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    size_t b = std::max(bytes, (size_t)1 << 20);
+    HIP_TRY(hipMalloc(&ctx->scratch, b));
+    ctx->scratch_bytes = b;
+    return 0;
+}
+
+// Digit polys per NTT batch: up to `total`, as many as fit in `budget` bytes
+// of LDS next to `fixed_bytes`.
+int pick_pb(const HostParams &p, int total, size_t fixed_bytes, size_t budget, int requested) {
+    if (requested > 0) return std::min(requested, total);
+    size_t per = (size_t)p.N * (p.narrow ? 4 : 8);
+    int pb = total;
+    while (pb > 1 && fixed_bytes + per * pb > budget) pb--;
+    return pb;
+}
+
+int check_ctx(const fheacc_ctx *c) { return c ? 0 : fail("null context"); }
+
+// Records a (start, stop) event pair around one kernel launch when profiling.
+struct Prof {
+    fheacc_ctx *c;
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    Prof(fheacc_ctx *c_, int cls_) : c(c_), cls(cls_) {
+        if (!c->profile) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess ||
+            hipEventRecord(a, c->stream) != hipSuccess)
+            a = nullptr;
+    }
+    ~Prof() {
+        if (a && hipEventRecord(b, c->stream) == hipSuccess) c->events.push_back({cls, a, b});
+    }
+};
+
+void drop_events(fheacc_ctx *c) {
+    for (auto &e : c->events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    c->events.clear();
+}
+
+}  // namespace
+
+int fheacc_ctx_profile(fheacc_ctx *c, int enable) {
+    if (check_ctx(c)) return -1;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    drop_events(c);
+    c->profile = enable != 0;
+    return 0;
+}
+
+int fheacc_ctx_kernel_ms(fheacc_ctx *c, double *ms, int64_t *n) {
+    if (check_ctx(c) || !ms || !n) return fail("null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 4; i++) {
+        ms[i] = 0;
+        n[i] = 0;
+    }
+    for (auto &e : c->events) {
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, e.a, e.b));
+        ms[e.cls] += t;
+        n[e.cls] += 1;
+    }
+    drop_events(c);
+    return 0;
+}
+
+const char *fheacc_last_error(void) { return g_err.c_str(); }
+
+int fheacc_ctx_create(const fheacc_params *params, int device, fheacc_ctx **out) {
+    if (!out) return fail("null output handle");
+    *out = nullptr;
+    fheacc_ctx *c = new (std::nothrow) fheacc_ctx();
+    if (!c) return fail("out of host memory", FHEACC_ENOMEM);
+    std::string err;
+    if (derive_params(params, &c->p, &err)) {
+        delete c;
+        return fail(err);
+    }
+    c->device = device;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+        delete c;
+        return fail("no HIP device " + std::to_string(device));
+    }
+    auto cleanup = [&](int rc) {
+        fheacc_ctx_destroy(c);
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) return cleanup(fail("hipSetDevice failed"));
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+        lds_max > 0)
+        c->lds_budget = std::min<size_t>((size_t)lds_max, 160 * 1024);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(fail("stream"));
+    c->own_stream = true;
+    const HostParams &p = c->p;
+    const int N = p.N;
+    const size_t ws = p.narrow ? 4 : 8;
+    std::vector<uint64_t> fw(N), fwp(N), iw(N), iwp(N);
+    for (int i = 0; i < N; i++) {
+        fw[i] = p.rou[i];
+        iw[i] = p.irou[i];
+        fwp[i] = p.narrow ? shoup32h((uint32_t)fw[i], (uint32_t)p.Q) : shoup64h(fw[i], p.Q);
+        iwp[i] = p.narrow ? shoup32h((uint32_t)iw[i], (uint32_t)p.Q) : shoup64h(iw[i], p.Q);
+    }
+    void **dst[4] = {&c->t.fw, &c->t.fwp, &c->t.iw, &c->t.iwp};
+    std::vector<uint64_t> *srcs[4] = {&fw, &fwp, &iw, &iwp};
+    for (int t = 0; t < 4; t++) {
+        if (hipMalloc(dst[t], N * ws) != hipSuccess) return cleanup(fail("table alloc", FHEACC_ENOMEM));
+        std::vector<uint32_t> narrow;
+        const void *h = srcs[t]->data();
+        if (p.narrow) {
+            narrow.assign(srcs[t]->begin(), srcs[t]->end());
+            h = narrow.data();
+        }
+        if (hipMemcpy(*dst[t], h, N * ws, hipMemcpyHostToDevice) != hipSuccess) return cleanup(fail("table copy"));
+    }
+    c->t.iN = p.iN;
+    c->t.iNp = p.narrow ? shoup32h((uint32_t)p.iN, (uint32_t)p.Q) : shoup64h(p.iN, p.Q);
+    c->t.cdt_len = (int)p.cdt.size();
+    if (hipMalloc(&c->t.cdt, p.cdt.size() * 8) != hipSuccess) return cleanup(fail("cdt alloc", FHEACC_ENOMEM));
+    if (hipMemcpy(c->t.cdt, p.cdt.data(), p.cdt.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail("cdt copy"));
+    if (hipMalloc(&c->status, sizeof(int)) != hipSuccess) return cleanup(fail("status alloc", FHEACC_ENOMEM));
+    if (hipMemset(c->status, 0, sizeof(int)) != hipSuccess) return cleanup(fail("status clear"));
+    *out = c;
+    return 0;
+}
+
+int fheacc_ctx_destroy(fheacc_ctx *c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : {c->t.fw, c->t.fwp, c->t.iw, c->t.iwp, (void *)c->t.cdt, c->scratch, (void *)c->status})
+        if (p) (void)hipFree(p);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int fheacc_ctx_info(const fheacc_ctx *c, fheacc_info *o) {
+    if (check_ctx(c) || !o) return fail("null argument");
+    const HostParams &p = c->p;
+    o->Q = p.Q;
+    o->iN = p.iN;
+    o->ROU = p.ROU;
+    o->digitG = p.digitG;
+    o->digitR = p.digitR;
+    o->digitKS = p.digitKS;
+    o->digitKS_R = p.digitKS_R;
+    o->narrow = p.narrow;
+    for (int g = 0; g < 6; g++) o->gate_constant[g] = p.gate_constant[g];
+    o->bsk_words = (uint64_t)p.Br * p.digitR * p.n * p.digitG * 4 * p.N;
+    o->ksk_words = (uint64_t)p.Bks * p.digitKS * p.N * (p.n + 1);
+    o->subs_words = (uint64_t)p.logN * p.digitKS_R * 2 * p.N;
+    o->bsk_device_bytes = bsk_dev_bytes(p);
+    o->ksk_device_bytes = ksk_dev_bytes(p);
+    o->subs_device_bytes = subs_dev_bytes(p);
+    return 0;
+}
+
+int fheacc_ctx_set_stream(fheacc_ctx *c, void *s) {
+    if (check_ctx(c)) return -1;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->own_stream) HIP_TRY(hipStreamDestroy(c->stream));
+    c->stream = (hipStream_t)s;  // NULL: the device's null stream
+    c->own_stream = false;
+    return 0;
+}
+
+int fheacc_sync(fheacc_ctx *c) {
+    if (check_ctx(c)) return -1;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    int st = 0;
+    HIP_TRY(hipMemcpy(&st, c->status, sizeof(int), hipMemcpyDeviceToHost));
+    if (st) {
+        HIP_TRY(hipMemset(c->status, 0, sizeof(int)));
+        return fail("identical operands passed to the gate bootstrap (RLWE.cpp:727-730)");
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- keys
+
+int fheacc_keys_create(fheacc_ctx *c, fheacc_keys **out) {
+    if (check_ctx(c) || !out) return fail("null argument");
+    fheacc_keys *k = new (std::nothrow) fheacc_keys();
+    if (!k) return fail("out of host memory", FHEACC_ENOMEM);
+    k->ctx = c;
+    *out = k;
+    return 0;
+}
+
+int fheacc_keys_destroy(fheacc_keys *k) {
+    if (!k) return 0;
+    (void)hipSetDevice(k->ctx->device);
+    (void)hipStreamSynchronize(k->ctx->stream);
+    for (void *p : {k->bsk, k->ksk, k->subs})
+        if (p) (void)hipFree(p);
+    delete k;
+    return 0;
+}
+
+static int alloc_key(void **p, size_t bytes) {
+    if (*p) return 0;
+    HIP_TRY(hipMalloc(p, bytes));
+    return 0;
+}
+
+// map[device RGSW e] = reference RGSW index (residue 0 not stored)
+static std::vector<int64_t> bsk_map(const HostParams &p) {
+    std::vector<int64_t> m((size_t)p.n * p.digitR * (p.Br - 1));
+    size_t e = 0;
+    for (int k = 0; k < p.n; k++)
+        for (uint32_t j = 0; j < p.digitR; j++)
+            for (uint64_t r = 1; r < p.Br; r++) m[e++] = ((int64_t)r * p.digitR + j) * p.n + k;
+    return m;
+}
+
+int fheacc_keys_upload_bsk(fheacc_keys *k, const uint64_t *bsk) {
+    if (!k || !bsk) return fail("null argument");
+    fheacc_ctx *c = k->ctx;
+    const HostParams &p = c->p;
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = alloc_key(&k->bsk, bsk_dev_bytes(p))) return rc;
+    k->bsk_bytes = bsk_dev_bytes(p);
+    const size_t rw = (size_t)p.digitG * 4 * p.N;
+    const size_t words = (size_t)p.Br * p.digitR * p.n * rw;
+    DevBuf stage, dmap;
+    HIP_TRY(hipMalloc(&stage.p, words * 8));
+    HIP_TRY(hipMemcpy(stage.p, bsk, words * 8, hipMemcpyHostToDevice));
+    std::vector<int64_t> m = bsk_map(p);
+    HIP_TRY(hipMalloc(&dmap.p, m.size() * 8));
+    HIP_TRY(hipMemcpy(dmap.p, m.data(), m.size() * 8, hipMemcpyHostToDevice));
+    hipError_t e = p.narrow
+        ? launch_rgsw_to_device<uint32_t>((const uint64_t *)stage.p, rw, (uint32_t *)k->bsk, rgsw_elems(p),
+                                          p.digitG, p.N, make_sg(p.Q), (const int64_t *)dmap.p, m.size(), c->stream)
+        : launch_rgsw_to_device<uint64_t>((const uint64_t *)stage.p, rw, (uint64_t *)k->bsk, rgsw_elems(p),
+                                          p.digitG, p.N, make_sg(p.Q), (const int64_t *)dmap.p, m.size(), c->stream);
+    HIP_TRY(e);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fheacc_keys_download_bsk(fheacc_keys *k, uint64_t *bsk) {
+    if (!k || !bsk || !k->bsk) return fail("no bootstrap key resident");
+    fheacc_ctx *c = k->ctx;
+    const HostParams &p = c->p;
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t rw = (size_t)p.digitG * 4 * p.N;
+    const size_t words = (size_t)p.Br * p.digitR * p.n * rw;
+    DevBuf stage, dmap;
+    HIP_TRY(hipMalloc(&stage.p, words * 8));
+    HIP_TRY(hipMemsetAsync(stage.p, 0, words * 8, c->stream));
+    std::vector<int64_t> m = bsk_map(p);
+    HIP_TRY(hipMalloc(&dmap.p, m.size() * 8));
+    HIP_TRY(hipMemcpy(dmap.p, m.data(), m.size() * 8, hipMemcpyHostToDevice));
+    hipError_t e = p.narrow
+        ? launch_rgsw_to_ref<uint32_t>((const uint32_t *)k->bsk, rgsw_elems(p), (uint64_t *)stage.p, rw, p.digitG,
+                                       p.N, (const int64_t *)dmap.p, m.size(), c->stream)
+        : launch_rgsw_to_ref<uint64_t>((const uint64_t *)k->bsk, rgsw_elems(p), (uint64_t *)stage.p, rw, p.digitG,
+                                       p.N, (const int64_t *)dmap.p, m.size(), c->stream);
+    HIP_TRY(e);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(bsk, stage.p, words * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fheacc_keys_upload_ksk(fheacc_keys *k, const uint64_t *ksk) {
+    if (!k || !ksk) return fail("null argument");
+    fheacc_ctx *c = k->ctx;
+    const HostParams &p = c->p;
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = alloc_key(&k->ksk, ksk_dev_bytes(p))) return rc;
+    k->ksk_bytes = ksk_dev_bytes(p);
+    const size_t words = (size_t)p.Bks * p.digitKS * p.N * (p.n + 1);
+    DevBuf stage;
+    HIP_TRY(hipMalloc(&stage.p, words * 8));
+    HIP_TRY(hipMemcpy(stage.p, ksk, words * 8, hipMemcpyHostToDevice));
+    hipError_t e = p.narrow
+        ? launch_ksk_convert<uint32_t>((const uint64_t *)stage.p, (uint32_t *)k->ksk, nullptr, (int)p.Bks, p.digitKS,
+                                       p.N, p.n + 1, n_pad(p), c->stream)
+        : launch_ksk_convert<uint64_t>((const uint64_t *)stage.p, (uint64_t *)k->ksk, nullptr, (int)p.Bks, p.digitKS,
+                                       p.N, p.n + 1, n_pad(p), c->stream);
+    HIP_TRY(e);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fheacc_keys_download_ksk(fheacc_keys *k, uint64_t *ksk) {
+    if (!k || !ksk || !k->ksk) return fail("no key-switch key resident");
+    fheacc_ctx *c = k->ctx;
+    const HostParams &p = c->p;
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t words = (size_t)p.Bks * p.digitKS * p.N * (p.n + 1);
+    DevBuf stage;
+    HIP_TRY(hipMalloc(&stage.p, words * 8));
+    hipError_t e = p.narrow
+        ? launch_ksk_convert<uint32_t>(nullptr, (uint32_t *)k->ksk, (uint64_t *)stage.p, (int)p.Bks, p.digitKS, p.N,
+                                       p.n + 1, n_pad(p), c->stream)
+        : launch_ksk_convert<uint64_t>(nullptr, (uint64_t *)k->ksk, (uint64_t *)stage.p, (int)p.Bks, p.digitKS, p.N,
+                                       p.n + 1, n_pad(p), c->stream);
+    HIP_TRY(e);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(ksk, stage.p, words * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fheacc_keys_upload_subs(fheacc_keys *k, const uint64_t *subs) {
+    if (!k || !subs) return fail("null argument");
+    fheacc_ctx *c = k->ctx;
+    const HostParams &p = c->p;
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = alloc_key(&k->subs, subs_dev_bytes(p))) return rc;
+    k->subs_bytes = subs_dev_bytes(p);
+    const int64_t total = (int64_t)p.logN * p.digitKS_R * p.N;
+    DevBuf stage;
+    HIP_TRY(hipMalloc(&stage.p, total * 2 * 8));
+    HIP_TRY(hipMemcpy(stage.p, subs, total * 2 * 8, hipMemcpyHostToDevice));
+    hipError_t e = p.narrow
+        ? launch_subs_convert<uint32_t>(stage.p, k->subs, p.N, total, false, make_sg(p.Q), c->stream)
+        : launch_subs_convert<uint64_t>(stage.p, k->subs, p.N, total, false, make_sg(p.Q), c->stream);
+    HIP_TRY(e);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int subs_download(fheacc_keys *k, uint64_t *subs) {
+    fheacc_ctx *c = k->ctx;
+    const HostParams &p = c->p;
+    const int64_t total = (int64_t)p.logN * p.digitKS_R * p.N;
+    DevBuf stage;
+    HIP_TRY(hipMalloc(&stage.p, total * 2 * 8));
+    hipError_t e = p.narrow
+        ? launch_subs_convert<uint32_t>(k->subs, stage.p, p.N, total, true, make_sg(p.Q), c->stream)
+        : launch_subs_convert<uint64_t>(k->subs, stage.p, p.N, total, true, make_sg(p.Q), c->stream);
+    HIP_TRY(e);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(subs, stage.p, total * 2 * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fheacc_keys_download_subs(fheacc_keys *k, uint64_t *subs) {
+    if (!k || !subs || !k->subs) return fail("no subs key resident");
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    return subs_download(k, subs);
+}
+
+template <typename W>
+static int keygen_impl(fheacc_keys *k, uint64_t seed, const uint64_t *lwe_sk, const uint64_t *rlwe_sk, bool subs) {
+    fheacc_ctx *c = k->ctx;
+    const HostParams &p = c->p;
+    const uint64_t Q = p.Q;
+    const size_t ws = sizeof(W);
+    DevBuf dz, dzn, dznp, dsk, dpow, dpowp, ds, dsp, dcij, dcijp;
+    HIP_TRY(hipMalloc(&dz.p, p.N * 8));
+    HIP_TRY(hipMemcpy(dz.p, rlwe_sk, p.N * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&dzn.p, p.N * ws));
+    HIP_TRY(hipMalloc(&dznp.p, p.N * ws));
+    KeygenArgs<W> a{};
+    a.A = make_arith<W>(p);
+    a.tw = make_tw<W>(c);
+    a.seed = seed;
+    a.cdt = c->t.cdt;
+    a.cdt_len = c->t.cdt_len;
+    a.E_rlwe = p.E_rlwe;
+    a.zn = (W *)dzn.p;
+    a.znp = (W *)dznp.p;
+    a.z = (const uint64_t *)dz.p;
+    a.sg = make_sg(Q);
+    a.q = (uint32_t)p.q;
+    a.Br = (uint32_t)p.Br;
+    a.digitR = p.digitR;
+    KskGenArgs<W> ka{};
+    const int d = subs ? p.digitKS_R : p.digitG;
+    std::vector<uint64_t> pw(d), pwp(d);
+    uint64_t base = subs ? (1ULL << p.Bks_R_bits) % Q : (1ULL << p.BGbits) % Q, cur = 1 % Q;
+    for (int h = 0; h < d; h++) {
+        pw[h] = cur;
+        pwp[h] = p.narrow ? shoup32h((uint32_t)cur, (uint32_t)Q) : shoup64h(cur, Q);
+        cur = mulmodh(cur, base, Q);
+    }
+    HIP_TRY(hipMalloc(&dpow.p, d * 8));
+    HIP_TRY(hipMalloc(&dpowp.p, d * 8));
+    HIP_TRY(hipMemcpy(dpow.p, pw.data(), d * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dpowp.p, pwp.data(), d * 8, hipMemcpyHostToDevice));
+    a.bg_pow = (const uint64_t *)dpow.p;
+    a.bg_powp = (const uint64_t *)dpowp.p;
+    a.d = d;
+    if (subs) {
+        if (int rc = alloc_key(&k->subs, subs_dev_bytes(p))) return rc;
+        k->subs_bytes = subs_dev_bytes(p);
+        a.bsk = (W *)k->subs;
+        a.rgsw_elems = (size_t)p.N * (p.narrow ? 2 : 4);
+        a.n = p.logN;
+    } else {
+        if (int rc = alloc_key(&k->bsk, bsk_dev_bytes(p))) return rc;
+        if (int rc = alloc_key(&k->ksk, ksk_dev_bytes(p))) return rc;
+        k->bsk_bytes = bsk_dev_bytes(p);
+        k->ksk_bytes = ksk_dev_bytes(p);
+        HIP_TRY(hipMalloc(&dsk.p, p.n * 8));
+        HIP_TRY(hipMemcpy(dsk.p, lwe_sk, p.n * 8, hipMemcpyHostToDevice));
+        a.lwe_sk = (const uint64_t *)dsk.p;
+        a.bsk = (W *)k->bsk;
+        a.rgsw_elems = rgsw_elems(p);
+        a.n = p.n;
+        // LWE key-switch key inputs: s mapped into Z_Q (LWE.cpp:62-75)
+        std::vector<W> s(p.n), sp(p.n);
+        for (int h = 0; h < p.n; h++) {
+            uint64_t v = lwe_sk[h] >= p.q / 2 ? lwe_sk[h] + (Q - p.q) : lwe_sk[h];
+            s[h] = (W)v;
+            sp[h] = (W)(p.narrow ? shoup32h((uint32_t)v, (uint32_t)Q) : shoup64h(v, Q));
+        }
+        std::vector<uint64_t> cij((size_t)p.Bks * p.digitKS), cijp(cij.size());
+        for (uint64_t i = 0; i < p.Bks; i++) {
+            uint64_t pk = 1;
+            for (uint32_t j = 0; j < p.digitKS; j++) {
+                uint64_t v = mulmodh(i % Q, pk, Q);
+                cij[i * p.digitKS + j] = v;
+                cijp[i * p.digitKS + j] = p.narrow ? shoup32h((uint32_t)v, (uint32_t)Q) : shoup64h(v, Q);
+                pk = mulmodh(pk, p.Bks, Q);
+            }
+        }
+        HIP_TRY(hipMalloc(&ds.p, p.n * ws));
+        HIP_TRY(hipMalloc(&dsp.p, p.n * ws));
+        HIP_TRY(hipMalloc(&dcij.p, cij.size() * 8));
+        HIP_TRY(hipMalloc(&dcijp.p, cij.size() * 8));
+        HIP_TRY(hipMemcpy(ds.p, s.data(), p.n * ws, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dsp.p, sp.data(), p.n * ws, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dcij.p, cij.data(), cij.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dcijp.p, cijp.data(), cij.size() * 8, hipMemcpyHostToDevice));
+        ka.A = a.A;
+        ka.seed = seed;
+        ka.cdt = a.cdt;
+        ka.cdt_len = a.cdt_len;
+        ka.E_rlwe = p.E_rlwe;
+        ka.ksk = (W *)k->ksk;
+        ka.rows = (int64_t)p.N * p.digitKS * p.Bks;
+        ka.n = p.n;
+        ka.n_pad = n_pad(p);
+        ka.N = p.N;
+        ka.dks = p.digitKS;
+        ka.Bks = (int)p.Bks;
+        ka.s = (const W *)ds.p;
+        ka.sp = (const W *)dsp.p;
+        ka.z = (const uint64_t *)dz.p;
+        ka.cij = (const uint64_t *)dcij.p;
+        ka.cijp = (const uint64_t *)dcijp.p;
+    }
+    HIP_TRY(launch_keygen<W>(a, ka, p.logN, (const uint64_t *)dz.p, c->stream, subs));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int fheacc_keys_generate(fheacc_keys *k, uint64_t seed, const uint64_t *lwe_sk, const uint64_t *rlwe_sk) {
+    if (!k || !lwe_sk || !rlwe_sk) return fail("null argument");
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    return k->ctx->p.narrow ? keygen_impl<uint32_t>(k, seed, lwe_sk, rlwe_sk, false)
+                            : keygen_impl<uint64_t>(k, seed, lwe_sk, rlwe_sk, false);
+}
+
+int fheacc_keys_generate_subs(fheacc_keys *k, uint64_t seed, const uint64_t *rlwe_sk) {
+    if (!k || !rlwe_sk) return fail("null argument");
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    return k->ctx->p.narrow ? keygen_impl<uint32_t>(k, seed, nullptr, rlwe_sk, true)
+                            : keygen_impl<uint64_t>(k, seed, nullptr, rlwe_sk, true);
+}
+
+static int digest_of(fheacc_ctx *c, const void *buf, size_t bytes, uint64_t *out) {
+    if (!buf) {
+        *out = 0;
+        return 0;
+    }
+    const int blocks = 1024;
+    DevBuf part;
+    HIP_TRY(hipMalloc(&part.p, blocks * 8));
+    HIP_TRY(launch_digest((const uint64_t *)buf, (int64_t)(bytes / 8), (uint64_t *)part.p, blocks, c->stream));
+    std::vector<uint64_t> h(blocks);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(h.data(), part.p, blocks * 8, hipMemcpyDeviceToHost));
+    uint64_t s = 0;
+    for (uint64_t v : h) s += v;
+    *out = s;
+    return 0;
+}
+
+int fheacc_keys_digest(fheacc_keys *k, uint64_t *bd, uint64_t *kd) {
+    if (!k) return fail("null argument");
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    uint64_t b = 0, s = 0;
+    if (int rc = digest_of(k->ctx, k->bsk, k->bsk_bytes, &b)) return rc;
+    if (int rc = digest_of(k->ctx, k->ksk, k->ksk_bytes, &s)) return rc;
+    if (bd) *bd = b;
+    if (kd) *kd = s;
+    return 0;
+}
+
+// ---------------------------------------------------------------- host API
+
+int fheacc_lwe_secret(const fheacc_ctx *c, uint64_t seed, int binary, uint64_t *sk) {
+    if (check_ctx(c) || !sk) return fail("null argument");
+    const HostParams &p = c->p;
+    for (int i = 0; i < p.n; i++) {
+        uint64_t w = rng_word(seed, stream_id(TAG_LWE_SK, 0), (uint64_t)i);
+        sk[i] = binary ? (w >> 63) : ((uint64_t)(((u128)w * 3) >> 64) + p.q - 1) % p.q;
+    }
+    return 0;
+}
+
+int fheacc_rlwe_secret(const fheacc_ctx *c, uint64_t seed, uint64_t *sk) {
+    if (check_ctx(c) || !sk) return fail("null argument");
+    const HostParams &p = c->p;
+    for (int i = 0; i < p.N; i++) {
+        uint64_t w = rng_word(seed, stream_id(TAG_RLWE_SK, 0), (uint64_t)i);
+        sk[i] = ((uint64_t)(((u128)w * 3) >> 64) + p.Q - 1) % p.Q;
+    }
+    return 0;
+}
+
+// LWE.cpp:108-146 (t = 4): b = e + m q/4 + <a, s> mod q
+int fheacc_lwe_encrypt(const fheacc_ctx *c, const uint64_t *sk, const uint8_t *bits, int64_t count, uint64_t seed,
+                       uint64_t first_index, uint64_t *out) {
+    if (check_ctx(c) || !sk || !bits || !out) return fail("null argument");
+    const HostParams &p = c->p;
+    const uint64_t q = p.q;
+    for (int64_t i = 0; i < count; i++) {
+        const uint64_t stream = stream_id(TAG_LWE_ENC, first_index + i);
+        uint64_t *o = out + i * (p.n + 1);
+        int64_t e = gauss_sample(p, rng_word(seed, stream, p.n)) % (int64_t)p.E;
+        uint64_t b = (uint64_t)(e + (int64_t)((uint64_t)bits[i] * (q / 4))) % q;
+        for (int h = 0; h < p.n; h++) {
+            o[h] = (uint64_t)(((u128)rng_word(seed, stream, h) * q) >> 64);
+            b = (b + (o[h] * sk[h]) % q) % q;
+        }
+        o[p.n] = b;
+    }
+    return 0;
+}
+
+// LWE.cpp:148-190
+int fheacc_lwe_decrypt(const fheacc_ctx *c, const uint64_t *sk, const uint64_t *ct, int64_t count, uint8_t *bits) {
+    if (check_ctx(c) || !sk || !ct || !bits) return fail("null argument");
+    const HostParams &p = c->p;
+    for (int64_t i = 0; i < count; i++) {
+        const uint64_t *x = ct + i * (p.n + 1);
+        uint64_t m = x[p.n];
+        for (int h = 0; h < p.n; h++) m = (m - x[h] * sk[h]) % p.q;
+        m = (m + p.E * 2) % p.q;
+        bits[i] = (uint8_t)(m * 4 / p.q);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- gate path
+
+template <typename W>
+static int bootstrap_dev_impl(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
+                              const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap) {
+    const HostParams &p = c->p;
+    const size_t ksd_bytes = (size_t)B * p.N * p.digitKS;
+    const size_t ksd_al = (ksd_bytes + 255) / 256 * 256;
+    if (int rc = ensure_scratch(c, ksd_al + (size_t)B * 8)) return rc;
+    uint8_t *ksd = (uint8_t *)c->scratch;
+    uint64_t *ksb = (uint64_t *)((char *)c->scratch + ksd_al);
+    BRArgs<W> a{};
+    a.A = make_arith<W>(p);
+    a.tw = make_tw<W>(c);
+    a.c1 = c1;
+    a.c2 = c2;
+    a.gates = gates;
+    a.bsk = (const W *)k->bsk;
+    a.rgsw_elems = rgsw_elems(p);
+    a.n = p.n;
+    a.q = (uint32_t)p.q;
+    a.digitR = p.digitR;
+    a.Br = (uint32_t)p.Br;
+    a.d = p.digitG;
+    a.bits = p.BGbits;
+    const size_t fixed = 2 * (size_t)p.N * sizeof(W) + (size_t)p.n * 2 + 16;
+    a.pb = pick_pb(p, 2 * p.digitG, fixed, c->lds_budget, c->tune.br_pb);
+    for (int g = 0; g < 6; g++) a.gate_constant[g] = p.gate_constant[g];
+    a.cap_acc = cap ? cap->acc : nullptr;
+    a.ksd = ksd;
+    a.ksb = ksb;
+    a.dks = p.digitKS;
+    a.Bks = (uint32_t)p.Bks;
+    a.status = c->status;
+    const size_t lds = fixed + (size_t)a.pb * p.N * sizeof(W);
+    if (lds > c->lds_budget) return fail("parameter set does not fit the LDS budget");
+    {
+        Prof pr(c, 0);
+        HIP_TRY(launch_blind_rotate<W>(a, p.logN, c->tune.br_threads, B, lds, c->stream));
+    }
+    KSArgs<W> s{};
+    s.A = a.A;
+    s.ksk = (const W *)k->ksk;
+    s.ksd = ksd;
+    s.ksb = ksb;
+    s.n = p.n;
+    s.n_pad = n_pad(p);
+    s.S = p.N * p.digitKS;
+    s.Bks = (uint32_t)p.Bks;
+    s.Q = p.Q;
+    int lq = 0;
+    while ((1ULL << lq) < p.q) lq++;
+    s.lq1 = lq + 1;
+    s.out = out;
+    s.cap_ks = cap ? cap->ks : nullptr;
+    s.B = B;
+    s.KB = c->tune.ks_kb;
+    s.CB = c->tune.ks_cb;
+    s.SB = c->tune.ks_sb;
+    const bool gather = B <= c->tune.ks_gather_max || ks_staged_lds<W>(s.Bks) > c->lds_budget;
+    {
+        Prof pr(c, 1);
+        HIP_TRY(launch_lwe_keyswitch<W>(s, gather, c->stream));
+    }
+    return 0;
+}
+
+int fheacc_bootstrap_batch_dev(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
+                               const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap) {
+    if (check_ctx(c) || !k || !gates || !c1 || !c2 || !out) return fail("null argument");
+    if (!k->bsk || !k->ksk) return fail("bootstrap and key-switch keys must be resident");
+    if (B <= 0) return B == 0 ? 0 : fail("negative batch");
+    if (B > 0x7fffffff) return fail("batch too large");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return c->p.narrow ? bootstrap_dev_impl<uint32_t>(c, k, gates, c1, c2, out, B, cap)
+                       : bootstrap_dev_impl<uint64_t>(c, k, gates, c1, c2, out, B, cap);
+}
+
+int fheacc_bootstrap_batch(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
+                           const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap) {
+    if (check_ctx(c) || !k || !gates || !c1 || !c2 || !out) return fail("null argument");
+    if (B <= 0) return B == 0 ? 0 : fail("negative batch");
+    const HostParams &p = c->p;
+    const size_t lw = (size_t)p.n + 1;
+    for (int64_t i = 0; i < B; i++) {
+        if (gates[i] < 0 || gates[i] > 5) return fail("gate out of range");
+        if (!memcmp(c1 + i * lw, c2 + i * lw, lw * 8))
+            return fail("identical operands passed to the gate bootstrap (RLWE.cpp:727-730)");
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t in_b = B * lw * 8, acc_b = cap && cap->acc ? B * 2 * (size_t)p.N * 8 : 0,
+                 ks_b = cap && cap->ks ? in_b : 0;
+    DevBuf buf;
+    HIP_TRY(hipMalloc(&buf.p, 3 * in_b + acc_b + ks_b + B * 4 + 64));
+    char *base = (char *)buf.p;
+    uint64_t *d1 = (uint64_t *)base, *d2 = (uint64_t *)(base + in_b), *dout = (uint64_t *)(base + 2 * in_b);
+    uint64_t *dacc = acc_b ? (uint64_t *)(base + 3 * in_b) : nullptr;
+    uint64_t *dks = ks_b ? (uint64_t *)(base + 3 * in_b + acc_b) : nullptr;
+    int32_t *dg = (int32_t *)(base + 3 * in_b + acc_b + ks_b);
+    HIP_TRY(hipMemcpyAsync(d1, c1, in_b, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(d2, c2, in_b, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(dg, gates, B * 4, hipMemcpyHostToDevice, c->stream));
+    fheacc_capture dcap{dacc, dks};
+    if (int rc = fheacc_bootstrap_batch_dev(c, k, dg, d1, d2, dout, B, &dcap)) return rc;
+    HIP_TRY(hipMemcpyAsync(out, dout, in_b, hipMemcpyDeviceToHost, c->stream));
+    if (dacc) HIP_TRY(hipMemcpyAsync(cap->acc, dacc, acc_b, hipMemcpyDeviceToHost, c->stream));
+    if (dks) HIP_TRY(hipMemcpyAsync(cap->ks, dks, ks_b, hipMemcpyDeviceToHost, c->stream));
+    return fheacc_sync(c);
+}
+
+// ---------------------------------------------------------------- kernels
+
+int fheacc_ntt_dev(fheacc_ctx *c, uint64_t *polys, int64_t count, int inverse) {
+    if (check_ctx(c) || !polys) return fail("null argument");
+    if (count <= 0) return 0;
+    HIP_TRY(hipSetDevice(c->device));
+    const HostParams &p = c->p;
+    const int per_wg = p.N >= 1024 ? 2 : std::max(1, 2048 / p.N);
+    if (p.narrow) {
+        NTTArgs<uint32_t> a{make_arith<uint32_t>(p), make_tw<uint32_t>(c), polys, count, per_wg, inverse};
+        Prof pr(c, 3);
+        HIP_TRY(launch_ntt<uint32_t>(a, p.logN, c->stream));
+    } else {
+        NTTArgs<uint64_t> a{make_arith<uint64_t>(p), make_tw<uint64_t>(c), polys, count, per_wg, inverse};
+        Prof pr(c, 3);
+        HIP_TRY(launch_ntt<uint64_t>(a, p.logN, c->stream));
+    }
+    return 0;
+}
+
+template <typename W>
+static int ep_impl(fheacc_ctx *c, const uint64_t *acc, const uint64_t *rgsw, int64_t stride, uint64_t *out,
+                   int64_t count) {
+    const HostParams &p = c->p;
+    const int64_t nk = stride ? count : 1;
+    const size_t ke = rgsw_elems(p);
+    if (int rc = ensure_scratch(c, nk * ke * sizeof(W))) return rc;
+    const size_t rw = (size_t)p.digitG * 4 * p.N;
+    HIP_TRY(launch_rgsw_to_device<W>(rgsw, stride ? stride : (int64_t)rw, (W *)c->scratch, ke, p.digitG, p.N,
+                                     make_sg(p.Q), nullptr, nk, c->stream));
+    EPArgs<W> a{};
+    a.A = make_arith<W>(p);
+    a.tw = make_tw<W>(c);
+    a.acc = acc;
+    a.out = out;
+    a.key = (const W *)c->scratch;
+    a.key_stride = stride ? (int64_t)ke : 0;
+    a.d = p.digitG;
+    a.bits = p.BGbits;
+    a.pb = pick_pb(p, 2 * p.digitG, 2 * (size_t)p.N * sizeof(W), c->lds_budget, c->tune.br_pb);
+    Prof pr(c, 2);
+    HIP_TRY(launch_external_product<W>(a, p.logN, count, c->stream));
+    return 0;
+}
+
+int fheacc_external_product_dev(fheacc_ctx *c, const uint64_t *acc, const uint64_t *rgsw, int64_t stride,
+                                uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !acc || !rgsw || !out) return fail("null argument");
+    if (count <= 0) return 0;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return c->p.narrow ? ep_impl<uint32_t>(c, acc, rgsw, stride, out, count)
+                       : ep_impl<uint64_t>(c, acc, rgsw, stride, out, count);
+}
+
+template <typename W>
+static int subs_impl(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, int f, uint64_t *out, int64_t count) {
+    const HostParams &p = c->p;
+    SubsArgs<W> a{};
+    a.A = make_arith<W>(p);
+    a.tw = make_tw<W>(c);
+    a.in = in;
+    a.out = out;
+    a.key = (const W *)k->subs;
+    a.key_stride = (int64_t)p.digitKS_R * p.N * (p.narrow ? 2 : 4);
+    a.sub_factor = f;
+    a.d = p.digitKS_R;
+    a.bits = p.Bks_R_bits;
+    a.dch = pick_pb(p, p.digitKS_R, 4 * (size_t)p.N * sizeof(W), c->lds_budget, 0);
+    if ((4 + (size_t)a.dch) * p.N * sizeof(W) > c->lds_budget) return fail("subs key switch exceeds the LDS budget");
+    HIP_TRY(launch_subs_keyswitch<W>(a, p.logN, count, c->stream));
+    return 0;
+}
+
+int fheacc_subs_keyswitch_dev(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, int f, uint64_t *out,
+                              int64_t count) {
+    if (check_ctx(c) || !k || !in || !out) return fail("null argument");
+    if (!k->subs) return fail("no subs key resident");
+    if (f < 0 || f >= c->p.logN) return fail("sub_factor out of range");
+    if (count <= 0) return 0;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return c->p.narrow ? subs_impl<uint32_t>(c, k, in, f, out, count)
+                       : subs_impl<uint64_t>(c, k, in, f, out, count);
+}
+
+
+// ---------------------------------------------------------------- tuning
+
+int fheacc_ctx_set_tuning(fheacc_ctx *c, int threads, int pb) {
+    if (check_ctx(c)) return -1;
+    if (threads != 128 && threads != 256 && threads != 512) return fail("threads must be 128, 256 or 512");
+    if (pb < 0 || pb > 2 * (int)c->p.digitG) return fail("polys per batch out of range");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->tune.br_threads = threads;
+    c->tune.br_pb = pb;
+    return 0;
+}
+
+int fheacc_ctx_get_tuning(const fheacc_ctx *c, int *threads, int *pb) {
+    if (check_ctx(c) || !threads || !pb) return fail("null argument");
+    *threads = c->tune.br_threads;
+    *pb = c->tune.br_pb;
+    return 0;
+}
+
+// Synthetic operands: uniform words mod q (c1 != c2 almost surely; equal
+// pairs only raise the status flag, which is cleared afterwards).
+__global__ void k_fill_lwe(uint64_t *c1, uint64_t *c2, int32_t *g, int64_t words, uint64_t qm, int64_t B) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ULL + 0x2545F4914F6CDD1DULL;
+        h ^= h >> 29;
+        h *= 0xBF58476D1CE4E5B9ULL;
+        h ^= h >> 32;
+        c1[i] = h & qm;
+        c2[i] = (h >> 20) & qm;
+        if (i < B) g[i] = 3;
+    }
+}
+
+int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size_t report_len) {
+    if (check_ctx(c) || !k) return fail("null argument");
+    if (!k->bsk || !k->ksk) return fail("bootstrap and key-switch keys must be resident");
+    if (B <= 0 || B > (1 << 20)) return fail("tuning batch out of range");
+    HIP_TRY(hipSetDevice(c->device));
+    const HostParams &p = c->p;
+    const int64_t words = B * (p.n + 1);
+    DevBuf buf;
+    HIP_TRY(hipMalloc(&buf.p, 3 * words * 8 + B * 4 + 64));
+    uint64_t *c1 = (uint64_t *)buf.p, *c2 = c1 + words, *out = c2 + words;
+    int32_t *g = (int32_t *)(out + words);
+    hipLaunchKernelGGL(k_fill_lwe, dim3(256), dim3(256), 0, c->stream, c1, c2, g, words, p.q - 1, B);
+    HIP_TRY(hipGetLastError());
+    const Tuning saved = c->tune;
+    const bool saved_prof = c->profile;
+    std::string rep = "{\"batch\": " + std::to_string(B) + ", \"candidates\": [";
+    double best = 1e30;
+    int best_t = saved.br_threads, best_pb = saved.br_pb;
+    const int wsz = p.narrow ? 4 : 8;
+    bool first = true;
+    for (int threads : {128, 256, 512}) {
+        if (threads > p.N) continue;
+        for (int pb = 2; pb <= 2 * (int)p.digitG; pb += 2) {
+            const size_t lds = 2 * (size_t)p.N * wsz + (size_t)p.n * 2 + 16 + (size_t)pb * p.N * wsz;
+            if (lds > c->lds_budget) continue;
+            c->tune.br_threads = threads;
+            c->tune.br_pb = pb;
+            {
+                std::lock_guard<std::mutex> lk(c->mu);
+                c->profile = false;
+            }
+            int rc = fheacc_bootstrap_batch_dev(c, k, g, c1, c2, out, B, nullptr);  // warm
+            if (rc) {
+                c->tune = saved;
+                return rc;
+            }
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            {
+                std::lock_guard<std::mutex> lk(c->mu);
+                drop_events(c);
+                c->profile = true;
+            }
+            for (int r = 0; r < 2 && rc == 0; r++) rc = fheacc_bootstrap_batch_dev(c, k, g, c1, c2, out, B, nullptr);
+            double ms[4];
+            int64_t cnt[4];
+            if (rc == 0) rc = fheacc_ctx_kernel_ms(c, ms, cnt);
+            if (rc) {
+                c->tune = saved;
+                c->profile = saved_prof;
+                return rc;
+            }
+            const double br = ms[0] / std::max<int64_t>(cnt[0], 1);
+            if (!first) rep += ", ";
+            first = false;
+            rep += "{\"threads\": " + std::to_string(threads) + ", \"polys_per_batch\": " + std::to_string(pb) +
+                   ", \"lds_bytes\": " + std::to_string(lds) + ", \"blind_rotate_ms\": " + std::to_string(br) + "}";
+            if (br < best) {
+                best = br;
+                best_t = threads;
+                best_pb = pb;
+            }
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->profile = saved_prof;
+        drop_events(c);
+    }
+    c->tune.br_threads = best_t;
+    c->tune.br_pb = best_pb;
+    HIP_TRY(hipMemset(c->status, 0, sizeof(int)));
+    rep += "], \"chosen\": {\"threads\": " + std::to_string(best_t) + ", \"polys_per_batch\": " +
+           std::to_string(best_pb) + ", \"blind_rotate_ms\": " + std::to_string(best) + "}}";
+    if (report && report_len) {
+        strncpy(report, rep.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return 0;
+}

@@ -1,0 +1,95 @@
+// fheacc_internal.hpp -- context, key set and kernel argument structures.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fheacc.h"
+
+namespace fheacc {
+
+// Derived parameter block: what the reference keeps in its globals
+// (FHE_ACC_runtime/include/params.h:13-134), one instance per set.
+struct HostParams {
+    int32_t n;
+    uint64_t q;
+    int32_t N, logN;
+    uint64_t Q, iN, ROU;
+    uint32_t BGbits, digitG;
+    uint64_t Br;
+    uint32_t digitR;
+    uint64_t Bks;
+    uint32_t digitKS;
+    uint32_t Bks_R_bits, digitKS_R;
+    uint64_t E, E_rlwe;
+    double stddev;
+    uint64_t gate_constant[6];
+    std::vector<uint64_t> rou, irou;  // reference tables (params.cpp:149-175)
+    std::vector<uint64_t> cdt;        // Gaussian magnitude table (2^63 scale)
+    bool narrow;                      // Q < 2^30: 32-bit words on device
+};
+
+int derive_params(const fheacc_params *in, HostParams *out, std::string *err);
+
+// Host mirror of the device sampler (identical word streams).
+uint64_t rng_word(uint64_t seed, uint64_t stream, uint64_t ctr);
+int64_t gauss_sample(const HostParams &p, uint64_t word);
+
+enum : uint64_t {
+    TAG_LWE_SK = 1, TAG_RLWE_SK = 2, TAG_BSK = 3, TAG_KSK = 4, TAG_SUBS = 5,
+    TAG_LWE_ENC = 6, TAG_RLWE_ENC = 7
+};
+__host__ __device__ inline uint64_t stream_id(uint64_t tag, uint64_t idx) { return (tag << 56) | idx; }
+
+struct DeviceTables {
+    void *fw = nullptr, *fwp = nullptr, *iw = nullptr, *iwp = nullptr;  // W-typed [N]
+    uint64_t iN = 0, iNp = 0;
+    uint64_t *cdt = nullptr;  // [cdt_len]
+    int cdt_len = 0;
+};
+
+// Tuning knobs picked per parameter set by the autotuner (tune.cpp).
+struct Tuning {
+    int br_threads = 256;     // threads per blind-rotation workgroup
+    int br_pb = 0;            // digit polys per NTT batch (0: as many as fit)
+    int ks_cb = 512;          // ciphertexts per key-switch workgroup
+    int ks_kb = 8;            // output columns per key-switch workgroup
+    int ks_sb = 32;           // key-switch steps staged per LDS fill
+    int ks_gather_max = 32;   // batches up to this size use the gather key switch
+};
+
+}  // namespace fheacc
+
+struct fheacc_ctx {
+    fheacc::HostParams p;
+    fheacc::DeviceTables t;
+    fheacc::Tuning tune;
+    int device = 0;
+    size_t lds_budget = 64 * 1024;  // bytes of LDS one workgroup may use
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    // per-context scratch, grown on demand
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int *status = nullptr;  // device status word (identical-operand flag)
+    std::mutex mu;
+    // profiling: (class, start, stop) event records while enabled
+    bool profile = false;
+    struct EvRec {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<EvRec> events;
+};
+
+struct fheacc_keys {
+    fheacc_ctx *ctx = nullptr;
+    void *bsk = nullptr;   // device layout, see kernels.hip
+    void *ksk = nullptr;
+    void *subs = nullptr;
+    size_t bsk_bytes = 0, ksk_bytes = 0, subs_bytes = 0;
+    uint64_t bsk_digest = 0, ksk_digest = 0;
+};

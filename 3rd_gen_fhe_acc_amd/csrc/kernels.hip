@@ -1,0 +1,985 @@
+// kernels.hip -- HIP kernels of the gate-bootstrapping path for gfx950.
+//
+// Device layouts (W = uint32_t for Q < 2^30, else uint64_t):
+//   RGSW (external-product key)  [digitG][N] x {K0a, K0b, K1a, K1b}
+//        K0 = c_text[h][0] (multiplies the digits of a), K1 = c_text[h][1]
+//        (digits of b), RLWE.cpp:642-648.  Narrow: uint4 per (h, k).  Wide:
+//        each value followed by its Shoup companion floor(K * 2^64 / Q).
+//   bootstrap key  [n][digitR][Br-1] RGSW: the keys of one blind-rotation
+//        step (i, j) are contiguous; residue 0 is skipped by the path
+//        (RLWE.cpp:759) and not stored.
+//   LWE ks key     [N][digitKS][Bks][n_pad] W: the Bks candidate rows of one
+//        key-switch step (coefficient i, digit j) are contiguous.
+//   subs key       [log2 N][digitKS_R][N] x {Ka, Kb} (+ companions if wide).
+//
+// One ciphertext per workgroup for every polynomial kernel; polynomials live
+// in LDS and are transformed by ntt_lds.hpp.  Nothing here is floating
+// point: no MFMA.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fheacc_internal.hpp"
+#include "kernels.hpp"
+#include "ntt_lds.hpp"
+
+namespace fheacc {
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t stream_key(uint64_t seed, uint64_t stream) {
+    return mix64(seed + stream * 0x9E3779B97F4A7C15ULL);
+}
+__device__ __forceinline__ uint64_t stream_word(uint64_t key, uint64_t ctr) {
+    return mix64(key + (ctr + 1) * 0x9E3779B97F4A7C15ULL);
+}
+__device__ __forceinline__ int64_t gauss_dev(const uint64_t *cdt, int len, uint64_t w) {
+    uint64_t v = w >> 1;
+    int k = 0;
+    while (k < len - 1 && v >= cdt[k]) k++;
+    if (!k) return 0;
+    return (w & 1) ? -(int64_t)k : (int64_t)k;
+}
+
+// floor((2 v q + Q) / (2 Q)), LWE.cpp:255-260 (not reduced mod q)
+__device__ __forceinline__ uint64_t scale_round_dev(uint64_t v, int lq1, uint64_t Q) {
+    uint64_t lo = v << lq1, hi = v >> (64 - lq1);
+    uint64_t lo2 = lo + Q;
+    hi += lo2 < lo;
+    lo = lo2;
+    const uint64_t twoQ = 2 * Q;
+    double approx = ((double)v * (double)(1ULL << lq1) + (double)Q) / (double)twoQ;
+    uint64_t r = approx > 0 ? (uint64_t)approx : 0;
+    // exact fix-up: r * 2Q <= num < (r + 1) * 2Q
+    for (int it = 0; it < 2; it++) {
+        uint64_t ph = __umul64hi(r, twoQ), pl = r * twoQ;
+        if (ph > hi || (ph == hi && pl > lo)) { r--; continue; }
+        uint64_t r1 = r + 1;
+        uint64_t qh = __umul64hi(r1, twoQ), ql = r1 * twoQ;
+        if (qh < hi || (qh == hi && ql <= lo)) r = r1;
+        else break;
+    }
+    return r;
+}
+
+// ShoupGen (kernels.hpp): k * 2^64 / Q for k < Q, from a = floor(2^64/Q),
+// b = 2^64 mod Q and b's Shoup companion; the wide key layout stores it
+// beside each key word.
+
+template <typename W>
+__device__ __forceinline__ W mulmod_generic(const Arith<W> &A, W x, W y, W yp);
+template <>
+__device__ __forceinline__ uint32_t mulmod_generic(const Arith<uint32_t> &A, uint32_t x, uint32_t y, uint32_t) {
+    return A.reduce64((uint64_t)x * y);
+}
+template <>
+__device__ __forceinline__ uint64_t mulmod_generic(const Arith<uint64_t> &A, uint64_t x, uint64_t y, uint64_t yp) {
+    return Arith<uint64_t>::red(A.mul_shoup(x, y, yp), A.Q);
+}
+
+// ---------------------------------------------------------------------------
+// external product (RLWE.cpp:623-704) on an accumulator held in LDS
+// ---------------------------------------------------------------------------
+
+template <typename W>
+struct MacSum;
+template <>
+struct MacSum<uint32_t> {
+    uint64_t v = 0;
+    __device__ __forceinline__ void add(const Arith<uint32_t> &, uint32_t d, uint32_t k, uint32_t) {
+        v += (uint64_t)d * k;
+    }
+    __device__ __forceinline__ uint32_t get(const Arith<uint32_t> &A) const { return A.reduce64(v); }
+};
+template <>
+struct MacSum<uint64_t> {
+    uint64_t v = 0;
+    __device__ __forceinline__ void add(const Arith<uint64_t> &A, uint64_t d, uint64_t k, uint64_t kp) {
+        v += A.mul_shoup(d, k, kp);
+    }
+    __device__ __forceinline__ uint64_t get(const Arith<uint64_t> &A) const { return A.reduce_small(v); }
+};
+
+// Key words of row h at coefficient k: (K0a, K0b, K1a, K1b) and companions.
+template <typename W>
+struct KeyQuad {
+    W v[4], p[4];
+};
+__device__ __forceinline__ KeyQuad<uint32_t> load_quad(const uint32_t *key, int idx) {
+    uint4 q = reinterpret_cast<const uint4 *>(key)[idx];
+    KeyQuad<uint32_t> r;
+    r.v[0] = q.x; r.v[1] = q.y; r.v[2] = q.z; r.v[3] = q.w;
+    r.p[0] = r.p[1] = r.p[2] = r.p[3] = 0;
+    return r;
+}
+__device__ __forceinline__ KeyQuad<uint64_t> load_quad(const uint64_t *key, int idx) {
+    const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(key) + 4 * (size_t)idx;
+    KeyQuad<uint64_t> r;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        ulonglong2 t = p[i];
+        r.v[i] = t.x;
+        r.p[i] = t.y;
+    }
+    return r;
+}
+
+// acc (2N words in LDS, NTT domain, canonical) <- acc (x) rgsw.
+// The 2d digit polys are taken in the order (h, a), (h, b), h = 0..d-1, and
+// transformed pb at a time in dig (pb polys of LDS scratch).
+template <typename W, int LOGN, int NT>
+__device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const Twiddles<W> &tw, int d,
+                         int bits, int pb) {
+    constexpr int N = 1 << LOGN;
+    constexpr int CPT = (N + NT - 1) / NT;
+    ntt_inverse<NT>(acc, 2, LOGN, A, tw);
+    MacSum<W> sa[CPT], sb[CPT];
+    const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
+    const int total = 2 * d;
+    for (int P0 = 0; P0 < total; P0 += pb) {
+        const int np = min(pb, total - P0);
+        for (int idx = threadIdx.x; idx < (np << LOGN); idx += NT) {
+            const int P = P0 + (idx >> LOGN), k = idx & (N - 1);
+            const W x = acc[((P & 1) << LOGN) + k];
+            dig[idx] = (x >> ((P >> 1) * bits)) & mask;
+        }
+        __syncthreads();
+        ntt_forward<NT>(dig, np, LOGN, A, tw);
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int k = threadIdx.x + c * NT;
+            if (k < N) {
+                for (int pp = 0; pp < np;) {
+                    const int P = P0 + pp;
+                    KeyQuad<W> kq = load_quad(key, ((P >> 1) << LOGN) + k);
+                    if (!(P & 1)) {
+                        const W da = dig[(pp << LOGN) + k];
+                        sa[c].add(A, da, kq.v[0], kq.p[0]);
+                        sb[c].add(A, da, kq.v[1], kq.p[1]);
+                        if (pp + 1 < np) {
+                            const W db = dig[((pp + 1) << LOGN) + k];
+                            sa[c].add(A, db, kq.v[2], kq.p[2]);
+                            sb[c].add(A, db, kq.v[3], kq.p[3]);
+                            pp += 2;
+                        } else {
+                            pp += 1;
+                        }
+                    } else {
+                        const W db = dig[(pp << LOGN) + k];
+                        sa[c].add(A, db, kq.v[2], kq.p[2]);
+                        sb[c].add(A, db, kq.v[3], kq.p[3]);
+                        pp += 1;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int k = threadIdx.x + c * NT;
+        if (k < N) {
+            acc[k] = sa[c].get(A);
+            acc[N + k] = sb[c].get(A);
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// K10 + K5 + blind rotation + KS preparation, one ciphertext per workgroup
+// (RLWE.cpp:726-767, LWE.cpp:203-222,233-240, RLWE.cpp:218-226)
+// ---------------------------------------------------------------------------
+
+template <typename W, int LOGN, int NT>
+__global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
+    constexpr int N = 1 << LOGN;
+    extern __shared__ __align__(16) unsigned char smem[];
+    W *acc = reinterpret_cast<W *>(smem);
+    W *dig = acc + 2 * N;
+    uint16_t *av = reinterpret_cast<uint16_t *>(dig + a.pb * N);
+    __shared__ uint32_t s_b;
+    const int64_t c = blockIdx.x;
+    const int n = a.n;
+    const uint32_t q = a.q, qm = a.q - 1;
+    const int g = a.gates[c];
+    const bool xr = (g == 4 || g == 5);
+    const uint64_t *x1 = a.c1 + c * (n + 1), *x2 = a.c2 + c * (n + 1);
+    int same = 1;
+    for (int k = threadIdx.x; k <= n; k += NT) {
+        uint64_t u = x1[k], v = x2[k];
+        same &= (u == v);
+        uint32_t r = (uint32_t)(xr ? 2 * (u - v) : u + v) & qm;
+        if (k < n) av[k] = (uint16_t)r;
+        else s_b = r;
+    }
+    if (__syncthreads_and(same) && threadIdx.x == 0) atomicOr(a.status, 1);
+    // accumulator init: only b carries the test polynomial
+    const uint64_t Q8 = a.A.Q / 8 + 1, nQ8 = a.A.Q - Q8;
+    const uint32_t b = s_b;
+    const uint32_t b1 = (uint32_t)(a.gate_constant[g] & qm), b2 = (b1 + q / 2) & qm;
+    const int ef = (2 * N) / (int)q;
+    for (int idx = threadIdx.x; idx < 2 * N; idx += NT) acc[idx] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)(q / 2); i += NT) {
+        const uint32_t tmp = (b - (uint32_t)i) & qm;
+        bool neg = (b1 < b2) ? (tmp >= b1 && tmp < b2) : !(tmp >= b2 && tmp < b1);
+        acc[N + i * ef] = (W)(neg ? nQ8 : Q8);
+    }
+    __syncthreads();
+    ntt_forward<NT>(acc + N, 1, LOGN, a.A, a.tw);
+    // blind rotation: i outer, j inner, zero residues skipped
+    const W *bsk = a.bsk;
+    for (int i = 0; i < n; i++) {
+        uint32_t nega = (q - av[i]) & qm;
+        for (int j = 0; j < a.digitR; j++) {
+            const uint32_t r = nega % a.Br;
+            nega /= a.Br;
+            if (r) {
+                const W *key = bsk + ((size_t)(i * a.digitR + j) * (a.Br - 1) + (r - 1)) * a.rgsw_elems;
+                ext_prod<W, LOGN, NT>(acc, dig, key, a.A, a.tw, a.d, a.bits, a.pb);
+            }
+        }
+    }
+    if (a.cap_acc) {
+        uint64_t *dst = a.cap_acc + c * 2 * N;
+        for (int idx = threadIdx.x; idx < 2 * N; idx += NT) dst[idx] = acc[idx];
+    }
+    __syncthreads();
+    // LWE_keyswitch preparation: NTT-domain reversal of a, inverse NTT,
+    // base-Bks digits of every a coefficient, b0 + Q/8 + 1
+    for (int idx = threadIdx.x; idx < N / 2; idx += NT) {
+        W t = acc[idx];
+        acc[idx] = acc[N - 1 - idx];
+        acc[N - 1 - idx] = t;
+    }
+    __syncthreads();
+    ntt_inverse<NT>(acc, 2, LOGN, a.A, a.tw);
+    const int dks = a.dks;
+    const uint32_t Bks = a.Bks;
+    uint8_t *dst = a.ksd + c * (size_t)N * dks;
+    for (int i = threadIdx.x; i < N; i += NT) {
+        uint64_t x = acc[i];
+        for (int j = 0; j < dks; j++) {
+            dst[i * dks + j] = (uint8_t)(x % Bks);
+            x /= Bks;
+        }
+    }
+    if (threadIdx.x == 0) {
+        uint64_t v = (uint64_t)acc[N] + Q8;
+        a.ksb[c] = v >= a.A.Q ? v - a.A.Q : v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LWE key switch + modulus switch (LWE.cpp:241-269)
+// ---------------------------------------------------------------------------
+
+template <typename W>
+__device__ __forceinline__ uint64_t ks_finish(const KSArgs<W> &a, int64_t c, int col, uint64_t s) {
+    const uint64_t Q = a.Q;
+    uint64_t v;
+    if (col < a.n) v = s ? Q - s : 0;
+    else {
+        uint64_t b = a.ksb[c];
+        v = b >= s ? b - s : b + Q - s;
+    }
+    if (a.cap_ks) a.cap_ks[c * (a.n + 1) + col] = v;
+    return scale_round_dev(v, a.lq1, Q);
+}
+
+template <typename W>
+__device__ __forceinline__ void ks_add(uint64_t &s, W v, uint64_t Q);
+template <>
+__device__ __forceinline__ void ks_add(uint64_t &s, uint32_t v, uint64_t) { s += v; }
+template <>
+__device__ __forceinline__ void ks_add(uint64_t &s, uint64_t v, uint64_t Q) {
+    uint64_t t = s + v;
+    s = t >= Q ? t - Q : t;
+}
+template <typename W>
+__device__ __forceinline__ uint64_t ks_reduce(uint64_t s, const Arith<W> &A);
+template <>
+__device__ __forceinline__ uint64_t ks_reduce(uint64_t s, const Arith<uint32_t> &A) { return A.reduce64(s); }
+template <>
+__device__ __forceinline__ uint64_t ks_reduce(uint64_t s, const Arith<uint64_t> &) { return s; }
+
+// Staged form: a workgroup owns KB output columns of NT ciphertexts, one
+// ciphertext per thread.  The Bks candidate rows (KB words each) of 16 steps
+// are staged in LDS per fill; each thread reads its own 16 digits with one
+// 16-byte load and adds the rows they select.  The key is streamed once per
+// NT ciphertexts.  Rows are padded to RS words to spread LDS banks.
+template <typename W, int NT, int KB>
+__global__ __launch_bounds__(NT) void k_lwe_keyswitch_staged(KSArgs<W> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int SB = 16;                             // steps per LDS fill
+    constexpr int RS = KB + 16 / (int)sizeof(W);       // padded row stride (words)
+    constexpr int V = 16 / (int)sizeof(W);             // words per 16-byte vector
+    const uint32_t Bks = a.Bks;
+    W *kt = reinterpret_cast<W *>(smem);               // [SB][Bks][RS]
+    const int col0 = blockIdx.x * KB;
+    const int64_t c = (int64_t)blockIdx.y * NT + threadIdx.x;
+    const bool live = c < a.B;
+    const int S = a.S;
+    const uint8_t *dg = a.ksd + (live ? c : 0) * (size_t)S;
+    uint64_t s[KB];
+#pragma unroll
+    for (int j = 0; j < KB; j++) s[j] = 0;
+    const int vec_per_fill = SB * Bks * (KB / V);
+    for (int s0 = 0; s0 < S; s0 += SB) {
+        const int nsb = min(SB, S - s0);
+        for (int e = threadIdx.x; e < vec_per_fill; e += NT) {
+            const int row = e / (KB / V), vv = e - row * (KB / V);  // row = st * Bks + dd
+            const int st = row / Bks;
+            const int gc = col0 + vv * V;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (st < nsb && gc + V <= a.n_pad)
+                val = *reinterpret_cast<const uint4 *>(a.ksk + ((size_t)(s0 * Bks) + row) * a.n_pad + gc);
+            *reinterpret_cast<uint4 *>(kt + row * RS + vv * V) = val;
+        }
+        uint8_t d16[SB];
+        if (nsb == SB) {
+            uint4 dv = *reinterpret_cast<const uint4 *>(dg + s0);
+            const uint32_t w4[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+            for (int t = 0; t < SB; t++) d16[t] = (uint8_t)(w4[t >> 2] >> (8 * (t & 3)));
+        } else {
+#pragma unroll
+            for (int t = 0; t < SB; t++) d16[t] = t < nsb ? dg[s0 + t] : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int st = 0; st < SB; st++) {
+            if (st < nsb) {
+                const W *row = kt + (st * (int)Bks + d16[st]) * RS;
+#pragma unroll
+                for (int vv = 0; vv < KB / V; vv++) {
+                    uint4 q = *reinterpret_cast<const uint4 *>(row + vv * V);
+                    if (sizeof(W) == 4) {
+                        ks_add<W>(s[vv * 4 + 0], (W)q.x, a.Q);
+                        ks_add<W>(s[vv * 4 + 1], (W)q.y, a.Q);
+                        ks_add<W>(s[vv * 4 + 2], (W)q.z, a.Q);
+                        ks_add<W>(s[vv * 4 + 3], (W)q.w, a.Q);
+                    } else {
+                        ks_add<W>(s[vv * 2 + 0], (W)(((uint64_t)q.y << 32) | q.x), a.Q);
+                        ks_add<W>(s[vv * 2 + 1], (W)(((uint64_t)q.w << 32) | q.z), a.Q);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+        const int col = col0 + j;
+        if (col <= a.n) a.out[c * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s[j], a.A));
+    }
+}
+
+// Gather form for small batches: each ciphertext reads only its N*digitKS rows.
+template <typename W, int NT>
+__global__ __launch_bounds__(NT) void k_lwe_keyswitch_gather(KSArgs<W> a) {
+    const int col = blockIdx.x * NT + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    const uint8_t *dg = a.ksd + c * (size_t)a.S;
+    uint64_t s = 0;
+    if (col <= a.n) {
+        for (int st = 0; st < a.S; st++) ks_add<W>(s, a.ksk[((size_t)st * a.Bks + dg[st]) * a.n_pad + col], a.Q);
+        a.out[c * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s, a.A));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched NTT / external product / subs + RLWE key switch
+// ---------------------------------------------------------------------------
+
+template <typename W, int LOGN, int NT>
+__global__ __launch_bounds__(NT) void k_ntt(NTTArgs<W> a) {
+    constexpr int N = 1 << LOGN;
+    extern __shared__ __align__(16) unsigned char smem[];
+    W *s = reinterpret_cast<W *>(smem);
+    const int64_t p0 = (int64_t)blockIdx.x * a.per_wg;
+    const int np = (int)min((int64_t)a.per_wg, a.count - p0);
+    uint64_t *g = a.polys + p0 * N;
+    for (int i = threadIdx.x; i < np * N; i += NT) s[i] = (W)g[i];
+    __syncthreads();
+    if (a.inverse) ntt_inverse<NT>(s, np, LOGN, a.A, a.tw);
+    else ntt_forward<NT>(s, np, LOGN, a.A, a.tw);
+    for (int i = threadIdx.x; i < np * N; i += NT) g[i] = s[i];
+}
+
+template <typename W, int LOGN, int NT>
+__global__ __launch_bounds__(NT) void k_external_product(EPArgs<W> a) {
+    constexpr int N = 1 << LOGN;
+    extern __shared__ __align__(16) unsigned char smem[];
+    W *acc = reinterpret_cast<W *>(smem);
+    W *dig = acc + 2 * N;
+    const int64_t c = blockIdx.x;
+    const uint64_t *src = a.acc + c * 2 * N;
+    for (int i = threadIdx.x; i < 2 * N; i += NT) acc[i] = (W)src[i];
+    __syncthreads();
+    ext_prod<W, LOGN, NT>(acc, dig, a.key + c * a.key_stride, a.A, a.tw, a.d, a.bits, a.pb);
+    uint64_t *dst = a.out + c * 2 * N;
+    for (int i = threadIdx.x; i < 2 * N; i += NT) dst[i] = acc[i];
+}
+
+// RLWE_substitute (RLWE.cpp:444-462, poly_substitute 23-37) followed by
+// RLWE_keyswitch (RLWE.cpp:464-510).  The keyswitch's own iNTT of the
+// substituted ciphertext is the identity on the time-domain words already
+// in LDS, so the digits are taken from them directly.
+template <typename W, int LOGN, int NT>
+__global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
+    constexpr int N = 1 << LOGN;
+    constexpr int CPT = (N + NT - 1) / NT;
+    extern __shared__ __align__(16) unsigned char smem[];
+    W *x = reinterpret_cast<W *>(smem);  // input, time domain
+    W *y = x + 2 * N;                    // substituted
+    W *dig = y + 2 * N;                  // dch digit polys
+    const int64_t c = blockIdx.x;
+    const uint64_t *src = a.in + c * 2 * N;
+    for (int i = threadIdx.x; i < 2 * N; i += NT) x[i] = (W)src[i];
+    __syncthreads();
+    ntt_inverse<NT>(x, 2, LOGN, a.A, a.tw);
+    const int power = (N >> a.sub_factor) + 1;
+    for (int idx = threadIdx.x; idx < 2 * N; idx += NT) {
+        const int half = idx >> LOGN, i = idx & (N - 1);
+        const int64_t ip = (int64_t)i * power;
+        const int io = (int)(ip & (N - 1));
+        const W v = x[idx];
+        y[(half << LOGN) + io] = ((ip >> LOGN) & 1) ? (v ? (W)a.A.Q - v : (W)0) : v;
+    }
+    __syncthreads();
+    // output b starts from NTT(subs(b)); the digits come from subs(a)
+    for (int i = threadIdx.x; i < N; i += NT) x[N + i] = y[N + i];
+    __syncthreads();
+    ntt_forward<NT>(x + N, 1, LOGN, a.A, a.tw);
+    MacSum<W> sa[CPT], sb[CPT];
+    const int d = a.d, bits = a.bits, dch = a.dch;
+    const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
+    const W *key = a.key + (size_t)a.sub_factor * a.key_stride;
+    for (int h0 = 0; h0 < d; h0 += dch) {
+        const int nd = min(dch, d - h0);
+        for (int k = threadIdx.x; k < N; k += NT) {
+            const W v = y[k];
+            for (int hh = 0; hh < nd; hh++) dig[(hh << LOGN) + k] = (v >> ((h0 + hh) * bits)) & mask;
+        }
+        __syncthreads();
+        ntt_forward<NT>(dig, nd, LOGN, a.A, a.tw);
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) {
+            const int k = threadIdx.x + cc * NT;
+            if (k < N) {
+                for (int hh = 0; hh < nd; hh++) {
+                    const W dv = dig[(hh << LOGN) + k];
+                    const size_t e = ((size_t)(h0 + hh) << LOGN) + k;
+                    if (sizeof(W) == 4) {
+                        uint2 kv = reinterpret_cast<const uint2 *>(key)[e];
+                        sa[cc].add(a.A, dv, (W)kv.x, 0);
+                        sb[cc].add(a.A, dv, (W)kv.y, 0);
+                    } else {
+                        const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(key) + 2 * e;
+                        ulonglong2 ka = kp[0], kb = kp[1];
+                        sa[cc].add(a.A, dv, (W)ka.x, (W)ka.y);
+                        sb[cc].add(a.A, dv, (W)kb.x, (W)kb.y);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    uint64_t *dst = a.out + c * 2 * N;
+    const uint64_t Q = a.A.Q;
+#pragma unroll
+    for (int cc = 0; cc < CPT; cc++) {
+        const int k = threadIdx.x + cc * NT;
+        if (k < N) {
+            const uint64_t ma = sa[cc].get(a.A), mb = sb[cc].get(a.A);
+            dst[k] = ma ? Q - ma : 0;
+            const uint64_t bv = x[N + k];
+            dst[N + k] = bv >= mb ? bv - mb : bv + Q - mb;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// key layout conversion and seeded generation
+// ---------------------------------------------------------------------------
+
+// reference RGSW [d][2][2][N] u64 -> device quad layout
+template <typename W>
+__global__ void k_rgsw_to_device(const uint64_t *src, int64_t src_stride, W *dst, int64_t dst_stride,
+                                 int d, int N, ShoupGen sg, const int64_t *map, int64_t count) {
+    const int64_t e = blockIdx.y;
+    if (e >= count) return;
+    const int64_t se = map ? map[e] : e;
+    const uint64_t *s = src + se * src_stride;
+    W *o = dst + e * dst_stride;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < d * N; idx += gridDim.x * blockDim.x) {
+        const int h = idx / N, k = idx - h * N;
+        const uint64_t *r = s + (size_t)h * 4 * N;
+        uint64_t v0 = r[k], v1 = r[N + k], v2 = r[2 * N + k], v3 = r[3 * N + k];
+        if (sizeof(W) == 4) {
+            reinterpret_cast<uint4 *>(o)[idx] = make_uint4((uint32_t)v0, (uint32_t)v1, (uint32_t)v2, (uint32_t)v3);
+        } else {
+            ulonglong2 *p = reinterpret_cast<ulonglong2 *>(o) + 4 * (size_t)idx;
+            p[0] = make_ulonglong2(v0, sg(v0));
+            p[1] = make_ulonglong2(v1, sg(v1));
+            p[2] = make_ulonglong2(v2, sg(v2));
+            p[3] = make_ulonglong2(v3, sg(v3));
+        }
+    }
+}
+
+// device quad layout -> reference RGSW layout
+template <typename W>
+__global__ void k_rgsw_to_ref(const W *src, int64_t src_stride, uint64_t *dst, int64_t dst_stride, int d, int N,
+                              const int64_t *map, int64_t count) {
+    const int64_t e = blockIdx.y;
+    if (e >= count) return;
+    const W *s = src + e * src_stride;
+    uint64_t *o = dst + (map ? map[e] : e) * dst_stride;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < d * N; idx += gridDim.x * blockDim.x) {
+        const int h = idx / N, k = idx - h * N;
+        uint64_t *r = o + (size_t)h * 4 * N;
+        if (sizeof(W) == 4) {
+            uint4 q = reinterpret_cast<const uint4 *>(s)[idx];
+            r[k] = q.x; r[N + k] = q.y; r[2 * N + k] = q.z; r[3 * N + k] = q.w;
+        } else {
+            const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(s) + 4 * (size_t)idx;
+            r[k] = p[0].x; r[N + k] = p[1].x; r[2 * N + k] = p[2].x; r[3 * N + k] = p[3].x;
+        }
+    }
+}
+
+// reference KSK [Bks][dks][N][n+1] -> device [N][dks][Bks][n_pad] (and back)
+template <typename W>
+__global__ void k_ksk_convert(const uint64_t *src, W *dst, uint64_t *back, int Bks, int dks, int N, int n1,
+                              int n_pad) {
+    const int64_t row = blockIdx.x;  // device row index ((i * dks + j) * Bks + dd)
+    const int dd = row % Bks;
+    const int64_t ij = row / Bks;
+    const int j = ij % dks, i = ij / dks;
+    const int64_t rrow = ((int64_t)dd * dks + j) * N + i;
+    for (int k = threadIdx.x; k < n_pad; k += blockDim.x) {
+        if (back) {
+            if (k < n1) back[rrow * n1 + k] = dst[row * n_pad + k];
+        } else {
+            dst[row * n_pad + k] = k < n1 ? (W)src[rrow * n1 + k] : (W)0;
+        }
+    }
+}
+
+// Forward NTT of the RLWE secret (and Shoup companions) for key generation.
+template <typename W, int LOGN, int NT>
+__global__ __launch_bounds__(NT) void k_secret_ntt(const uint64_t *z, W *zn, W *znp, Arith<W> A, Twiddles<W> tw,
+                                                   ShoupGen sg) {
+    constexpr int N = 1 << LOGN;
+    extern __shared__ __align__(16) unsigned char smem[];
+    W *s = reinterpret_cast<W *>(smem);
+    for (int i = threadIdx.x; i < N; i += NT) s[i] = (W)z[i];
+    __syncthreads();
+    ntt_forward<NT>(s, 1, LOGN, A, tw);
+    for (int i = threadIdx.x; i < N; i += NT) {
+        zn[i] = s[i];
+        znp[i] = sizeof(W) == 4 ? (W)(((uint64_t)s[i] << 32) / A.Q) : (W)sg(s[i]);
+    }
+}
+
+// RLWE encryption of zero into LDS (RLWE.cpp:324-344): a uniform and e
+// Gaussian in the time domain, both transformed, b = a*s + e.  Word stream as
+// the host sampler.  buf: 2N words.  Leaves a in buf[0..N), b in buf[N..2N).
+template <typename W, int LOGN, int NT>
+__device__ void rlwe_enc_zero(W *buf, const W *zn, const W *znp, uint64_t seed, uint64_t stream,
+                              const KeygenArgs<W> &a) {
+    constexpr int N = 1 << LOGN;
+    const uint64_t key = stream_key(seed, stream);
+    const uint64_t Q = a.A.Q;
+    for (int k = threadIdx.x; k < N; k += NT) {
+        buf[k] = (W)__umul64hi(stream_word(key, k), Q);
+        int64_t e = gauss_dev(a.cdt, a.cdt_len, stream_word(key, N + k)) % (int64_t)a.E_rlwe;
+        buf[N + k] = (W)(e < 0 ? (int64_t)Q + e : e);
+    }
+    __syncthreads();
+    ntt_forward<NT>(buf, 2, LOGN, a.A, a.tw);
+    for (int k = threadIdx.x; k < N; k += NT) {
+        W p = mulmod_generic<W>(a.A, buf[k], zn[k], znp[k]);
+        uint64_t t = (uint64_t)p + buf[N + k];
+        buf[N + k] = (W)(t >= Q ? t - Q : t);
+    }
+    __syncthreads();
+}
+
+// Bootstrap key: one workgroup per (RGSW, row h); both columns g = 0, 1.
+// RLWE.cpp:512-526 (X^mm, sign), 561-577 (rows), 579-606 (btkey[r][j][k]).
+template <typename W, int LOGN, int NT>
+__global__ __launch_bounds__(NT) void k_bsk_gen(KeygenArgs<W> a) {
+    constexpr int N = 1 << LOGN;
+    extern __shared__ __align__(16) unsigned char smem[];
+    W *e0 = reinterpret_cast<W *>(smem);  // one column's RLWE: a, b
+    W *m = e0 + 2 * N;                    // message monomial
+    const int d = a.d;
+    const int64_t wid = blockIdx.x;
+    const int h = wid % d;
+    const int64_t dev_rgsw = wid / d;  // ((k * digitR + j) * (Br-1) + r - 1)
+    const int rr = dev_rgsw % (a.Br - 1) + 1;
+    const int64_t kj = dev_rgsw / (a.Br - 1);
+    const int j = kj % a.digitR;
+    const int k = kj / a.digitR;
+    uint32_t powbr = 1;
+    for (int t = 0; t < j; t++) powbr = (uint32_t)(((uint64_t)powbr * a.Br) % a.q);
+    const uint32_t msg = (uint32_t)((((uint64_t)rr * powbr) % a.q * a.lwe_sk[k]) % a.q);
+    uint32_t mm = msg * (uint32_t)(2 * N / a.q);
+    bool neg = false;
+    if (mm >= (uint32_t)N) { mm -= N; neg = true; }
+    const uint64_t Q = a.A.Q;
+    uint64_t coef = a.bg_pow[h];  // BG^h mod Q
+    if (neg) coef = coef ? Q - coef : 0;
+    for (int i = threadIdx.x; i < N; i += NT) m[i] = (W)(i == (int)mm ? coef : 0);
+    const int64_t ref_e = ((int64_t)rr * a.digitR + j) * a.n + k;
+    const uint64_t sbase = stream_id(TAG_BSK, (uint64_t)ref_e * 2 * d) + (uint64_t)h * 2;
+    __syncthreads();
+    ntt_forward<NT>(m, 1, LOGN, a.A, a.tw);
+    W *o = a.bsk + (size_t)dev_rgsw * a.rgsw_elems;
+    // column g = 0 carries m in a (K0a, K0b); g = 1 carries m in b (K1a, K1b)
+    for (int g = 0; g < 2; g++) {
+        rlwe_enc_zero<W, LOGN, NT>(e0, a.zn, a.znp, a.seed, sbase + g, a);
+        for (int i = threadIdx.x; i < N; i += NT) {
+            uint64_t va = e0[i], vb = e0[N + i];
+            uint64_t t = (g == 0 ? va : vb) + (uint64_t)m[i];
+            t = t >= Q ? t - Q : t;
+            if (g == 0) va = t;
+            else vb = t;
+            const int idx = (h << LOGN) + i;
+            if (sizeof(W) == 4) {
+                reinterpret_cast<uint2 *>(o)[2 * idx + g] = make_uint2((uint32_t)va, (uint32_t)vb);
+            } else {
+                ulonglong2 *p = reinterpret_cast<ulonglong2 *>(o) + 4 * (size_t)idx + 2 * g;
+                p[0] = make_ulonglong2(va, a.sg(va));
+                p[1] = make_ulonglong2(vb, a.sg(vb));
+            }
+        }
+        __syncthreads();  // e0 is reused by the next column
+    }
+}
+
+// LWE key-switch key (LWE.cpp:55-106): one wave per row.
+template <typename W>
+__global__ __launch_bounds__(256) void k_ksk_gen(KskGenArgs<W> a) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + wave;  // device row ((kk * dks + j) * Bks + dd)
+    if (row >= a.rows) return;
+    const int dd = row % a.Bks;
+    const int64_t kj = row / a.Bks;
+    const int j = kj % a.dks, kk = kj / a.dks;
+    const int64_t ref_e = ((int64_t)dd * a.dks + j) * a.N + kk;
+    const uint64_t key = stream_key(a.seed, stream_id(TAG_KSK, (uint64_t)ref_e));
+    const uint64_t Q = a.A.Q;
+    W *o = a.ksk + row * a.n_pad;
+    uint64_t part = 0;
+    for (int h = lane; h < a.n; h += 64) {
+        W av = (W)__umul64hi(stream_word(key, h), Q);
+        o[h] = av;
+        uint64_t p = mulmod_generic<W>(a.A, av, a.s[h], a.sp[h]);
+        part += p;
+        if (part >= Q) part -= Q;
+    }
+    for (int h = a.n + 1 + lane; h < a.n_pad; h += 64) o[h] = 0;
+    // wave reduction mod Q
+    for (int off = 32; off > 0; off >>= 1) {
+        uint64_t o2 = __shfl_down(part, off, 64);
+        part += o2;
+        if (part >= Q) part -= Q;
+    }
+    if (lane == 0) {
+        const int ci = dd * a.dks + j;
+        uint64_t mult = mulmod_generic<W>(a.A, (W)a.z[kk], (W)a.cij[ci], (W)a.cijp[ci]);
+        int64_t e = gauss_dev(a.cdt, a.cdt_len, stream_word(key, a.n)) % (int64_t)a.E_rlwe;
+        uint64_t b = e < 0 ? mult + Q + e : mult + e;  // (e + mult) mod Q
+        b = b >= Q ? b - Q : b;
+        b += part;
+        b = b >= Q ? b - Q : b;
+        o[a.n] = (W)b;
+    }
+}
+
+// Subs key (app.cpp:4-24 with RLWE.cpp:293-322): key[i][l] = RLWE_0 with
+// NTT(z(X^(N/2^i+1))) * Bks_R^l added to b.  One workgroup per (i, l).
+template <typename W, int LOGN, int NT>
+__global__ __launch_bounds__(NT) void k_subs_gen(KeygenArgs<W> a) {
+    constexpr int N = 1 << LOGN;
+    extern __shared__ __align__(16) unsigned char smem[];
+    W *buf = reinterpret_cast<W *>(smem);
+    W *old = buf + 2 * N;
+    const int l = blockIdx.x % a.d, i = blockIdx.x / a.d;
+    const uint64_t Q = a.A.Q;
+    // substituted secret (time domain) scaled by Bks_R^l, then transformed
+    const int power = (N >> i) + 1;
+    for (int k = threadIdx.x; k < N; k += NT) {
+        const int64_t ip = (int64_t)k * power;
+        const int io = (int)(ip & (N - 1));
+        uint64_t v = a.z[k];
+        old[io] = (W)(((ip >> LOGN) & 1) ? (v ? Q - v : 0) : v);
+    }
+    __syncthreads();
+    ntt_forward<NT>(old, 1, LOGN, a.A, a.tw);
+    rlwe_enc_zero<W, LOGN, NT>(buf, a.zn, a.znp, a.seed, stream_id(TAG_SUBS, (uint64_t)i * 64) + l, a);
+    W *o = a.bsk + (size_t)(i * a.d + l) * a.rgsw_elems;
+    for (int k = threadIdx.x; k < N; k += NT) {
+        // the reference scales the NTT-domain old key by Bks_R once per l
+        W sc = mulmod_generic<W>(a.A, old[k], (W)a.bg_pow[l], (W)a.bg_powp[l]);
+        uint64_t t = (uint64_t)buf[N + k] + sc;
+        uint64_t vb = t >= Q ? t - Q : t, va = buf[k];
+        if (sizeof(W) == 4) {
+            reinterpret_cast<uint2 *>(o)[k] = make_uint2((uint32_t)va, (uint32_t)vb);
+        } else {
+            ulonglong2 *p = reinterpret_cast<ulonglong2 *>(o) + 2 * (size_t)k;
+            p[0] = make_ulonglong2(va, a.sg(va));
+            p[1] = make_ulonglong2(vb, a.sg(vb));
+        }
+    }
+}
+
+// subs key device -> reference [logN][dks_R][2][N]
+template <typename W>
+__global__ void k_subs_to_ref(const W *src, uint64_t *dst, int N, int64_t total) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t il = e / N;
+        const int k = e - il * N;
+        uint64_t va, vb;
+        if (sizeof(W) == 4) {
+            uint2 v = reinterpret_cast<const uint2 *>(src)[e];
+            va = v.x; vb = v.y;
+        } else {
+            const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(src) + 2 * e;
+            va = p[0].x; vb = p[1].x;
+        }
+        dst[il * 2 * N + k] = va;
+        dst[il * 2 * N + N + k] = vb;
+    }
+}
+
+template <typename W>
+__global__ void k_subs_from_ref(const uint64_t *src, W *dst, int N, int64_t total, ShoupGen sg) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t il = e / N;
+        const int k = e - il * N;
+        uint64_t va = src[il * 2 * N + k], vb = src[il * 2 * N + N + k];
+        if (sizeof(W) == 4) {
+            reinterpret_cast<uint2 *>(dst)[e] = make_uint2((uint32_t)va, (uint32_t)vb);
+        } else {
+            ulonglong2 *p = reinterpret_cast<ulonglong2 *>(dst) + 2 * e;
+            p[0] = make_ulonglong2(va, sg(va));
+            p[1] = make_ulonglong2(vb, sg(vb));
+        }
+    }
+}
+
+// FNV-1a style digest over 64-bit words (order dependent), one partial per block
+__global__ void k_digest(const uint64_t *w, int64_t count, uint64_t *partials) {
+    __shared__ uint64_t sh[256];
+    uint64_t h = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        h += mix64(w[i] ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(i + 1)));
+    sh[threadIdx.x] = h;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// host launchers (template dispatch on word width, log2 N, workgroup size)
+// ---------------------------------------------------------------------------
+
+#define FHEACC_LOGN_SWITCH(logN, ...)                   \
+    switch (logN) {                                      \
+        case 4: { constexpr int LOGN = 4; __VA_ARGS__; } break;   \
+        case 5: { constexpr int LOGN = 5; __VA_ARGS__; } break;   \
+        case 6: { constexpr int LOGN = 6; __VA_ARGS__; } break;   \
+        case 7: { constexpr int LOGN = 7; __VA_ARGS__; } break;   \
+        case 8: { constexpr int LOGN = 8; __VA_ARGS__; } break;   \
+        case 9: { constexpr int LOGN = 9; __VA_ARGS__; } break;   \
+        case 10: { constexpr int LOGN = 10; __VA_ARGS__; } break; \
+        case 11: { constexpr int LOGN = 11; __VA_ARGS__; } break; \
+        default: return hipErrorInvalidValue;            \
+    }
+
+template <typename W, int LOGN, int NT>
+static hipError_t launch_br_nt(const BRArgs<W> &a, int64_t B, size_t lds, hipStream_t st) {
+    auto k = k_blind_rotate<W, LOGN, NT>;
+    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3((unsigned)B), dim3(NT), lds, st, a);
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_blind_rotate(const BRArgs<W> &a, int logN, int nt, int64_t B, size_t lds, hipStream_t st) {
+    FHEACC_LOGN_SWITCH(logN, {
+        if (nt == 128 && LOGN >= 7) return launch_br_nt<W, LOGN, 128>(a, B, lds, st);
+        if (nt == 512 && LOGN >= 9) return launch_br_nt<W, LOGN, 512>(a, B, lds, st);
+        return launch_br_nt<W, LOGN, 256>(a, B, lds, st);
+    })
+    return hipSuccess;
+}
+
+template <typename W>
+hipError_t launch_lwe_keyswitch(const KSArgs<W> &a, bool gather, hipStream_t st) {
+    if (gather) {
+        dim3 grid((a.n + 1 + 255) / 256, (unsigned)a.B);
+        hipLaunchKernelGGL((k_lwe_keyswitch_gather<W, 256>), grid, dim3(256), 0, st, a);
+        return hipGetLastError();
+    }
+    constexpr int KB = sizeof(W) == 4 ? 16 : 8;
+    dim3 grid((a.n + 1 + KB - 1) / KB, (unsigned)((a.B + 255) / 256));
+    const size_t lds = ks_staged_lds<W>(a.Bks);
+    auto k = k_lwe_keyswitch_staged<W, 256, KB>;
+    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, grid, dim3(256), lds, st, a);
+    return hipGetLastError();
+}
+
+template <typename W>
+size_t ks_staged_lds(uint32_t Bks) {
+    constexpr int KB = sizeof(W) == 4 ? 16 : 8;
+    return (size_t)16 * Bks * (KB + 16 / sizeof(W)) * sizeof(W);
+}
+template size_t ks_staged_lds<uint32_t>(uint32_t);
+template size_t ks_staged_lds<uint64_t>(uint32_t);
+
+template <typename W>
+hipError_t launch_ntt(const NTTArgs<W> &a, int logN, hipStream_t st) {
+    const int64_t wgs = (a.count + a.per_wg - 1) / a.per_wg;
+    const size_t lds = (size_t)a.per_wg * sizeof(W) << logN;
+    FHEACC_LOGN_SWITCH(logN, {
+        hipLaunchKernelGGL((k_ntt<W, LOGN, 256>), dim3((unsigned)wgs), dim3(256), lds, st, a);
+    })
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_external_product(const EPArgs<W> &a, int logN, int64_t count, hipStream_t st) {
+    const size_t lds = (size_t)(2 + a.pb) * sizeof(W) << logN;
+    FHEACC_LOGN_SWITCH(logN, {
+        auto k = k_external_product<W, LOGN, 256>;
+        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(256), lds, st, a);
+    })
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_subs_keyswitch(const SubsArgs<W> &a, int logN, int64_t count, hipStream_t st) {
+    const size_t lds = (size_t)(4 + a.dch) * sizeof(W) << logN;
+    FHEACC_LOGN_SWITCH(logN, {
+        auto k = k_subs_keyswitch<W, LOGN, 256>;
+        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(256), lds, st, a);
+    })
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_rgsw_to_device(const uint64_t *src, int64_t src_stride, W *dst, int64_t dst_stride, int d, int N,
+                                 ShoupGen sg, const int64_t *map, int64_t count, hipStream_t st) {
+    int bx = (d * N + 255) / 256;
+    for (int64_t off = 0; off < count; off += 65535) {
+        int64_t cnt = count - off < 65535 ? count - off : 65535;
+        hipLaunchKernelGGL(k_rgsw_to_device<W>, dim3(bx, (unsigned)cnt), dim3(256), 0, st,
+                           map ? src : src + off * src_stride, src_stride, dst + off * dst_stride, dst_stride, d, N,
+                           sg, map ? map + off : nullptr, cnt);
+    }
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_rgsw_to_ref(const W *src, int64_t src_stride, uint64_t *dst, int64_t dst_stride, int d, int N,
+                              const int64_t *map, int64_t count, hipStream_t st) {
+    int bx = (d * N + 255) / 256;
+    for (int64_t off = 0; off < count; off += 65535) {
+        int64_t cnt = count - off < 65535 ? count - off : 65535;
+        hipLaunchKernelGGL(k_rgsw_to_ref<W>, dim3(bx, (unsigned)cnt), dim3(256), 0, st, src + off * src_stride,
+                           src_stride, dst, dst_stride, d, N, map ? map + off : nullptr, cnt);
+    }
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_ksk_convert(const uint64_t *src, W *dst, uint64_t *back, int Bks, int dks, int N, int n1,
+                              int n_pad, hipStream_t st) {
+    const int64_t rows = (int64_t)N * dks * Bks;
+    hipLaunchKernelGGL(k_ksk_convert<W>, dim3((unsigned)rows), dim3(128), 0, st, src, dst, back, Bks, dks, N, n1,
+                       n_pad);
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_keygen(const KeygenArgs<W> &a, const KskGenArgs<W> &ka, int logN, const uint64_t *z, hipStream_t st,
+                         bool subs) {
+    FHEACC_LOGN_SWITCH(logN, {
+        constexpr int N = 1 << LOGN;
+        hipLaunchKernelGGL((k_secret_ntt<W, LOGN, 256>), dim3(1), dim3(256), N * sizeof(W), st, z, (W *)a.zn,
+                           (W *)a.znp, a.A, a.tw, a.sg);
+        if (subs) {
+            size_t lds = 3 * N * sizeof(W);
+            hipLaunchKernelGGL((k_subs_gen<W, LOGN, 256>), dim3((unsigned)(a.n * a.d)), dim3(256), lds, st, a);
+        } else {
+            size_t lds = 3 * N * sizeof(W);
+            auto k = k_bsk_gen<W, LOGN, 256>;
+            hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            const int64_t wgs = (int64_t)a.n * a.digitR * (a.Br - 1) * a.d;
+            hipLaunchKernelGGL(k, dim3((unsigned)wgs), dim3(256), lds, st, a);
+            hipLaunchKernelGGL(k_ksk_gen<W>, dim3((unsigned)((ka.rows + 3) / 4)), dim3(256), 0, st, ka);
+        }
+    })
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_subs_convert(const void *src, void *dst, int N, int64_t total, bool to_ref, ShoupGen sg,
+                               hipStream_t st) {
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 65535) blocks = 65535;
+    if (to_ref) hipLaunchKernelGGL(k_subs_to_ref<W>, dim3(blocks), dim3(256), 0, st, (const W *)src, (uint64_t *)dst, N, total);
+    else hipLaunchKernelGGL(k_subs_from_ref<W>, dim3(blocks), dim3(256), 0, st, (const uint64_t *)src, (W *)dst, N, total, sg);
+    return hipGetLastError();
+}
+
+hipError_t launch_digest(const uint64_t *w, int64_t count, uint64_t *partials, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_digest, dim3(blocks), dim3(256), 0, st, w, count, partials);
+    return hipGetLastError();
+}
+
+#define FHEACC_INSTANTIATE(W)                                                                                     \
+    template hipError_t launch_blind_rotate<W>(const BRArgs<W> &, int, int, int64_t, size_t, hipStream_t);       \
+    template hipError_t launch_lwe_keyswitch<W>(const KSArgs<W> &, bool, hipStream_t);                            \
+    template hipError_t launch_ntt<W>(const NTTArgs<W> &, int, hipStream_t);                                      \
+    template hipError_t launch_external_product<W>(const EPArgs<W> &, int, int64_t, hipStream_t);                 \
+    template hipError_t launch_subs_keyswitch<W>(const SubsArgs<W> &, int, int64_t, hipStream_t);                 \
+    template hipError_t launch_rgsw_to_device<W>(const uint64_t *, int64_t, W *, int64_t, int, int, ShoupGen,     \
+                                                 const int64_t *, int64_t, hipStream_t);                          \
+    template hipError_t launch_rgsw_to_ref<W>(const W *, int64_t, uint64_t *, int64_t, int, int, const int64_t *, \
+                                              int64_t, hipStream_t);                                              \
+    template hipError_t launch_ksk_convert<W>(const uint64_t *, W *, uint64_t *, int, int, int, int, int,         \
+                                              hipStream_t);                                                       \
+    template hipError_t launch_keygen<W>(const KeygenArgs<W> &, const KskGenArgs<W> &, int, const uint64_t *,     \
+                                         hipStream_t, bool);                                                      \
+    template hipError_t launch_subs_convert<W>(const void *, void *, int, int64_t, bool, ShoupGen, hipStream_t);
+
+FHEACC_INSTANTIATE(uint32_t)
+FHEACC_INSTANTIATE(uint64_t)
+
+}  // namespace fheacc

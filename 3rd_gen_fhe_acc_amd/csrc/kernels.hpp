@@ -1,0 +1,159 @@
+// kernels.hpp -- kernel argument blocks and host launchers (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "modarith.hpp"
+#include "ntt_lds.hpp"
+
+namespace fheacc {
+
+struct ShoupGen;
+
+template <typename W>
+struct BRArgs {
+    Arith<W> A;
+    Twiddles<W> tw;
+    const uint64_t *c1, *c2;
+    const int32_t *gates;
+    const W *bsk;
+    size_t rgsw_elems;  // W elements per device RGSW
+    int n;
+    uint32_t q;
+    int digitR;
+    uint32_t Br;
+    int d, bits, pb;  // pb: digit polys transformed per LDS batch
+    uint64_t gate_constant[6];
+    uint64_t *cap_acc;
+    uint8_t *ksd;
+    uint64_t *ksb;
+    int dks;
+    uint32_t Bks;
+    int *status;
+};
+
+template <typename W>
+struct KSArgs {
+    Arith<W> A;
+    const W *ksk;
+    const uint8_t *ksd;
+    const uint64_t *ksb;
+    int n, n_pad, S;
+    uint32_t Bks;
+    uint64_t Q;
+    int lq1;
+    uint64_t *out, *cap_ks;
+    int64_t B;
+    int CB, KB, SB;
+};
+
+template <typename W>
+struct NTTArgs {
+    Arith<W> A;
+    Twiddles<W> tw;
+    uint64_t *polys;
+    int64_t count;
+    int per_wg;
+    int inverse;
+};
+
+template <typename W>
+struct EPArgs {
+    Arith<W> A;
+    Twiddles<W> tw;
+    const uint64_t *acc;
+    uint64_t *out;
+    const W *key;
+    int64_t key_stride;
+    int d, bits, pb;
+};
+
+template <typename W>
+struct SubsArgs {
+    Arith<W> A;
+    Twiddles<W> tw;
+    const uint64_t *in;
+    uint64_t *out;
+    const W *key;
+    int64_t key_stride;
+    int sub_factor;
+    int d, bits, dch;
+};
+
+struct ShoupGen {
+    uint64_t Q, a, b, bp;
+    __host__ __device__ ShoupGen() : Q(0), a(0), b(0), bp(0) {}
+    __device__ __forceinline__ uint64_t operator()(uint64_t k) const {
+        uint64_t qh = __umul64hi(k, bp);
+        uint64_t r = k * b - qh * Q;
+        if (r >= Q) qh++;
+        return k * a + qh;
+    }
+};
+
+template <typename W>
+struct KeygenArgs {
+    Arith<W> A;
+    Twiddles<W> tw;
+    uint64_t seed;
+    const uint64_t *cdt;
+    int cdt_len;
+    uint64_t E_rlwe;
+    W *zn, *znp;
+    W *bsk;              // bootstrap key, or subs key in subs mode
+    size_t rgsw_elems;   // per device RGSW (subs mode: per (i, l) RLWE key)
+    int d;               // digitG (subs mode: digitKS_R)
+    int n;               // LWE n (subs mode: log2 N)
+    int digitR;
+    uint32_t Br, q;
+    const uint64_t *lwe_sk;
+    const uint64_t *z;                 // RLWE secret, time domain
+    const uint64_t *bg_pow, *bg_powp;  // BG^h (subs mode: Bks_R^l) mod Q
+    ShoupGen sg;
+};
+
+template <typename W>
+struct KskGenArgs {
+    Arith<W> A;
+    uint64_t seed;
+    const uint64_t *cdt;
+    int cdt_len;
+    uint64_t E_rlwe;
+    W *ksk;
+    int64_t rows;
+    int n, n_pad, N, dks, Bks;
+    const W *s, *sp;
+    const uint64_t *z;
+    const uint64_t *cij, *cijp;
+};
+
+template <typename W>
+hipError_t launch_blind_rotate(const BRArgs<W> &a, int logN, int nt, int64_t B, size_t lds, hipStream_t st);
+template <typename W>
+hipError_t launch_lwe_keyswitch(const KSArgs<W> &a, bool gather, hipStream_t st);
+template <typename W>
+size_t ks_staged_lds(uint32_t Bks);  // LDS bytes of the staged key switch
+template <typename W>
+hipError_t launch_ntt(const NTTArgs<W> &a, int logN, hipStream_t st);
+template <typename W>
+hipError_t launch_external_product(const EPArgs<W> &a, int logN, int64_t count, hipStream_t st);
+template <typename W>
+hipError_t launch_subs_keyswitch(const SubsArgs<W> &a, int logN, int64_t count, hipStream_t st);
+template <typename W>
+hipError_t launch_rgsw_to_device(const uint64_t *src, int64_t src_stride, W *dst, int64_t dst_stride, int d, int N,
+                                 ShoupGen sg, const int64_t *map, int64_t count, hipStream_t st);
+template <typename W>
+hipError_t launch_rgsw_to_ref(const W *src, int64_t src_stride, uint64_t *dst, int64_t dst_stride, int d, int N,
+                              const int64_t *map, int64_t count, hipStream_t st);
+template <typename W>
+hipError_t launch_ksk_convert(const uint64_t *src, W *dst, uint64_t *back, int Bks, int dks, int N, int n1,
+                              int n_pad, hipStream_t st);
+template <typename W>
+hipError_t launch_keygen(const KeygenArgs<W> &a, const KskGenArgs<W> &ka, int logN, const uint64_t *z, hipStream_t st,
+                         bool subs);
+template <typename W>
+hipError_t launch_subs_convert(const void *src, void *dst, int N, int64_t total, bool to_ref, ShoupGen sg,
+                               hipStream_t st);
+hipError_t launch_digest(const uint64_t *w, int64_t count, uint64_t *partials, int blocks, hipStream_t st);
+
+}  // namespace fheacc

@@ -1,0 +1,82 @@
+// modarith.hpp -- modular arithmetic on the integer VALU of gfx950.
+//
+// Two word widths share every kernel through the Arith<W> policy:
+//   Arith<uint32_t>  "narrow", Q < 2^30 (27-bit sets): one VGPR per word,
+//                    Shoup products from v_mul_hi_u32/v_mul_lo_u32 and 64-bit
+//                    lazy sums from v_mad_u64_u32.
+//   Arith<uint64_t>  "wide", Q < 2^62 (54-bit sets): Shoup products built on
+//                    __umul64hi.
+// Every result that leaves a kernel is canonical in [0, Q), the reference's
+// representation (FHE_ACC_runtime/include/nbt.h:59-101); intermediate values
+// are kept lazily in [0, 2Q) or [0, 4Q) (Harvey's butterflies), which is exact
+// arithmetic mod Q and so does not change any canonical output.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fheacc {
+
+template <typename W>
+struct Arith;
+
+template <>
+struct Arith<uint32_t> {
+    using W = uint32_t;
+    W Q, Q2;
+    uint64_t mu;  // floor(2^64 / Q), Barrett constant for 64-bit sums
+
+    // [0, 2m) -> [0, m): if x < m the subtraction wraps above x
+    __device__ __forceinline__ static W red(W x, W m) {
+        W y = x - m;
+        return y < x ? y : x;
+    }
+    // x * w mod Q in [0, 2Q) for any 32-bit x; wp = floor(w * 2^32 / Q)
+    __device__ __forceinline__ W mul_shoup(W x, W w, W wp) const {
+        W qh = __umulhi(x, wp);
+        return x * w - qh * Q;
+    }
+    // 64-bit sum -> [0, Q)
+    __device__ __forceinline__ W reduce64(uint64_t x) const {
+        uint64_t qh = __umul64hi(x, mu);
+        W r = (W)x - (W)qh * Q;  // true remainder in [0, 2Q), exact mod 2^32
+        return red(r, Q);
+    }
+    __device__ __forceinline__ static uint64_t mac(uint64_t acc, W a, W b) {
+        return acc + (uint64_t)a * b;
+    }
+};
+
+template <>
+struct Arith<uint64_t> {
+    using W = uint64_t;
+    W Q, Q2;
+    uint64_t mu;  // unused for the wide path (sums are reduced from [0, 2dQ))
+
+    __device__ __forceinline__ static W red(W x, W m) {
+        W y = x - m;
+        return y < x ? y : x;
+    }
+    __device__ __forceinline__ W mul_shoup(W x, W w, W wp) const {
+        W qh = __umul64hi(x, wp);
+        return x * w - qh * Q;
+    }
+    // x < 2^62, any multiple of Q below that -> [0, Q)
+    __device__ __forceinline__ W reduce_small(uint64_t x) const {
+        uint64_t qh = (uint64_t)((double)x / (double)Q);
+        int64_t r = (int64_t)(x - qh * Q);
+        if (r < 0) r += (int64_t)Q;
+        if (r >= (int64_t)Q) r -= (int64_t)Q;
+        if (r < 0) r += (int64_t)Q;
+        return (W)r;
+    }
+};
+
+// Host-side constant derivation (Shoup companions, Barrett mu).
+static inline uint32_t shoup32(uint32_t w, uint32_t Q) {
+    return (uint32_t)(((unsigned __int128)w << 32) / Q);
+}
+static inline uint64_t shoup64(uint64_t w, uint64_t Q) {
+    return (uint64_t)(((unsigned __int128)w << 64) / Q);
+}
+
+}  // namespace fheacc

@@ -1,0 +1,158 @@
+// fhe_acc.cpp -- the reference-shaped C++ host API over the C ABI.
+#include "fhe_acc.hpp"
+
+#include <stdio.h>
+#include <stdlib.h>
+
+namespace fheacc {
+
+void check(int rc) {
+    if (rc != 0) throw Error(rc, std::string("fheacc: ") + fheacc_last_error());
+}
+
+Context::Context(const Params &p, int device) : p_(p) {
+    fheacc_params cp{p.n, p.q, p.N, p.Q, p.nbit, p.BGbits, p.Br, p.Bks, p.Bks_R_bits, p.stddev, p.E_rlwe};
+    check(fheacc_ctx_create(&cp, device, &h_));
+    check(fheacc_ctx_info(h_, &info_));
+}
+
+Context::~Context() { fheacc_ctx_destroy(h_); }
+
+KeySet::KeySet(const Context &ctx) : ctx_(ctx) { check(fheacc_keys_create(ctx.handle(), &h_)); }
+
+KeySet::~KeySet() { fheacc_keys_destroy(h_); }
+
+}  // namespace fheacc
+
+namespace LWE {
+
+LWE_secretkey::LWE_secretkey(const fheacc::Context &ctx, uint64_t seed, bool binary) {
+    a.assign(ctx.n(), 0);
+    fheacc::check(fheacc_lwe_secret(ctx.handle(), seed, binary ? 1 : 0, a.data()));
+    b = 1;
+}
+
+keyswitch_key::keyswitch_key(const RLWE::bootstrap_key &bt) : keys(bt.keys) {}
+
+LWE_ciphertext LWE_encrypt(const fheacc::Context &ctx, const LWE_secretkey &s, const LWE_plaintext &p, uint64_t seed,
+                           uint64_t index) {
+    std::vector<ciphertext_t> w(ctx.n() + 1);
+    uint8_t bit = (uint8_t)p.m;
+    fheacc::check(fheacc_lwe_encrypt(ctx.handle(), s.a.data(), &bit, 1, seed, index, w.data()));
+    LWE_ciphertext c;
+    c.a.assign(w.begin(), w.begin() + ctx.n());
+    c.b = w[ctx.n()];
+    return c;
+}
+
+LWE_plaintext LWE_decrypt(const fheacc::Context &ctx, const LWE_secretkey &s, const LWE_ciphertext &c) {
+    std::vector<ciphertext_t> w(c.a);
+    w.push_back(c.b);
+    uint8_t bit = 0;
+    fheacc::check(fheacc_lwe_decrypt(ctx.handle(), s.a.data(), w.data(), 1, &bit));
+    return LWE_plaintext(bit);
+}
+
+// LWE.cpp:203-222
+LWE_ciphertext LWE_evaluate(const fheacc::Context &ctx, const LWE_ciphertext &c1, const LWE_ciphertext &c2,
+                            GATES gate) {
+    const ciphertext_t q = ctx.q();
+    LWE_ciphertext r(ctx.n());
+    const bool x = gate == XOR || gate == XNOR;
+    for (int i = 0; i < ctx.n(); i++) r.a[i] = (x ? 2 * (c1.a[i] - c2.a[i]) : c1.a[i] + c2.a[i]) % q;
+    r.b = (x ? 2 * (c1.b - c2.b) : c1.b + c2.b) % q;
+    return r;
+}
+
+// LWE.cpp:224-231
+LWE_ciphertext LWE_eval_NOT(const fheacc::Context &ctx, const LWE_ciphertext &c) {
+    const ciphertext_t q = ctx.q();
+    LWE_ciphertext r(ctx.n());
+    for (int i = 0; i < ctx.n(); i++) r.a[i] = (q - c.a[i]) % q;
+    r.b = ((q / 4) - c.b) % q;
+    return r;
+}
+
+}  // namespace LWE
+
+namespace RLWE {
+
+RLWE_secretkey::RLWE_secretkey(const fheacc::Context &ctx, uint64_t seed) {
+    a.assign(ctx.N(), 0);
+    fheacc::check(fheacc_rlwe_secret(ctx.handle(), seed, a.data()));
+}
+
+bootstrap_key::bootstrap_key(const fheacc::Context &ctx, const LWE::LWE_secretkey &s, const RLWE_secretkey &z,
+                             uint64_t seed)
+    : keys(std::make_shared<fheacc::KeySet>(ctx)) {
+    fheacc::check(fheacc_keys_generate(keys->handle(), seed, s.a.data(), z.a.data()));
+}
+
+bootstrap_key::bootstrap_key(const fheacc::Context &ctx, const std::vector<ciphertext_t> &bt,
+                             const std::vector<ciphertext_t> &ks)
+    : keys(std::make_shared<fheacc::KeySet>(ctx)) {
+    if (bt.size() != ctx.info().bsk_words || ks.size() != ctx.info().ksk_words)
+        throw fheacc::Error(-1, "key sizes do not match the parameter set");
+    fheacc::check(fheacc_keys_upload_bsk(keys->handle(), bt.data()));
+    fheacc::check(fheacc_keys_upload_ksk(keys->handle(), ks.data()));
+}
+
+LWE::LWE_ciphertext eval_bootstrap(const LWE::LWE_ciphertext &c1, const LWE::LWE_ciphertext &c2,
+                                   const bootstrap_key &btkey, const LWE::keyswitch_key &kskey, const GATES gate) {
+    if (c1 == c2) {
+        // RLWE.cpp:727-730
+        printf("Please only use independent ciphertexts!!!\n");
+        exit(1);
+    }
+    if (!btkey.keys || btkey.keys != kskey.keys) throw fheacc::Error(-1, "bootstrap and key-switch keys differ");
+    int rc = 0;
+    std::vector<LWE::LWE_ciphertext> r = fheacc::eval_bootstrap_batch({c1}, {c2}, kskey, {gate}, &rc);
+    fheacc::check(rc);
+    return r[0];
+}
+
+}  // namespace RLWE
+
+namespace fheacc {
+
+std::vector<LWE::LWE_ciphertext> eval_bootstrap_batch(const std::vector<LWE::LWE_ciphertext> &c1,
+                                                      const std::vector<LWE::LWE_ciphertext> &c2,
+                                                      const LWE::keyswitch_key &kskey, const std::vector<GATES> &gates,
+                                                      int *rc) {
+    *rc = 0;
+    if (!kskey.keys || c1.size() != c2.size() || c1.size() != gates.size()) {
+        *rc = -1;
+        return std::vector<LWE::LWE_ciphertext>(c1.size());
+    }
+    const Context &ctx = kskey.keys->context();
+    const int n = ctx.n();
+    const size_t B = c1.size(), lw = (size_t)n + 1;
+    std::vector<ciphertext_t> w1(B * lw), w2(B * lw), out(B * lw);
+    std::vector<int32_t> g(B);
+    for (size_t i = 0; i < B; i++) {
+        if ((int)c1[i].a.size() != n || (int)c2[i].a.size() != n) {
+            *rc = -1;
+            return std::vector<LWE::LWE_ciphertext>(B);
+        }
+        std::copy(c1[i].a.begin(), c1[i].a.end(), w1.begin() + i * lw);
+        w1[i * lw + n] = c1[i].b;
+        std::copy(c2[i].a.begin(), c2[i].a.end(), w2.begin() + i * lw);
+        w2[i * lw + n] = c2[i].b;
+        g[i] = (int32_t)gates[i];
+    }
+    int r = fheacc_bootstrap_batch(ctx.handle(), kskey.keys->handle(), g.data(), w1.data(), w2.data(), out.data(),
+                                   (int64_t)B, nullptr);
+    std::vector<LWE::LWE_ciphertext> res(B);
+    if (r != 0) {
+        fprintf(stderr, "eval_bootstrap_batch: %s\n", fheacc_last_error());
+        *rc = r;
+        return res;
+    }
+    for (size_t i = 0; i < B; i++) {
+        res[i].a.assign(out.begin() + i * lw, out.begin() + i * lw + n);
+        res[i].b = out[i * lw + n];
+    }
+    return res;
+}
+
+}  // namespace fheacc

@@ -1,0 +1,180 @@
+// fhe_acc.hpp -- C++ host API over libfheacc.so, shaped like the reference's
+// host API so that its callers (FHE_ACC_runtime/apps/*.cpp, src/test.cpp)
+// port by swapping includes and passing a Context where the reference reads
+// its global parameter block.
+//
+//   reference                                   here
+//   params.h globals n, q, N, Q, BG, Br, Bks... fheacc::Context(Params)
+//   LWE::LWE_secretkey(RNG_uniform&)            LWE::LWE_secretkey(ctx, seed, binary)
+//   RLWE::RLWE_secretkey(RNG_uniform&)          RLWE::RLWE_secretkey(ctx, seed)
+//   RLWE::bootstrap_key(s, z, uni, norm)        RLWE::bootstrap_key(ctx, s, z, seed)
+//   LWE::keyswitch_key(z, s, uni, norm)         LWE::keyswitch_key(btkey) (same device set)
+//   LWE::LWE_encrypt(s, p, uni, norm)           LWE::LWE_encrypt(ctx, s, p, seed, index)
+//   LWE::LWE_decrypt(s, c)                      LWE::LWE_decrypt(ctx, s, c)
+//   LWE::LWE_evaluate / LWE_eval_NOT            same names (host, LWE.cpp:203-231)
+//   RLWE::eval_bootstrap(c1, c2, bt, ks, gate)  same name and arguments
+//   FPGA::fpga_eval_bootstrap_x{4,8,12,16}      fheacc::eval_bootstrap_batch (any B, int* rc)
+//
+// Keys live on the device; the classes hold shared handles.  eval_bootstrap
+// on identical operands prints the reference's message and exits the process
+// as RLWE.cpp:727-730 does; the batched call reports it through *rc = -1
+// like the accelerator driver (myfpga.cpp:1011-1022).
+#pragma once
+#include <stdint.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/fheacc.h"
+
+typedef uint64_t message_t;
+typedef uint64_t ciphertext_t;
+
+// params.h:107
+enum GATES { OR, AND, NOR, NAND, XOR, XNOR };
+
+namespace fheacc {
+
+struct Params {
+    int n = 500;
+    ciphertext_t q = 1024;
+    int N = 1024;
+    ciphertext_t Q = 0;  // 0: previous_prime(first_prime(nbit, N), N)
+    int nbit = 27;
+    uint32_t BGbits = 7;
+    ciphertext_t Br = 32;
+    ciphertext_t Bks = 32;
+    uint32_t Bks_R_bits = 9;
+    double stddev = 3.19;
+    ciphertext_t E_rlwe = 18;
+};
+
+class Error : public std::runtime_error {
+   public:
+    Error(int rc, const std::string &what) : std::runtime_error(what), rc(rc) {}
+    int rc;
+};
+
+class Context {
+   public:
+    explicit Context(const Params &p, int device = 0);
+    ~Context();
+    Context(const Context &) = delete;
+    Context &operator=(const Context &) = delete;
+    fheacc_ctx *handle() const { return h_; }
+    const fheacc_info &info() const { return info_; }
+    const Params &params() const { return p_; }
+    int n() const { return p_.n; }
+    ciphertext_t q() const { return p_.q; }
+    int N() const { return p_.N; }
+    ciphertext_t Q() const { return info_.Q; }
+
+   private:
+    fheacc_ctx *h_ = nullptr;
+    Params p_;
+    fheacc_info info_{};
+};
+
+// A client's device-resident bootstrap + key-switch keys.
+class KeySet {
+   public:
+    explicit KeySet(const Context &ctx);
+    ~KeySet();
+    KeySet(const KeySet &) = delete;
+    KeySet &operator=(const KeySet &) = delete;
+    fheacc_keys *handle() const { return h_; }
+    const Context &context() const { return ctx_; }
+
+   private:
+    const Context &ctx_;
+    fheacc_keys *h_ = nullptr;
+};
+
+void check(int rc);
+
+}  // namespace fheacc
+
+namespace RLWE {
+class RLWE_secretkey;
+class bootstrap_key;
+}  // namespace RLWE
+
+namespace LWE {
+
+class LWE_plaintext {
+   public:
+    LWE_plaintext() : m(0) {}
+    LWE_plaintext(message_t m_in) : m(m_in) {}
+    message_t m;
+};
+
+class LWE_ciphertext {
+   public:
+    LWE_ciphertext() = default;
+    explicit LWE_ciphertext(int n) : a(n, 0), b(0) {}
+    bool operator==(const LWE_ciphertext &o) const { return b == o.b && a == o.a; }
+    std::vector<ciphertext_t> a;
+    ciphertext_t b = 0;
+};
+
+class LWE_secretkey : public LWE_ciphertext {
+   public:
+    LWE_secretkey() = default;
+    // seeded (the reference draws from std::random_device, RNG.cpp:5-6)
+    LWE_secretkey(const fheacc::Context &ctx, uint64_t seed, bool binary = true);
+};
+
+class keyswitch_key {
+   public:
+    keyswitch_key() = default;
+    // the key-switch key is generated with, and lives beside, the bootstrap key
+    explicit keyswitch_key(const RLWE::bootstrap_key &bt);
+    std::shared_ptr<fheacc::KeySet> keys;
+};
+
+LWE_ciphertext LWE_encrypt(const fheacc::Context &ctx, const LWE_secretkey &s, const LWE_plaintext &p, uint64_t seed,
+                           uint64_t index);
+LWE_plaintext LWE_decrypt(const fheacc::Context &ctx, const LWE_secretkey &s, const LWE_ciphertext &c);
+LWE_ciphertext LWE_evaluate(const fheacc::Context &ctx, const LWE_ciphertext &c1, const LWE_ciphertext &c2,
+                            GATES gate);
+LWE_ciphertext LWE_eval_NOT(const fheacc::Context &ctx, const LWE_ciphertext &c);
+
+}  // namespace LWE
+
+namespace RLWE {
+
+class RLWE_secretkey {
+   public:
+    RLWE_secretkey() = default;
+    RLWE_secretkey(const fheacc::Context &ctx, uint64_t seed);
+    std::vector<ciphertext_t> a;  // time domain, mod Q
+};
+
+class bootstrap_key {
+   public:
+    bootstrap_key() = default;
+    // Generates btkey (RLWE.cpp:579-606) and the LWE key-switch key
+    // (LWE.cpp:55-106) on the device from a seed.
+    bootstrap_key(const fheacc::Context &ctx, const LWE::LWE_secretkey &s, const RLWE_secretkey &z, uint64_t seed);
+    // Uploads keys in the reference layouts (flattened btkey / kskey).
+    bootstrap_key(const fheacc::Context &ctx, const std::vector<ciphertext_t> &btkey_flat,
+                  const std::vector<ciphertext_t> &kskey_flat);
+    std::shared_ptr<fheacc::KeySet> keys;
+};
+
+LWE::LWE_ciphertext eval_bootstrap(const LWE::LWE_ciphertext &c1, const LWE::LWE_ciphertext &c2,
+                                   const bootstrap_key &btkey, const LWE::keyswitch_key &kskey, const GATES gate);
+
+}  // namespace RLWE
+
+namespace fheacc {
+
+// FPGA::fpga_eval_bootstrap_x{4,8,12,16} with any batch size; *rc = 0, -1, -ENOMEM.
+std::vector<LWE::LWE_ciphertext> eval_bootstrap_batch(const std::vector<LWE::LWE_ciphertext> &c1,
+                                                      const std::vector<LWE::LWE_ciphertext> &c2,
+                                                      const LWE::keyswitch_key &kskey, const std::vector<GATES> &gates,
+                                                      int *rc);
+
+}  // namespace fheacc

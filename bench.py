@@ -1,0 +1,230 @@
+"""Benchmark: batched gate bootstraps on one MI355X.
+
+Workload (one step): a batch of 4096 seeded NAND gate bootstraps with the
+benchmark parameter set -- LWE n=500, q=1024, binary LWE secret; RLWE N=1024,
+ternary secret, Q = 0x7FFF801 (27-bit, = 1 mod 2048); gadget base 2^7 (d=4);
+blind-rotation radix Br=32 (digitR=2); key-switch base 2^5 (digitKS=6);
+sigma 3.19 -- i.e. gate pre-combination, accumulator init, blind rotation,
+LWE key switch and modulus switch for every pair, with inputs and keys
+already resident in HBM.  Keys come from the library's seeded device key
+generator.
+
+Also measured: the same gate bootstrap one ciphertext per launch (the plain
+form), its p50/p99 latency, per-kernel HIP-event times, and a CPU baseline on
+a bounded sample (the reference's own eval_bootstrap compiled from its sources
+when oracle/_ref was built, else the oracle's C restatement).
+
+Prints one JSON line.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "bootstraps/sec (N=1024, batch 4096) vs 1-per-launch; p50/p99 gate latency"
+PARAMS = dict(n=500, q=1024, N=1024, Q=0, nbit=27, BGbits=7, Br=32, Bks=32, Bks_R_bits=9)
+SEED = 20260101
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--single", type=int, default=24, help="one-per-launch gate bootstraps to time")
+    ap.add_argument("--cpu-sample", type=int, default=6, help="gates in the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tuning", default="auto",
+                    help="'auto' (run the autotuner) or THREADS,POLYS_PER_BATCH for the blind-rotation kernel")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device-to-device (reference dispatch for HBM byte counters)")
+    return ap.parse_args()
+
+
+def cpu_baseline(ctx, keys, c1, c2, gates, gpu_out, sample):
+    """Time the CPU path on `sample` gates (single thread), bit-compare with the GPU."""
+    import oracle
+    from oracle import refbind
+
+    bsk = keys.download_bsk()
+    ksk = keys.download_ksk()
+    c1s, c2s, gs = c1[:sample], c2[:sample], gates[:sample]
+    if refbind.available():
+        ref = refbind.Reference(ctx.n, ctx.q, ctx.N, ctx.Q, PARAMS["BGbits"], PARAMS["Br"], PARAMS["Bks"])
+        t0 = time.perf_counter()
+        ref.load_keys(bsk, ksk)
+        load_s = time.perf_counter() - t0
+        out, secs = ref.bootstrap(c1s, c2s, gs)
+        kind = "reference"
+        what = "RLWE::eval_bootstrap (FHE_ACC_runtime/src/RLWE.cpp:726-786) compiled from the reference sources"
+    else:
+        o = oracle.Oracle(**PARAMS)
+        load_s = 0.0
+        t0 = time.perf_counter()
+        out = o.bootstrap(c1s, c2s, gs, bsk, ksk, threads=1)
+        secs = time.perf_counter() - t0
+        kind = "port"
+        what = "oracle/fhe_oracle.c restatement of eval_bootstrap"
+    return {
+        "value": sample / secs,
+        "unit": "bootstraps/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": "%d NAND gate bootstraps of the benchmark batch, single thread, %s; key load %.1f s untimed"
+                  % (sample, what, load_s),
+        "seconds": secs,
+        "bit_exact_vs_gpu": bool(np.array_equal(out, gpu_out[:sample])),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    fhe = importlib.import_module("3rd_gen_fhe_acc_amd")
+    torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.empty(1 << 27, dtype=torch.int64, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    B = args.batch
+    ctx = fhe.Context(**PARAMS)
+    keys = ctx.keys()
+    s = ctx.lwe_secret(SEED, binary=True)
+    z = ctx.rlwe_secret(SEED)
+    t0 = time.perf_counter()
+    keys.generate(SEED, s, z)
+    ctx.sync()
+    keygen_s = time.perf_counter() - t0
+    if args.tuning == "auto":
+        tune_report = ctx.autotune(keys, min(B, 2048))
+    else:
+        th, pb = (int(x) for x in args.tuning.split(","))
+        ctx.set_tuning(th, pb)
+        tune_report = {"chosen": {"threads": th, "polys_per_batch": pb}, "source": "--tuning"}
+    rng = np.random.default_rng(SEED)
+    m1 = rng.integers(0, 2, B).astype(np.uint8)
+    m2 = rng.integers(0, 2, B).astype(np.uint8)
+    c1 = ctx.lwe_encrypt(s, m1, SEED, 0)
+    c2 = ctx.lwe_encrypt(s, m2, SEED, B)
+    gates = np.full(B, fhe.GATES["NAND"], dtype=np.int32)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a).view(np.int64) if a.dtype == np.uint64 else a).cuda()
+
+    d1, d2, dg = dev(c1), dev(c2), dev(gates)
+    dout = torch.zeros_like(d1)
+    for _ in range(args.warmup):
+        ctx.bootstrap_dev(keys, dg, d1, d2, dout)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(args.steps):
+        ctx.bootstrap_dev(keys, dg, d1, d2, dout)
+    ev1.record()
+    torch.cuda.synchronize()
+    ctx.sync()
+    kms = ctx.kernel_ms()
+    ctx.profile(False)
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    value = B / (ms_step / 1e3)
+    out = dout.cpu().numpy().view(np.uint64)
+    bits = ctx.lwe_decrypt(s, out)
+    truth_ok = bool(np.array_equal(bits, 1 - (m1 & m2)))
+
+    # one ciphertext per launch
+    lat = []
+    for i in range(args.single + 2):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ctx.bootstrap_dev(keys, dg[i:i + 1], d1[i:i + 1], d2[i:i + 1], dout[i:i + 1])
+        e1.record()
+        torch.cuda.synchronize()
+        if i >= 2:
+            lat.append(e0.elapsed_time(e1))
+    ctx.sync()
+    lat = np.array(lat)
+    single_rate = 1e3 / lat.mean()
+
+    # roofline of the dominant kernel (blind rotation), algorithmic bytes
+    info = ctx.info
+    n, N, dks = PARAMS["n"], PARAMS["N"], ctx.digitKS
+    br_ms, br_n = kms["blind_rotate"]
+    ks_ms, ks_n = kms["lwe_keyswitch"]
+    br_avg = br_ms / max(br_n, 1)
+    ks_avg = ks_ms / max(ks_n, 1)
+    br_bytes = info["bsk_device_bytes"] + 2 * B * (n + 1) * 8 + B * 4 + B * N * dks + B * 8
+    achieved = br_bytes / (br_avg * 1e-3) / 1e9
+    nega = (PARAMS["q"] - ((c1[:, :n] + c2[:, :n]) % PARAMS["q"])) % PARAMS["q"]
+    ext_products = int(np.count_nonzero(nega % 32) + np.count_nonzero((nega // 32) % 32))
+    d = ctx.digitG
+    modmul_per_ep = 2 * d * (N // 2) * 10 + 2 * (N // 2) * 10 + 2 * N + 4 * d * N
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_blind_rotate_n1024_b4096.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    rec = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "bootstraps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "batched NAND gate bootstrap, cfg-0 parameters (n=500 q=1024 N=1024 "
+                               "Q=0x7FFF801 BG=2^7 d=4 Br=32 Bks=2^5), batch %d" % B,
+                   "global_batch": B, "parallelism": "single GPU", "Q": ctx.Q, "digitG": d,
+                   "digitR": ctx.digitR, "digitKS": dks},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "kernel": "k_blind_rotate", "avg_launch_ms": round(br_avg, 4),
+                     "algorithmic_bytes_per_launch": br_bytes},
+        "valu": {"kernel": "k_blind_rotate", "external_products_per_launch": ext_products,
+                 "modmul_per_external_product": modmul_per_ep,
+                 "modmul_per_s": round(ext_products * modmul_per_ep / (br_avg * 1e-3) / 1e12, 4),
+                 "unit": "T modmul/s"},
+        "kernels_ms_avg": {"blind_rotate": round(br_avg, 4), "lwe_keyswitch": round(ks_avg, 4)},
+        "single_per_launch": {"bootstraps_per_s": round(single_rate, 2),
+                              "latency_ms_p50": round(float(np.percentile(lat, 50)), 4),
+                              "latency_ms_p99": round(float(np.percentile(lat, 99)), 4),
+                              "samples": int(lat.size)},
+        "speedup_vs_single_per_launch": round(value / single_rate, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                       "p99": round(float(np.percentile(lat, 99)), 4),
+                       "note": "single gate, idle GPU (one ciphertext per launch)"},
+        "batch_latency_ms": round(ms_step, 4),
+        "decrypt_truth_table_ok": truth_ok,
+        "keygen_s": round(keygen_s, 3),
+        "tuning": tune_report,
+    }
+    if not args.no_cpu_baseline and args.cpu_sample > 0:
+        rec["cpu_baseline"] = cpu_baseline(ctx, keys, c1, c2, gates, out, args.cpu_sample)
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+/*
+ * fheacc.h -- C ABI of the MI355X gate-bootstrapping library (libfheacc.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int return codes
+ * (0 = ok, -1 = failure, -ENOMEM = allocation failure, the convention of the
+ * reference's accelerator driver, FHE_ACC_runtime/src/myfpga.cpp:1011-1022).
+ * On failure fheacc_last_error() describes the cause (thread-local).
+ *
+ * Word layouts (uint64 words, canonical in [0, modulus), the reference's
+ * std::vector<uint64_t> contents):
+ *   LWE ciphertext   [n+1]             a[0..n-1] then b         (LWE.h:36-64)
+ *   RLWE ciphertext  [2][N]            a then b, NTT domain     (RLWE.h:127-159)
+ *   RGSW ciphertext  [digitG][2][2][N] c_text[h][g].{a,b}       (RLWE.h:313-329)
+ *   bootstrap key    [Br][digitR][n] RGSW = btkey[r][j][k]      (RLWE.cpp:579-606)
+ *   LWE ks key       [Bks][digitKS][N] LWE(n) mod Q = kskey[d][j][i] (LWE.cpp:55-106)
+ *   RLWE ks key      [digitKS_R] RLWE = RLWE_kskey[l]           (RLWE.cpp:293-322)
+ *   subs key         [log2 N][digitKS_R] RLWE = subs_key[i]     (app.cpp:4-24)
+ * Gates follow enum GATES {OR, AND, NOR, NAND, XOR, XNOR} (params.h:107).
+ *
+ * Functions named *_dev take device pointers and are asynchronous on the
+ * context's stream; the others take host pointers and return after the
+ * results are in host memory.
+ */
+#ifndef FHEACC_H
+#define FHEACC_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FHEACC_OK 0
+#define FHEACC_ERR (-1)
+#define FHEACC_ENOMEM (-12)
+
+typedef struct fheacc_ctx fheacc_ctx;
+typedef struct fheacc_keys fheacc_keys;
+
+/* Parameter set.  Replaces the global block every app assigns
+ * (FHE_ACC_runtime/include/params.h:13-134, apps/fhe_wo_fpga.cpp:35-162). */
+typedef struct fheacc_params {
+    int32_t  n;           /* LWE dimension */
+    uint64_t q;           /* LWE modulus: power of two dividing 2N */
+    int32_t  N;           /* ring degree: power of two, 16..2048 */
+    uint64_t Q;           /* RLWE prime, Q = 1 mod 2N; 0 = derive from nbit */
+    int32_t  nbit;        /* Q = previous_prime(first_prime(nbit, N), N) when Q == 0 */
+    uint32_t BGbits;      /* gadget base 2^BGbits */
+    uint64_t Br;          /* blind-rotation radix */
+    uint64_t Bks;         /* RLWE->LWE key-switch base, 2..256 */
+    uint32_t Bks_R_bits;  /* RLWE->RLWE key-switch base 2^Bks_R_bits */
+    double   stddev;      /* key-generation Gaussian sigma (3.19) */
+    uint64_t E_rlwe;      /* key-generation noise bound (18) */
+} fheacc_params;
+
+/* Derived values (params.h globals computed as the apps compute them). */
+typedef struct fheacc_info {
+    uint64_t Q, iN, ROU;
+    uint32_t digitG, digitR, digitKS, digitKS_R;
+    uint32_t narrow;        /* 1: 32-bit device words (Q < 2^30) */
+    uint64_t gate_constant[6];
+    uint64_t bsk_words;     /* words of the reference-layout bootstrap key */
+    uint64_t ksk_words;     /* words of the reference-layout LWE ks key */
+    uint64_t subs_words;    /* words of the reference-layout subs key */
+    uint64_t bsk_device_bytes, ksk_device_bytes, subs_device_bytes;
+} fheacc_info;
+
+/* Capture buffers for intermediate words (each may be NULL).  Host or device
+ * memory to match the call. */
+typedef struct fheacc_capture {
+    uint64_t *acc;   /* [B][2][N] accumulator after blind rotation (NTT domain) */
+    uint64_t *ks;    /* [B][n+1] LWE mod Q after the key switch */
+} fheacc_capture;
+
+/* ---- context ---------------------------------------------------------
+ * Replaces the parameter derivation of the apps (Q, iN, ROU, tables:
+ * fhe_wo_fpga.cpp:89-113, nbt.cpp:209-346, params.cpp:149-175) and the
+ * accelerator configuration FPGA::OCL_config_wr_rd + BAR1_ROU_table_*_wr
+ * (myfpga.h:141-180, myfpga.cpp:82-594). */
+int fheacc_ctx_create(const fheacc_params *params, int device, fheacc_ctx **out);
+int fheacc_ctx_destroy(fheacc_ctx *ctx);
+int fheacc_ctx_info(const fheacc_ctx *ctx, fheacc_info *out);
+/* Launch on a caller-owned hipStream_t from now on (NULL: the null stream).
+ * A new context launches on a stream of its own. */
+int fheacc_ctx_set_stream(fheacc_ctx *ctx, void *hip_stream);
+int fheacc_sync(fheacc_ctx *ctx);
+const char *fheacc_last_error(void);
+/* Per-kernel timing with HIP events on the launch stream (the role of the
+ * reference's chrono prints around each phase, myfpga.cpp:1431-1525).
+ * enable != 0 starts recording and clears the totals; fheacc_ctx_kernel_ms
+ * synchronizes and returns total milliseconds and launch counts per kernel
+ * class: [0] blind rotation, [1] LWE key switch, [2] external product,
+ * [3] NTT. */
+int fheacc_ctx_profile(fheacc_ctx *ctx, int enable);
+int fheacc_ctx_kernel_ms(fheacc_ctx *ctx, double *ms4, int64_t *launches4);
+
+/* Launch shapes of the blind-rotation kernel: threads per workgroup (128,
+ * 256 or 512) and digit polys per LDS batch (0: as many as fit).  No
+ * counterpart in the reference (the FPGA's pipeline depth is fixed,
+ * myfpga.cpp:1533). */
+int fheacc_ctx_set_tuning(fheacc_ctx *ctx, int br_threads, int br_polys_per_batch);
+int fheacc_ctx_get_tuning(const fheacc_ctx *ctx, int *br_threads, int *br_polys_per_batch);
+/* Autotuner: times the batched gate bootstrap on B synthetic ciphertexts
+ * with `keys` for every candidate shape, keeps the fastest in the context,
+ * and writes a JSON report of all candidates to report (may be NULL). */
+int fheacc_autotune(fheacc_ctx *ctx, fheacc_keys *keys, int64_t B, char *report, size_t report_len);
+
+/* ---- keys (device resident, one set per client) -----------------------
+ * Replaces RLWE::bootstrap_key / LWE::keyswitch_key /
+ * application::substitute_key construction (RLWE.cpp:579-606,
+ * LWE.cpp:55-106, app.cpp:4-24) and the key DMA to the card
+ * (FPGA::dma_write_bootstrap_key, dma_write_subs_key, myfpga.cpp:596-712). */
+int fheacc_keys_create(fheacc_ctx *ctx, fheacc_keys **out);
+int fheacc_keys_destroy(fheacc_keys *keys);
+/* Host arrays in the reference layouts above.  btkey[0][*][*] (residue 0) is
+ * never read by the gate path and is not stored. */
+int fheacc_keys_upload_bsk(fheacc_keys *keys, const uint64_t *bsk);
+int fheacc_keys_upload_ksk(fheacc_keys *keys, const uint64_t *ksk);
+int fheacc_keys_upload_subs(fheacc_keys *keys, const uint64_t *subs);
+/* Read back in the reference layouts (residue-0 RGSWs are written as zero). */
+int fheacc_keys_download_bsk(fheacc_keys *keys, uint64_t *bsk);
+int fheacc_keys_download_ksk(fheacc_keys *keys, uint64_t *ksk);
+int fheacc_keys_download_subs(fheacc_keys *keys, uint64_t *subs);
+/* Seeded generation on the device, straight into the device layouts.
+ * lwe_sk: [n] mod q; rlwe_sk: [N] mod Q, time domain (host arrays). */
+int fheacc_keys_generate(fheacc_keys *keys, uint64_t seed, const uint64_t *lwe_sk, const uint64_t *rlwe_sk);
+int fheacc_keys_generate_subs(fheacc_keys *keys, uint64_t seed, const uint64_t *rlwe_sk);
+/* 64-bit digest of the resident keys (read-back integrity check, the role of
+ * FPGA::dma_read_compare_bootstrap_key, myfpga.cpp:772-850). */
+int fheacc_keys_digest(fheacc_keys *keys, uint64_t *bsk_digest, uint64_t *ksk_digest);
+
+/* ---- host-side helpers of the reference host API ------------------------
+ * LWE::LWE_secretkey, RLWE::RLWE_secretkey, LWE::LWE_encrypt,
+ * LWE::LWE_decrypt (LWE.cpp:22-53,108-190; RLWE.cpp:243-279), with this
+ * library's seeded sampler in place of std::random_device. */
+int fheacc_lwe_secret(const fheacc_ctx *ctx, uint64_t seed, int binary, uint64_t *sk);
+int fheacc_rlwe_secret(const fheacc_ctx *ctx, uint64_t seed, uint64_t *sk);
+int fheacc_lwe_encrypt(const fheacc_ctx *ctx, const uint64_t *sk, const uint8_t *bits, int64_t count,
+                       uint64_t seed, uint64_t first_index, uint64_t *out);
+int fheacc_lwe_decrypt(const fheacc_ctx *ctx, const uint64_t *sk, const uint64_t *c, int64_t count,
+                       uint8_t *bits);
+
+/* ---- the gate path ------------------------------------------------------
+ * RLWE::eval_bootstrap (RLWE.cpp:726-786) for B independent pairs, and the
+ * accelerator's batched form FPGA::fpga_eval_bootstrap_x{1,4,8,12,16}
+ * (myfpga.h:141-401, myfpga.cpp:1003-1660) with any B.  gates[i] per pair.
+ * Identical operands (the reference exits the process, RLWE.cpp:727-730)
+ * fail the call with -1.  Output: [B][n+1] LWE mod q. */
+int fheacc_bootstrap_batch(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t *gates, const uint64_t *c1,
+                           const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap);
+int fheacc_bootstrap_batch_dev(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t *gates, const uint64_t *c1,
+                               const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap);
+
+/* ---- kernels of the path, batched (device pointers) ----------------------
+ * NTT / iNTT in place over count polys of N words (nbt.cpp:423-491). */
+int fheacc_ntt_dev(fheacc_ctx *ctx, uint64_t *polys, int64_t count, int inverse);
+/* RLWE_mult_RGSW (RLWE.cpp:642-704): out[i] = acc[i] (x) rgsw[i]; rgsw is in
+ * the reference RGSW layout, rgsw_stride words apart (0: one shared RGSW). */
+int fheacc_external_product_dev(fheacc_ctx *ctx, const uint64_t *acc, const uint64_t *rgsw,
+                                int64_t rgsw_stride, uint64_t *out, int64_t count);
+/* RLWE_substitute then RLWE_keyswitch with subs key[sub_factor]
+ * (RLWE.cpp:444-510; the accelerator's RLWESUBS, myfpga.cpp:1662-1795). */
+int fheacc_subs_keyswitch_dev(fheacc_ctx *ctx, fheacc_keys *keys, const uint64_t *in, int sub_factor,
+                              uint64_t *out, int64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

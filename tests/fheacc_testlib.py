@@ -1,0 +1,27 @@
+"""Shared helpers for the tests: importing the package (its directory name
+starts with a digit) and the parameter sets used for parity."""
+import importlib
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+fhe = importlib.import_module("3rd_gen_fhe_acc_amd")
+
+# name -> (Context/Oracle keyword arguments)
+SETS = {
+    # narrow words, general Bks and Br (the reference's CPU-app style, N reduced)
+    "n16_N256_q27": dict(n=16, q=512, N=256, Q=0, nbit=27, BGbits=9, Br=23, Bks=25),
+    # the benchmark's parameter shape (BG=2^7 d=4, Br=32, Bks=2^5), N and n reduced
+    "n12_N512_q27": dict(n=12, q=1024, N=512, Q=0, nbit=27, BGbits=7, Br=32, Bks=32),
+    # wide words (54-bit Q), BG=2^18 d=3
+    "n10_N128_q54": dict(n=10, q=256, N=128, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=16, Bks=32),
+    # ring degree 2048, 54-bit Q (mixed-set B shape, n reduced)
+    "n8_N2048_q54": dict(n=8, q=1024, N=2048, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=32, Bks=32),
+}
+
+# full-size configurations
+CFG0 = dict(n=500, q=1024, N=1024, Q=0, nbit=27, BGbits=7, Br=32, Bks=32)
+CFG1 = dict(n=8, q=2048, N=1024, Q=0x3FFFFFFFFED001, nbit=54, BGbits=9, Br=32, Bks=32)

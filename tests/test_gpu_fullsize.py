@@ -1,0 +1,71 @@
+"""GPU checks at the benchmark's full size (n=500, N=1024, 27-bit Q, Br=32,
+batch 4096): every decrypted bit, and a sample of ciphertexts word for word
+against the oracle (the oracle takes ~1 s per gate here, so it checks a
+sample; the rest is pinned by the size-independent truth-table property).
+Also runs the reference-shaped C++ host API (host/gate_demo.cpp)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fheacc_testlib import CFG0, ROOT, fhe
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    ctx = fhe.Context(**CFG0)
+    s, z = ctx.lwe_secret(123), ctx.rlwe_secret(123)
+    keys = ctx.keys()
+    keys.generate(123, s, z)
+    return ctx, keys, s
+
+
+def test_full_batch_truth_table_and_sample_parity(cfg0):
+    ctx, keys, s = cfg0
+    B = 4096
+    rng = np.random.default_rng(7)
+    m1, m2 = rng.integers(0, 2, B), rng.integers(0, 2, B)
+    gates = rng.integers(0, 6, B).astype(np.int32)
+    c1 = ctx.lwe_encrypt(s, m1, 9, 0)
+    c2 = ctx.lwe_encrypt(s, m2, 9, B)
+    out, acc, ks = ctx.bootstrap(keys, gates, c1, c2, capture=True)
+    truth = {0: m1 | m2, 1: m1 & m2, 2: 1 - (m1 | m2), 3: 1 - (m1 & m2), 4: m1 ^ m2, 5: 1 - (m1 ^ m2)}
+    exp = np.array([truth[int(g)][i] for i, g in enumerate(gates)])
+    assert np.array_equal(ctx.lwe_decrypt(s, out), exp)
+    # outputs are LWE mod q words; the mod switch may return q itself (LWE.cpp:255-260)
+    assert out.max() <= ctx.q
+    o = Oracle(**CFG0)
+    idx = np.array([0, 1, 2, 3, 1000, 2047, 4094, 4095])
+    bsk, ksk = keys.download_bsk(), keys.download_ksk()
+    eo, ea, ek = o.bootstrap(c1[idx], c2[idx], gates[idx], bsk, ksk, capture=True)
+    assert np.array_equal(acc[idx], ea)
+    assert np.array_equal(ks[idx], ek)
+    assert np.array_equal(out[idx], eo)
+
+
+def test_batch_equals_single(cfg0):
+    # the same gate gives the same words alone and inside a batch
+    ctx, keys, s = cfg0
+    B = 64
+    rng = np.random.default_rng(8)
+    m1, m2 = rng.integers(0, 2, B), rng.integers(0, 2, B)
+    c1, c2 = ctx.lwe_encrypt(s, m1, 10, 0), ctx.lwe_encrypt(s, m2, 10, B)
+    gates = np.full(B, 3, dtype=np.int32)
+    full = ctx.bootstrap(keys, gates, c1, c2)
+    for i in (0, 17, 63):
+        one = ctx.bootstrap(keys, gates[i:i + 1], c1[i:i + 1], c2[i:i + 1])
+        assert np.array_equal(one[0], full[i])
+
+
+def test_host_api_gate_demo():
+    exe = os.path.join(ROOT, "3rd_gen_fhe_acc_amd", "lib", "gate_demo")
+    r = subprocess.run([exe, "2"], capture_output=True, text=True, timeout=240)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL GATES PASSED" in r.stdout

@@ -1,0 +1,226 @@
+"""GPU parity: the HIP library through its C ABI against the oracle, word
+for word, on seeded inputs at sizes the oracle finishes in seconds, plus the
+reference's golden dumps.  Integer arithmetic: the bar is bit-exact."""
+import os
+
+import numpy as np
+import pytest
+
+from fheacc_testlib import SETS, fhe
+from oracle import GATES, Oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint64).view(np.int64)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy().view(np.uint64)
+
+
+def pair(name):
+    kw = SETS[name]
+    return fhe.Context(**kw), Oracle(**kw)
+
+
+@pytest.mark.parametrize("N,nbit,Q", [(16, 27, 0), (256, 27, 0), (512, 27, 0), (1024, 27, 0), (2048, 27, 0),
+                                      (128, 54, 0x3FFFFFFFFED001), (1024, 54, 0x3FFFFFFFFED001),
+                                      (2048, 54, 0x3FFFFFFFFED001)])
+def test_ntt_matches_oracle(N, nbit, Q):
+    kw = dict(n=4, q=min(2 * N, 1024), N=N, Q=Q, nbit=nbit)
+    ctx, o = fhe.Context(**kw), Oracle(**kw)
+    rng = np.random.default_rng(N + nbit)
+    x = rng.integers(0, o.Q, (37, N), dtype=np.uint64)
+    t = dev(x)
+    ctx.ntt_dev(t)
+    assert np.array_equal(host(t), o.ntt(x))
+    ctx.ntt_dev(t, inverse=True)
+    assert np.array_equal(host(t), x)
+    t2 = dev(x)
+    ctx.ntt_dev(t2, inverse=True)
+    assert np.array_equal(host(t2), o.intt(x))
+
+
+def test_external_product_golden(golden_dir):
+    g = np.load(os.path.join(golden_dir, "rlwe_x_rgsw_n1024.npz"))
+    ctx = fhe.Context(n=8, q=512, N=1024, Q=int(g["Q"]), BGbits=9)
+    acc = dev(g["acc"])
+    out = torch.zeros_like(acc)
+    ctx.external_product_dev(acc, dev(g["rgsw"]), out)
+    assert np.array_equal(host(out), g["out"])
+
+
+@pytest.mark.parametrize("N,Q,BGbits", [(1024, 0x3FFFFFFFFED001, 9),   # microbench shape, d=6
+                                        (1024, 0, 7),                   # cfg-0 shape, d=4
+                                        (2048, 0x3FFFFFFFFED001, 18),   # set B, d=3
+                                        (512, 0, 6)])                   # set C, d=5
+@pytest.mark.parametrize("per_ct", [False, True])
+def test_external_product_matches_oracle(N, Q, BGbits, per_ct):
+    kw = dict(n=4, q=1024, N=N, Q=Q, nbit=27, BGbits=BGbits)
+    ctx, o = fhe.Context(**kw), Oracle(**kw)
+    rng = np.random.default_rng(N + BGbits + per_ct)
+    B = 6
+    acc = rng.integers(0, o.Q, (B, 2, N), dtype=np.uint64)
+    rgsw = rng.integers(0, o.Q, ((B if per_ct else 1), o.rgsw_words), dtype=np.uint64)
+    out = torch.zeros((B, 2, N), dtype=torch.int64, device="cuda")
+    ctx.external_product_dev(dev(acc), dev(rgsw), out, per_ciphertext_key=per_ct)
+    exp = o.mult_rgsw(acc, rgsw if per_ct else rgsw[0])
+    assert np.array_equal(host(out), exp)
+
+
+@pytest.mark.parametrize("name", ["n16_N256_q27", "n12_N512_q27", "n10_N128_q54"])
+def test_device_keygen_matches_oracle(name):
+    ctx, o = pair(name)
+    s, z = ctx.lwe_secret(21), ctx.rlwe_secret(21)
+    assert np.array_equal(s, o.lwe_secret(21)) and np.array_equal(z, o.rlwe_secret(21))
+    keys = ctx.keys()
+    keys.generate(77, s, z)
+    bsk_d = keys.download_bsk().reshape(o.Br, -1)
+    bsk_o = o.bsk_gen(s, z, 77).reshape(o.Br, -1)
+    assert not bsk_d[0].any()           # residue 0 is not stored
+    assert np.array_equal(bsk_d[1:], bsk_o[1:])
+    assert np.array_equal(keys.download_ksk(), o.ksk_gen(z, s, 77))
+
+
+@pytest.mark.parametrize("name", ["n16_N256_q27", "n12_N512_q27", "n10_N128_q54"])
+def test_key_upload_roundtrip(name):
+    ctx, o = pair(name)
+    rng = np.random.default_rng(5)
+    bsk = rng.integers(0, o.Q, o.bsk_words, dtype=np.uint64)
+    ksk = rng.integers(0, o.Q, o.ksk_words, dtype=np.uint64)
+    keys = ctx.keys()
+    keys.upload_bsk(bsk)
+    keys.upload_ksk(ksk)
+    back = keys.download_bsk().reshape(o.Br, -1)
+    assert np.array_equal(back[1:], bsk.reshape(o.Br, -1)[1:])
+    assert np.array_equal(keys.download_ksk(), ksk)
+    d1 = keys.digest()
+    keys.upload_bsk(bsk)
+    assert keys.digest() == d1
+
+
+def run_bootstrap_case(name, B, seed, upload=False):
+    ctx, o = pair(name)
+    s = ctx.lwe_secret(seed, binary=(name != "n16_N256_q27"))
+    z = ctx.rlwe_secret(seed)
+    keys = ctx.keys()
+    if upload:
+        keys.upload_bsk(o.bsk_gen(s, z, seed))
+        keys.upload_ksk(o.ksk_gen(z, s, seed))
+    else:
+        keys.generate(seed, s, z)
+    rng = np.random.default_rng(seed)
+    m1 = rng.integers(0, 2, B)
+    m2 = rng.integers(0, 2, B)
+    c1 = ctx.lwe_encrypt(s, m1, seed, 0)
+    c2 = ctx.lwe_encrypt(s, m2, seed, 10 ** 6)
+    assert np.array_equal(c1, o.lwe_encrypt(s, m1, seed, 0))
+    gates = rng.integers(0, 6, B).astype(np.int32)
+    out, acc, ks = ctx.bootstrap(keys, gates, c1, c2, capture=True)
+    if upload:
+        bsk, ksk = o.bsk_gen(s, z, seed), o.ksk_gen(z, s, seed)
+    else:
+        bsk, ksk = keys.download_bsk(), keys.download_ksk()
+    eo, ea, ek = o.bootstrap(c1, c2, gates, bsk, ksk, capture=True)
+    assert np.array_equal(acc, ea)
+    assert np.array_equal(ks, ek)
+    assert np.array_equal(out, eo)
+    truth = {0: m1 | m2, 1: m1 & m2, 2: 1 - (m1 | m2), 3: 1 - (m1 & m2), 4: m1 ^ m2, 5: 1 - (m1 ^ m2)}
+    exp_bits = np.array([truth[int(g)][i] for i, g in enumerate(gates)])
+    assert np.array_equal(ctx.lwe_decrypt(s, out), exp_bits)
+
+
+@pytest.mark.parametrize("name", ["n16_N256_q27", "n12_N512_q27", "n10_N128_q54", "n8_N2048_q54"])
+def test_bootstrap_matches_oracle(name):
+    run_bootstrap_case(name, B=24, seed=3)
+
+
+def test_every_launch_shape_bit_exact():
+    # the autotuner may pick any shape: all must give the same words
+    ctx, o = pair("n12_N512_q27")
+    s, z = ctx.lwe_secret(12), ctx.rlwe_secret(12)
+    keys = ctx.keys()
+    keys.generate(12, s, z)
+    rng = np.random.default_rng(12)
+    m1, m2 = rng.integers(0, 2, 40), rng.integers(0, 2, 40)
+    c1, c2 = ctx.lwe_encrypt(s, m1, 12, 0), ctx.lwe_encrypt(s, m2, 12, 40)
+    gates = rng.integers(0, 6, 40).astype(np.int32)
+    exp = o.bootstrap(c1, c2, gates, keys.download_bsk(), keys.download_ksk())
+    for threads in (128, 256, 512):
+        for pb in (1, 2, 3, 4, 8):
+            ctx.set_tuning(threads, pb)
+            assert np.array_equal(ctx.bootstrap(keys, gates, c1, c2), exp), (threads, pb)
+    rep = ctx.autotune(keys, 256)
+    assert rep["chosen"]["threads"] in (128, 256, 512)
+    assert np.array_equal(ctx.bootstrap(keys, gates, c1, c2), exp)
+
+
+def test_bootstrap_uploaded_keys():
+    run_bootstrap_case("n12_N512_q27", B=9, seed=4, upload=True)
+
+
+def test_bootstrap_staged_keyswitch_ragged():
+    # above the gather threshold and not a multiple of the key-switch tile
+    run_bootstrap_case("n12_N512_q27", B=613, seed=6)
+
+
+def test_bootstrap_single():
+    run_bootstrap_case("n16_N256_q27", B=1, seed=8)
+
+
+def test_identical_operands_rejected():
+    ctx, o = pair("n16_N256_q27")
+    s, z = ctx.lwe_secret(1), ctx.rlwe_secret(1)
+    keys = ctx.keys()
+    keys.generate(1, s, z)
+    c = ctx.lwe_encrypt(s, [1, 0], 1)
+    with pytest.raises(fhe.FheAccError):
+        ctx.bootstrap(keys, [GATES["NAND"]] * 2, c, c)
+    # the device path flags it at the next sync
+    g = torch.tensor([3, 3], dtype=torch.int32, device="cuda")
+    dc = dev(c)
+    out = torch.zeros_like(dc)
+    ctx.bootstrap_dev(keys, g, dc, dc, out)
+    with pytest.raises(fhe.FheAccError):
+        ctx.sync()
+    ctx.sync()  # flag cleared
+
+
+def test_subs_keyswitch_golden(golden_dir):
+    g = np.load(os.path.join(golden_dir, "rlwesubs_n2048.npz"))
+    ctx = fhe.Context(n=8, q=1024, N=2048, Q=int(g["Q"]), BGbits=18, Bks_R_bits=int(g["Bks_R_bits"]))
+    assert ctx.digitKS_R == 6
+    subs = np.random.default_rng(1).integers(0, ctx.Q, (11, 6, 2, 2048), dtype=np.uint64)
+    subs[0] = g["kskey"]
+    keys = ctx.keys()
+    keys.upload_subs(subs)
+    assert np.array_equal(keys.download_subs(), subs.reshape(-1))
+    x = dev(g["inp"])
+    out = torch.zeros_like(x)
+    ctx.subs_keyswitch_dev(keys, x, 0, out)
+    assert np.array_equal(host(out), g["out"])
+
+
+@pytest.mark.parametrize("name", ["n16_N256_q27", "n10_N128_q54"])
+def test_subs_keyswitch_matches_oracle(name):
+    ctx, o = pair(name)
+    z = ctx.rlwe_secret(9)
+    keys = ctx.keys()
+    keys.generate_subs(9, z)
+    subs = keys.download_subs()
+    assert np.array_equal(subs, o.subs_key_gen(z, 9))
+    rng = np.random.default_rng(2)
+    x = rng.integers(0, o.Q, (3, 2, o.N), dtype=np.uint64)
+    per = o.digitKS_R * 2 * o.N
+    for f in range(o.N.bit_length() - 1):
+        out = torch.zeros((3, 2, o.N), dtype=torch.int64, device="cuda")
+        ctx.subs_keyswitch_dev(keys, dev(x), f, out)
+        got = host(out)
+        for i in range(3):
+            exp = o.rlwe_keyswitch(o.substitute(x[i], f), subs[f * per:(f + 1) * per])
+            assert np.array_equal(got[i], exp)
