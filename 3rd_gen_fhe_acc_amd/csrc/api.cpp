@@ -103,13 +103,24 @@ int ensure_scratch(fheacc_ctx *ctx, size_t bytes) {
 // of LDS next to `fixed_bytes`.
 int pick_pb(const HostParams &p, int total, size_t fixed_bytes, size_t budget, int requested) {
     if (requested > 0) return std::min(requested, total);
-    size_t per = (size_t)p.N * (p.narrow ? 4 : 8);
+    const size_t w = p.narrow ? 4 : 8;
     int pb = total;
-    while (pb > 1 && fixed_bytes + per * pb > budget) pb--;
+    while (pb > 1 && fixed_bytes + (size_t)padded(pb * p.N) * w > budget) pb--;
     return pb;
 }
 
 int check_ctx(const fheacc_ctx *c) { return c ? 0 : fail("null context"); }
+
+// LDS of the blind-rotation kernel with pb digit polys per batch: pb >= 2d
+// runs the product in place in 2d polys, otherwise acc (2 polys) plus a
+// pb-poly digit buffer; then the n evaluated a words.
+size_t br_lds_bytes(const HostParams &p, int pb) {
+    const size_t w = p.narrow ? 4 : 8;
+    const int d2 = 2 * (int)p.digitG;
+    const size_t polys = pb >= d2 ? (size_t)padded(std::max(2, d2) * p.N)
+                                  : (size_t)padded(2 * p.N) + (size_t)padded(pb * p.N);
+    return polys * w + (size_t)p.n * 2 + 16;
+}
 
 // Records a (start, stop) event pair around one kernel launch when profiling.
 struct Prof {
@@ -681,8 +692,9 @@ static int bootstrap_dev_impl(fheacc_ctx *c, fheacc_keys *k, const int32_t *gate
     a.Br = (uint32_t)p.Br;
     a.d = p.digitG;
     a.bits = p.BGbits;
-    const size_t fixed = 2 * (size_t)p.N * sizeof(W) + (size_t)p.n * 2 + 16;
-    a.pb = pick_pb(p, 2 * p.digitG, fixed, c->lds_budget, c->tune.br_pb);
+    a.pb = c->tune.br_pb > 0 ? std::min(c->tune.br_pb, 2 * (int)p.digitG) : 2 * (int)p.digitG;
+    if (c->tune.br_pb <= 0)
+        while (a.pb > 1 && br_lds_bytes(p, a.pb) > c->lds_budget) a.pb--;
     for (int g = 0; g < 6; g++) a.gate_constant[g] = p.gate_constant[g];
     a.cap_acc = cap ? cap->acc : nullptr;
     a.ksd = ksd;
@@ -690,7 +702,7 @@ static int bootstrap_dev_impl(fheacc_ctx *c, fheacc_keys *k, const int32_t *gate
     a.dks = p.digitKS;
     a.Bks = (uint32_t)p.Bks;
     a.status = c->status;
-    const size_t lds = fixed + (size_t)a.pb * p.N * sizeof(W);
+    const size_t lds = br_lds_bytes(p, a.pb);
     if (lds > c->lds_budget) return fail("parameter set does not fit the LDS budget");
     {
         Prof pr(c, 0);
@@ -806,7 +818,7 @@ static int ep_impl(fheacc_ctx *c, const uint64_t *acc, const uint64_t *rgsw, int
     a.key_stride = stride ? (int64_t)ke : 0;
     a.d = p.digitG;
     a.bits = p.BGbits;
-    a.pb = pick_pb(p, 2 * p.digitG, 2 * (size_t)p.N * sizeof(W), c->lds_budget, c->tune.br_pb);
+    a.pb = pick_pb(p, 2 * p.digitG, (size_t)padded(2 * p.N) * sizeof(W), c->lds_budget, c->tune.br_pb);
     Prof pr(c, 2);
     HIP_TRY(launch_external_product<W>(a, p.logN, count, c->stream));
     return 0;
@@ -835,8 +847,9 @@ static int subs_impl(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, int f, u
     a.sub_factor = f;
     a.d = p.digitKS_R;
     a.bits = p.Bks_R_bits;
-    a.dch = pick_pb(p, p.digitKS_R, 4 * (size_t)p.N * sizeof(W), c->lds_budget, 0);
-    if ((4 + (size_t)a.dch) * p.N * sizeof(W) > c->lds_budget) return fail("subs key switch exceeds the LDS budget");
+    a.dch = pick_pb(p, p.digitKS_R, 2 * (size_t)padded(2 * p.N) * sizeof(W), c->lds_budget, 0);
+    if ((2 * (size_t)padded(2 * p.N) + padded(a.dch * p.N)) * sizeof(W) > c->lds_budget)
+        return fail("subs key switch exceeds the LDS budget");
     HIP_TRY(launch_subs_keyswitch<W>(a, p.logN, count, c->stream));
     return 0;
 }
@@ -910,7 +923,8 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
     for (int threads : {128, 256, 512}) {
         if (threads > p.N) continue;
         for (int pb = 2; pb <= 2 * (int)p.digitG; pb += 2) {
-            const size_t lds = 2 * (size_t)p.N * wsz + (size_t)p.n * 2 + 16 + (size_t)pb * p.N * wsz;
+            const size_t lds = br_lds_bytes(p, pb);
+            (void)wsz;
             if (lds > c->lds_budget) continue;
             c->tune.br_threads = threads;
             c->tune.br_pb = pb;

@@ -89,8 +89,12 @@ __device__ __forceinline__ uint64_t mulmod_generic(const Arith<uint64_t> &A, uin
 
 template <typename W>
 struct MacSum;
+// Digit words may arrive lazily reduced (below (2 log2 N + 1) Q): the narrow
+// sum of 2 digitG such products stays below 2^64 (checked when the context
+// picks 32-bit words), and Shoup products accept any 64-bit input.
 template <>
 struct MacSum<uint32_t> {
+    static constexpr bool kCanonicalInputs = false;
     uint64_t v = 0;
     __device__ __forceinline__ void add(const Arith<uint32_t> &, uint32_t d, uint32_t k, uint32_t) {
         v += (uint64_t)d * k;
@@ -99,6 +103,7 @@ struct MacSum<uint32_t> {
 };
 template <>
 struct MacSum<uint64_t> {
+    static constexpr bool kCanonicalInputs = false;
     uint64_t v = 0;
     __device__ __forceinline__ void add(const Arith<uint64_t> &A, uint64_t d, uint64_t k, uint64_t kp) {
         v += A.mul_shoup(d, k, kp);
@@ -138,32 +143,68 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
                          int bits, int pb) {
     constexpr int N = 1 << LOGN;
     constexpr int CPT = (N + NT - 1) / NT;
-    ntt_inverse<NT>(acc, 2, LOGN, A, tw);
+    // An opaque copy of the thread index: the caller runs this in a long
+    // loop, and without it the compiler hoists every per-thread address of
+    // every pass out of that loop and runs out of registers.
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    ntt_inverse<LOGN, NT>(acc, 2, A, tw, tid);
     MacSum<W> sa[CPT], sb[CPT];
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
     const int total = 2 * d;
     for (int P0 = 0; P0 < total; P0 += pb) {
         const int np = min(pb, total - P0);
-        for (int idx = threadIdx.x; idx < (np << LOGN); idx += NT) {
+        for (int idx = tid; idx < (np << LOGN); idx += NT) {
             const int P = P0 + (idx >> LOGN), k = idx & (N - 1);
-            const W x = acc[((P & 1) << LOGN) + k];
-            dig[idx] = (x >> ((P >> 1) * bits)) & mask;
+            const W x = acc[pad(((P & 1) << LOGN) + k)];
+            dig[pad(idx)] = (x >> ((P >> 1) * bits)) & mask;
         }
         __syncthreads();
-        ntt_forward<NT>(dig, np, LOGN, A, tw);
+        // digit transforms stay lazy (below (2 logN + 1) Q): the MAC sums
+        // reduce them
+        ntt_forward<LOGN, NT>(dig, np, A, tw, MacSum<W>::kCanonicalInputs, tid);
+        if (!(P0 & 1) && !(np & 1) && N >= NT) {
+            // whole (a, b) digit pairs: key quads of the next pair are loaded
+            // while the current pair is accumulated (two sets in flight)
+            const int npairs = np >> 1, h0 = P0 >> 1;
+            KeyQuad<W> cur[CPT], nxt[CPT];
+#pragma unroll
+            for (int c = 0; c < CPT; c++) cur[c] = load_quad(key, (h0 << LOGN) + tid + c * NT);
+#pragma unroll 1
+            for (int hp = 0; hp < npairs; hp++) {
+                if (hp + 1 < npairs) {
+#pragma unroll
+                    for (int c = 0; c < CPT; c++)
+                        nxt[c] = load_quad(key, ((h0 + hp + 1) << LOGN) + tid + c * NT);
+                }
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    const int k = tid + c * NT;
+                    const W da = dig[pad(((2 * hp) << LOGN) + k)], db = dig[pad(((2 * hp + 1) << LOGN) + k)];
+                    sa[c].add(A, da, cur[c].v[0], cur[c].p[0]);
+                    sb[c].add(A, da, cur[c].v[1], cur[c].p[1]);
+                    sa[c].add(A, db, cur[c].v[2], cur[c].p[2]);
+                    sb[c].add(A, db, cur[c].v[3], cur[c].p[3]);
+                }
+#pragma unroll
+                for (int c = 0; c < CPT; c++) cur[c] = nxt[c];
+            }
+            __syncthreads();
+            continue;
+        }
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
-            const int k = threadIdx.x + c * NT;
+            const int k = tid + c * NT;
             if (k < N) {
                 for (int pp = 0; pp < np;) {
                     const int P = P0 + pp;
                     KeyQuad<W> kq = load_quad(key, ((P >> 1) << LOGN) + k);
                     if (!(P & 1)) {
-                        const W da = dig[(pp << LOGN) + k];
+                        const W da = dig[pad((pp << LOGN) + k)];
                         sa[c].add(A, da, kq.v[0], kq.p[0]);
                         sb[c].add(A, da, kq.v[1], kq.p[1]);
                         if (pp + 1 < np) {
-                            const W db = dig[((pp + 1) << LOGN) + k];
+                            const W db = dig[pad(((pp + 1) << LOGN) + k)];
                             sa[c].add(A, db, kq.v[2], kq.p[2]);
                             sb[c].add(A, db, kq.v[3], kq.p[3]);
                             pp += 2;
@@ -171,7 +212,7 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
                             pp += 1;
                         }
                     } else {
-                        const W db = dig[(pp << LOGN) + k];
+                        const W db = dig[pad((pp << LOGN) + k)];
                         sa[c].add(A, db, kq.v[2], kq.p[2]);
                         sb[c].add(A, db, kq.v[3], kq.p[3]);
                         pp += 1;
@@ -183,10 +224,67 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
     }
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
-        const int k = threadIdx.x + c * NT;
+        const int k = tid + c * NT;
         if (k < N) {
-            acc[k] = sa[c].get(A);
-            acc[N + k] = sb[c].get(A);
+            acc[pad(k)] = sa[c].get(A);
+            acc[pad(N + k)] = sb[c].get(A);
+        }
+    }
+    __syncthreads();
+}
+
+// The same product with every digit poly in one LDS batch, computed in place:
+// buf holds 2d padded polys, the accumulator (a, b) in the first two.  Each
+// thread owns coefficients k = tid + c NT: it reads its a[k], b[k] into
+// registers before the digits overwrite them, and after the transforms it
+// reads all 2d digits of k before writing the new a[k], b[k] -- slots no
+// other thread touches -- so the accumulator needs no LDS of its own.
+template <typename W, int LOGN, int NT>
+__device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const Twiddles<W> &tw, int d,
+                                 int bits) {
+    constexpr int N = 1 << LOGN;
+    constexpr int CPT = (N + NT - 1) / NT;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));  // see ext_prod
+    ntt_inverse<LOGN, NT>(buf, 2, A, tw, tid);
+    const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
+    W xa[CPT], xb[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int k = tid + c * NT;
+        if (k < N) {
+            xa[c] = buf[pad(k)];
+            xb[c] = buf[pad(N + k)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int k = tid + c * NT;
+        if (k < N) {
+            for (int h = 0; h < d; h++) {
+                buf[pad(((2 * h) << LOGN) + k)] = (xa[c] >> (h * bits)) & mask;
+                buf[pad(((2 * h + 1) << LOGN) + k)] = (xb[c] >> (h * bits)) & mask;
+            }
+        }
+    }
+    __syncthreads();
+    ntt_forward<LOGN, NT>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int k = tid + c * NT;
+        if (k < N) {
+            MacSum<W> sa, sb;
+            for (int h = 0; h < d; h++) {
+                const KeyQuad<W> kq = load_quad(key, (h << LOGN) + k);
+                const W da = buf[pad(((2 * h) << LOGN) + k)], db = buf[pad(((2 * h + 1) << LOGN) + k)];
+                sa.add(A, da, kq.v[0], kq.p[0]);
+                sb.add(A, da, kq.v[1], kq.p[1]);
+                sa.add(A, db, kq.v[2], kq.p[2]);
+                sb.add(A, db, kq.v[3], kq.p[3]);
+            }
+            buf[pad(k)] = sa.get(A);
+            buf[pad(N + k)] = sb.get(A);
         }
     }
     __syncthreads();
@@ -195,15 +293,20 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
 // ---------------------------------------------------------------------------
 // K10 + K5 + blind rotation + KS preparation, one ciphertext per workgroup
 // (RLWE.cpp:726-767, LWE.cpp:203-222,233-240, RLWE.cpp:218-226)
+// LDS: acc (2 padded polys), dig (pb padded polys), the n evaluated a words.
 // ---------------------------------------------------------------------------
 
-template <typename W, int LOGN, int NT>
+template <typename W, int LOGN, int NT, bool INPLACE>
 __global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
     constexpr int N = 1 << LOGN;
     extern __shared__ __align__(16) unsigned char smem[];
+    // pb >= 2d: every digit poly in one batch, product computed in place in
+    // 2d polys; otherwise acc (2 polys) and a separate pb-poly digit buffer
+    constexpr bool inplace = INPLACE;  // the launcher picks it from pb >= 2d
     W *acc = reinterpret_cast<W *>(smem);
-    W *dig = acc + 2 * N;
-    uint16_t *av = reinterpret_cast<uint16_t *>(dig + a.pb * N);
+    W *dig = acc + padded(2 * N);
+    uint16_t *av = reinterpret_cast<uint16_t *>(
+        inplace ? acc + padded(max(2, 2 * a.d) * N) : dig + padded(a.pb * N));
     __shared__ uint32_t s_b;
     const int64_t c = blockIdx.x;
     const int n = a.n;
@@ -225,15 +328,15 @@ __global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
     const uint32_t b = s_b;
     const uint32_t b1 = (uint32_t)(a.gate_constant[g] & qm), b2 = (b1 + q / 2) & qm;
     const int ef = (2 * N) / (int)q;
-    for (int idx = threadIdx.x; idx < 2 * N; idx += NT) acc[idx] = 0;
+    for (int idx = threadIdx.x; idx < padded(2 * N); idx += NT) acc[idx] = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < (int)(q / 2); i += NT) {
         const uint32_t tmp = (b - (uint32_t)i) & qm;
         bool neg = (b1 < b2) ? (tmp >= b1 && tmp < b2) : !(tmp >= b2 && tmp < b1);
-        acc[N + i * ef] = (W)(neg ? nQ8 : Q8);
+        acc[pad(N + i * ef)] = (W)(neg ? nQ8 : Q8);
     }
     __syncthreads();
-    ntt_forward<NT>(acc + N, 1, LOGN, a.A, a.tw);
+    ntt_forward<LOGN, NT>(acc + pad(N), 1, a.A, a.tw);
     // blind rotation: i outer, j inner, zero residues skipped
     const W *bsk = a.bsk;
     for (int i = 0; i < n; i++) {
@@ -243,36 +346,37 @@ __global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
             nega /= a.Br;
             if (r) {
                 const W *key = bsk + ((size_t)(i * a.digitR + j) * (a.Br - 1) + (r - 1)) * a.rgsw_elems;
-                ext_prod<W, LOGN, NT>(acc, dig, key, a.A, a.tw, a.d, a.bits, a.pb);
+                if constexpr (inplace) ext_prod_inplace<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
+                else ext_prod<W, LOGN, NT>(acc, dig, key, a.A, a.tw, a.d, a.bits, a.pb);
             }
         }
     }
     if (a.cap_acc) {
         uint64_t *dst = a.cap_acc + c * 2 * N;
-        for (int idx = threadIdx.x; idx < 2 * N; idx += NT) dst[idx] = acc[idx];
+        for (int idx = threadIdx.x; idx < 2 * N; idx += NT) dst[idx] = acc[pad(idx)];
     }
     __syncthreads();
     // LWE_keyswitch preparation: NTT-domain reversal of a, inverse NTT,
     // base-Bks digits of every a coefficient, b0 + Q/8 + 1
     for (int idx = threadIdx.x; idx < N / 2; idx += NT) {
-        W t = acc[idx];
-        acc[idx] = acc[N - 1 - idx];
-        acc[N - 1 - idx] = t;
+        W t = acc[pad(idx)];
+        acc[pad(idx)] = acc[pad(N - 1 - idx)];
+        acc[pad(N - 1 - idx)] = t;
     }
     __syncthreads();
-    ntt_inverse<NT>(acc, 2, LOGN, a.A, a.tw);
+    ntt_inverse<LOGN, NT>(acc, 2, a.A, a.tw);
     const int dks = a.dks;
     const uint32_t Bks = a.Bks;
     uint8_t *dst = a.ksd + c * (size_t)N * dks;
     for (int i = threadIdx.x; i < N; i += NT) {
-        uint64_t x = acc[i];
+        uint64_t x = acc[pad(i)];
         for (int j = 0; j < dks; j++) {
             dst[i * dks + j] = (uint8_t)(x % Bks);
             x /= Bks;
         }
     }
     if (threadIdx.x == 0) {
-        uint64_t v = (uint64_t)acc[N] + Q8;
+        uint64_t v = (uint64_t)acc[pad(N)] + Q8;
         a.ksb[c] = v >= a.A.Q ? v - a.A.Q : v;
     }
 }
@@ -408,11 +512,11 @@ __global__ __launch_bounds__(NT) void k_ntt(NTTArgs<W> a) {
     const int64_t p0 = (int64_t)blockIdx.x * a.per_wg;
     const int np = (int)min((int64_t)a.per_wg, a.count - p0);
     uint64_t *g = a.polys + p0 * N;
-    for (int i = threadIdx.x; i < np * N; i += NT) s[i] = (W)g[i];
+    for (int i = threadIdx.x; i < np * N; i += NT) s[pad(i)] = (W)g[i];
     __syncthreads();
-    if (a.inverse) ntt_inverse<NT>(s, np, LOGN, a.A, a.tw);
-    else ntt_forward<NT>(s, np, LOGN, a.A, a.tw);
-    for (int i = threadIdx.x; i < np * N; i += NT) g[i] = s[i];
+    if (a.inverse) ntt_inverse<LOGN, NT>(s, np, a.A, a.tw);
+    else ntt_forward<LOGN, NT>(s, np, a.A, a.tw);
+    for (int i = threadIdx.x; i < np * N; i += NT) g[i] = s[pad(i)];
 }
 
 template <typename W, int LOGN, int NT>
@@ -420,14 +524,14 @@ __global__ __launch_bounds__(NT) void k_external_product(EPArgs<W> a) {
     constexpr int N = 1 << LOGN;
     extern __shared__ __align__(16) unsigned char smem[];
     W *acc = reinterpret_cast<W *>(smem);
-    W *dig = acc + 2 * N;
+    W *dig = acc + padded(2 * N);
     const int64_t c = blockIdx.x;
     const uint64_t *src = a.acc + c * 2 * N;
-    for (int i = threadIdx.x; i < 2 * N; i += NT) acc[i] = (W)src[i];
+    for (int i = threadIdx.x; i < 2 * N; i += NT) acc[pad(i)] = (W)src[i];
     __syncthreads();
     ext_prod<W, LOGN, NT>(acc, dig, a.key + c * a.key_stride, a.A, a.tw, a.d, a.bits, a.pb);
     uint64_t *dst = a.out + c * 2 * N;
-    for (int i = threadIdx.x; i < 2 * N; i += NT) dst[i] = acc[i];
+    for (int i = threadIdx.x; i < 2 * N; i += NT) dst[i] = acc[pad(i)];
 }
 
 // RLWE_substitute (RLWE.cpp:444-462, poly_substitute 23-37) followed by
@@ -439,27 +543,27 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
     constexpr int N = 1 << LOGN;
     constexpr int CPT = (N + NT - 1) / NT;
     extern __shared__ __align__(16) unsigned char smem[];
-    W *x = reinterpret_cast<W *>(smem);  // input, time domain
-    W *y = x + 2 * N;                    // substituted
-    W *dig = y + 2 * N;                  // dch digit polys
+    W *x = reinterpret_cast<W *>(smem);  // input, time domain (2 padded polys)
+    W *y = x + padded(2 * N);            // substituted (2 padded polys)
+    W *dig = y + padded(2 * N);          // dch padded digit polys
     const int64_t c = blockIdx.x;
     const uint64_t *src = a.in + c * 2 * N;
-    for (int i = threadIdx.x; i < 2 * N; i += NT) x[i] = (W)src[i];
+    for (int i = threadIdx.x; i < 2 * N; i += NT) x[pad(i)] = (W)src[i];
     __syncthreads();
-    ntt_inverse<NT>(x, 2, LOGN, a.A, a.tw);
+    ntt_inverse<LOGN, NT>(x, 2, a.A, a.tw);
     const int power = (N >> a.sub_factor) + 1;
     for (int idx = threadIdx.x; idx < 2 * N; idx += NT) {
         const int half = idx >> LOGN, i = idx & (N - 1);
         const int64_t ip = (int64_t)i * power;
         const int io = (int)(ip & (N - 1));
-        const W v = x[idx];
-        y[(half << LOGN) + io] = ((ip >> LOGN) & 1) ? (v ? (W)a.A.Q - v : (W)0) : v;
+        const W v = x[pad(idx)];
+        y[pad((half << LOGN) + io)] = ((ip >> LOGN) & 1) ? (v ? (W)a.A.Q - v : (W)0) : v;
     }
     __syncthreads();
     // output b starts from NTT(subs(b)); the digits come from subs(a)
-    for (int i = threadIdx.x; i < N; i += NT) x[N + i] = y[N + i];
+    for (int i = threadIdx.x; i < N; i += NT) x[pad(N + i)] = y[pad(N + i)];
     __syncthreads();
-    ntt_forward<NT>(x + N, 1, LOGN, a.A, a.tw);
+    ntt_forward<LOGN, NT>(x + pad(N), 1, a.A, a.tw);
     MacSum<W> sa[CPT], sb[CPT];
     const int d = a.d, bits = a.bits, dch = a.dch;
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
@@ -467,17 +571,17 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
     for (int h0 = 0; h0 < d; h0 += dch) {
         const int nd = min(dch, d - h0);
         for (int k = threadIdx.x; k < N; k += NT) {
-            const W v = y[k];
-            for (int hh = 0; hh < nd; hh++) dig[(hh << LOGN) + k] = (v >> ((h0 + hh) * bits)) & mask;
+            const W v = y[pad(k)];
+            for (int hh = 0; hh < nd; hh++) dig[pad((hh << LOGN) + k)] = (v >> ((h0 + hh) * bits)) & mask;
         }
         __syncthreads();
-        ntt_forward<NT>(dig, nd, LOGN, a.A, a.tw);
+        ntt_forward<LOGN, NT>(dig, nd, a.A, a.tw, MacSum<W>::kCanonicalInputs);
 #pragma unroll
         for (int cc = 0; cc < CPT; cc++) {
             const int k = threadIdx.x + cc * NT;
             if (k < N) {
                 for (int hh = 0; hh < nd; hh++) {
-                    const W dv = dig[(hh << LOGN) + k];
+                    const W dv = dig[pad((hh << LOGN) + k)];
                     const size_t e = ((size_t)(h0 + hh) << LOGN) + k;
                     if (sizeof(W) == 4) {
                         uint2 kv = reinterpret_cast<const uint2 *>(key)[e];
@@ -502,7 +606,7 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
         if (k < N) {
             const uint64_t ma = sa[cc].get(a.A), mb = sb[cc].get(a.A);
             dst[k] = ma ? Q - ma : 0;
-            const uint64_t bv = x[N + k];
+            const uint64_t bv = x[pad(N + k)];
             dst[N + k] = bv >= mb ? bv - mb : bv + Q - mb;
         }
     }
@@ -583,18 +687,19 @@ __global__ __launch_bounds__(NT) void k_secret_ntt(const uint64_t *z, W *zn, W *
     constexpr int N = 1 << LOGN;
     extern __shared__ __align__(16) unsigned char smem[];
     W *s = reinterpret_cast<W *>(smem);
-    for (int i = threadIdx.x; i < N; i += NT) s[i] = (W)z[i];
+    for (int i = threadIdx.x; i < N; i += NT) s[pad(i)] = (W)z[i];
     __syncthreads();
-    ntt_forward<NT>(s, 1, LOGN, A, tw);
+    ntt_forward<LOGN, NT>(s, 1, A, tw);
     for (int i = threadIdx.x; i < N; i += NT) {
-        zn[i] = s[i];
-        znp[i] = sizeof(W) == 4 ? (W)(((uint64_t)s[i] << 32) / A.Q) : (W)sg(s[i]);
+        const W v = s[pad(i)];
+        zn[i] = v;
+        znp[i] = sizeof(W) == 4 ? (W)(((uint64_t)v << 32) / A.Q) : (W)sg(v);
     }
 }
 
 // RLWE encryption of zero into LDS (RLWE.cpp:324-344): a uniform and e
 // Gaussian in the time domain, both transformed, b = a*s + e.  Word stream as
-// the host sampler.  buf: 2N words.  Leaves a in buf[0..N), b in buf[N..2N).
+// the host sampler.  buf: 2 padded polys; leaves a in the first, b in the second.
 template <typename W, int LOGN, int NT>
 __device__ void rlwe_enc_zero(W *buf, const W *zn, const W *znp, uint64_t seed, uint64_t stream,
                               const KeygenArgs<W> &a) {
@@ -602,16 +707,16 @@ __device__ void rlwe_enc_zero(W *buf, const W *zn, const W *znp, uint64_t seed, 
     const uint64_t key = stream_key(seed, stream);
     const uint64_t Q = a.A.Q;
     for (int k = threadIdx.x; k < N; k += NT) {
-        buf[k] = (W)__umul64hi(stream_word(key, k), Q);
+        buf[pad(k)] = (W)__umul64hi(stream_word(key, k), Q);
         int64_t e = gauss_dev(a.cdt, a.cdt_len, stream_word(key, N + k)) % (int64_t)a.E_rlwe;
-        buf[N + k] = (W)(e < 0 ? (int64_t)Q + e : e);
+        buf[pad(N + k)] = (W)(e < 0 ? (int64_t)Q + e : e);
     }
     __syncthreads();
-    ntt_forward<NT>(buf, 2, LOGN, a.A, a.tw);
+    ntt_forward<LOGN, NT>(buf, 2, a.A, a.tw);
     for (int k = threadIdx.x; k < N; k += NT) {
-        W p = mulmod_generic<W>(a.A, buf[k], zn[k], znp[k]);
-        uint64_t t = (uint64_t)p + buf[N + k];
-        buf[N + k] = (W)(t >= Q ? t - Q : t);
+        W p = mulmod_generic<W>(a.A, buf[pad(k)], zn[k], znp[k]);
+        uint64_t t = (uint64_t)p + buf[pad(N + k)];
+        buf[pad(N + k)] = (W)(t >= Q ? t - Q : t);
     }
     __syncthreads();
 }
@@ -623,7 +728,7 @@ __global__ __launch_bounds__(NT) void k_bsk_gen(KeygenArgs<W> a) {
     constexpr int N = 1 << LOGN;
     extern __shared__ __align__(16) unsigned char smem[];
     W *e0 = reinterpret_cast<W *>(smem);  // one column's RLWE: a, b
-    W *m = e0 + 2 * N;                    // message monomial
+    W *m = e0 + padded(2 * N);            // message monomial
     const int d = a.d;
     const int64_t wid = blockIdx.x;
     const int h = wid % d;
@@ -641,18 +746,18 @@ __global__ __launch_bounds__(NT) void k_bsk_gen(KeygenArgs<W> a) {
     const uint64_t Q = a.A.Q;
     uint64_t coef = a.bg_pow[h];  // BG^h mod Q
     if (neg) coef = coef ? Q - coef : 0;
-    for (int i = threadIdx.x; i < N; i += NT) m[i] = (W)(i == (int)mm ? coef : 0);
+    for (int i = threadIdx.x; i < N; i += NT) m[pad(i)] = (W)(i == (int)mm ? coef : 0);
     const int64_t ref_e = ((int64_t)rr * a.digitR + j) * a.n + k;
     const uint64_t sbase = stream_id(TAG_BSK, (uint64_t)ref_e * 2 * d) + (uint64_t)h * 2;
     __syncthreads();
-    ntt_forward<NT>(m, 1, LOGN, a.A, a.tw);
+    ntt_forward<LOGN, NT>(m, 1, a.A, a.tw);
     W *o = a.bsk + (size_t)dev_rgsw * a.rgsw_elems;
     // column g = 0 carries m in a (K0a, K0b); g = 1 carries m in b (K1a, K1b)
     for (int g = 0; g < 2; g++) {
         rlwe_enc_zero<W, LOGN, NT>(e0, a.zn, a.znp, a.seed, sbase + g, a);
         for (int i = threadIdx.x; i < N; i += NT) {
-            uint64_t va = e0[i], vb = e0[N + i];
-            uint64_t t = (g == 0 ? va : vb) + (uint64_t)m[i];
+            uint64_t va = e0[pad(i)], vb = e0[pad(N + i)];
+            uint64_t t = (g == 0 ? va : vb) + (uint64_t)m[pad(i)];
             t = t >= Q ? t - Q : t;
             if (g == 0) va = t;
             else vb = t;
@@ -716,7 +821,7 @@ __global__ __launch_bounds__(NT) void k_subs_gen(KeygenArgs<W> a) {
     constexpr int N = 1 << LOGN;
     extern __shared__ __align__(16) unsigned char smem[];
     W *buf = reinterpret_cast<W *>(smem);
-    W *old = buf + 2 * N;
+    W *old = buf + padded(2 * N);
     const int l = blockIdx.x % a.d, i = blockIdx.x / a.d;
     const uint64_t Q = a.A.Q;
     // substituted secret (time domain) scaled by Bks_R^l, then transformed
@@ -725,17 +830,17 @@ __global__ __launch_bounds__(NT) void k_subs_gen(KeygenArgs<W> a) {
         const int64_t ip = (int64_t)k * power;
         const int io = (int)(ip & (N - 1));
         uint64_t v = a.z[k];
-        old[io] = (W)(((ip >> LOGN) & 1) ? (v ? Q - v : 0) : v);
+        old[pad(io)] = (W)(((ip >> LOGN) & 1) ? (v ? Q - v : 0) : v);
     }
     __syncthreads();
-    ntt_forward<NT>(old, 1, LOGN, a.A, a.tw);
+    ntt_forward<LOGN, NT>(old, 1, a.A, a.tw);
     rlwe_enc_zero<W, LOGN, NT>(buf, a.zn, a.znp, a.seed, stream_id(TAG_SUBS, (uint64_t)i * 64) + l, a);
     W *o = a.bsk + (size_t)(i * a.d + l) * a.rgsw_elems;
     for (int k = threadIdx.x; k < N; k += NT) {
         // the reference scales the NTT-domain old key by Bks_R once per l
-        W sc = mulmod_generic<W>(a.A, old[k], (W)a.bg_pow[l], (W)a.bg_powp[l]);
-        uint64_t t = (uint64_t)buf[N + k] + sc;
-        uint64_t vb = t >= Q ? t - Q : t, va = buf[k];
+        W sc = mulmod_generic<W>(a.A, old[pad(k)], (W)a.bg_pow[l], (W)a.bg_powp[l]);
+        uint64_t t = (uint64_t)buf[pad(N + k)] + sc;
+        uint64_t vb = t >= Q ? t - Q : t, va = buf[pad(k)];
         if (sizeof(W) == 4) {
             reinterpret_cast<uint2 *>(o)[k] = make_uint2((uint32_t)va, (uint32_t)vb);
         } else {
@@ -815,7 +920,7 @@ __global__ void k_digest(const uint64_t *w, int64_t count, uint64_t *partials) {
 
 template <typename W, int LOGN, int NT>
 static hipError_t launch_br_nt(const BRArgs<W> &a, int64_t B, size_t lds, hipStream_t st) {
-    auto k = k_blind_rotate<W, LOGN, NT>;
+    auto k = a.pb >= 2 * a.d ? k_blind_rotate<W, LOGN, NT, true> : k_blind_rotate<W, LOGN, NT, false>;
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3((unsigned)B), dim3(NT), lds, st, a);
@@ -860,7 +965,7 @@ template size_t ks_staged_lds<uint64_t>(uint32_t);
 template <typename W>
 hipError_t launch_ntt(const NTTArgs<W> &a, int logN, hipStream_t st) {
     const int64_t wgs = (a.count + a.per_wg - 1) / a.per_wg;
-    const size_t lds = (size_t)a.per_wg * sizeof(W) << logN;
+    const size_t lds = (size_t)padded(a.per_wg << logN) * sizeof(W);
     FHEACC_LOGN_SWITCH(logN, {
         hipLaunchKernelGGL((k_ntt<W, LOGN, 256>), dim3((unsigned)wgs), dim3(256), lds, st, a);
     })
@@ -869,7 +974,7 @@ hipError_t launch_ntt(const NTTArgs<W> &a, int logN, hipStream_t st) {
 
 template <typename W>
 hipError_t launch_external_product(const EPArgs<W> &a, int logN, int64_t count, hipStream_t st) {
-    const size_t lds = (size_t)(2 + a.pb) * sizeof(W) << logN;
+    const size_t lds = (size_t)(padded(2 << logN) + padded(a.pb << logN)) * sizeof(W);
     FHEACC_LOGN_SWITCH(logN, {
         auto k = k_external_product<W, LOGN, 256>;
         hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -881,7 +986,7 @@ hipError_t launch_external_product(const EPArgs<W> &a, int logN, int64_t count, 
 
 template <typename W>
 hipError_t launch_subs_keyswitch(const SubsArgs<W> &a, int logN, int64_t count, hipStream_t st) {
-    const size_t lds = (size_t)(4 + a.dch) * sizeof(W) << logN;
+    const size_t lds = (size_t)(2 * padded(2 << logN) + padded(a.dch << logN)) * sizeof(W);
     FHEACC_LOGN_SWITCH(logN, {
         auto k = k_subs_keyswitch<W, LOGN, 256>;
         hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -930,13 +1035,13 @@ hipError_t launch_keygen(const KeygenArgs<W> &a, const KskGenArgs<W> &ka, int lo
                          bool subs) {
     FHEACC_LOGN_SWITCH(logN, {
         constexpr int N = 1 << LOGN;
-        hipLaunchKernelGGL((k_secret_ntt<W, LOGN, 256>), dim3(1), dim3(256), N * sizeof(W), st, z, (W *)a.zn,
+        hipLaunchKernelGGL((k_secret_ntt<W, LOGN, 256>), dim3(1), dim3(256), padded(N) * sizeof(W), st, z, (W *)a.zn,
                            (W *)a.znp, a.A, a.tw, a.sg);
         if (subs) {
-            size_t lds = 3 * N * sizeof(W);
+            size_t lds = (padded(2 * N) + padded(N)) * sizeof(W);
             hipLaunchKernelGGL((k_subs_gen<W, LOGN, 256>), dim3((unsigned)(a.n * a.d)), dim3(256), lds, st, a);
         } else {
-            size_t lds = 3 * N * sizeof(W);
+            size_t lds = (padded(2 * N) + padded(N)) * sizeof(W);
             auto k = k_bsk_gen<W, LOGN, 256>;
             hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;

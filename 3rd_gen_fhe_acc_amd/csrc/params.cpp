@@ -156,7 +156,8 @@ int derive_params(const fheacc_params *in, HostParams *p, std::string *err) {
     // lazy external-product sums), 64-bit words below 2^64.
     const u128 lazy = (u128)(2 * p->logN + 1) * Q;
     if (lazy >= ((u128)1 << 64)) return bad("Q too large for lazy 64-bit butterflies");
-    p->narrow = lazy < ((u128)1 << 32) && (u128)(2 * p->digitG) * (u128)(Q - 1) * (Q - 1) < ((u128)1 << 64);
+    // (the external-product digits reach the MAC lazily, below the same bound)
+    p->narrow = lazy < ((u128)1 << 32) && (u128)(2 * p->digitG) * lazy * (Q - 1) < ((u128)1 << 64);
     return 0;
 }
 

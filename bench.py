@@ -152,7 +152,7 @@ def main():
 
     # one ciphertext per launch
     lat = []
-    for i in range(args.single + 2):
+    for i in range(args.single + 2 if args.single > 0 else 0):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         ctx.bootstrap_dev(keys, dg[i:i + 1], d1[i:i + 1], d2[i:i + 1], dout[i:i + 1])
@@ -161,7 +161,7 @@ def main():
         if i >= 2:
             lat.append(e0.elapsed_time(e1))
     ctx.sync()
-    lat = np.array(lat)
+    lat = np.array(lat if lat else [float("nan")])
     single_rate = 1e3 / lat.mean()
 
     # roofline of the dominant kernel (blind rotation), algorithmic bytes
