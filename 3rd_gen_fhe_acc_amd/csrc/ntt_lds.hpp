@@ -42,6 +42,13 @@ __host__ __device__ __forceinline__ constexpr int padded(int words) { return wor
 // stages per pass for `rem` remaining stages: 4s, never a lone 1-stage pass
 // unless the transform has a single stage
 __host__ __device__ constexpr int pass_stages(int rem) { return rem <= 4 ? rem : (rem <= 6 ? 3 : 4); }
+// the last pass of that schedule over `rem` stages; the inverse transform
+// runs the forward schedule mirrored (3,3,4 for 1024 points), so that its
+// stride-1 first pass has 8-word clusters, which the padding keeps
+// conflict-free
+__host__ __device__ constexpr int last_pass_stages(int rem) {
+    return rem <= pass_stages(rem) ? rem : last_pass_stages(rem - pass_stages(rem));
+}
 
 // Physical offset of cluster element e (stride S) from the cluster's first
 // word b0: pad(b0 + e S) - pad(b0).  When S 2^R >= 8, b0 = 8m + off with
@@ -55,8 +62,107 @@ struct ClusterOffsets {
     }
 };
 
+// Twiddles of one cluster, loaded once and reused for every poly the thread
+// handles at the same in-poly cluster index (they depend on blk only).
+template <int R, typename W>
+struct ClusterTwiddles {
+    W w[(1 << R) - 1], wp[(1 << R) - 1];
+};
+
+// Forward twiddle j of stage k (j < 2^k): rou[2^st + blk 2^k + j]
+template <int S0, int R, typename W>
+__device__ __forceinline__ ClusterTwiddles<R, W> fwd_twiddles(int blk, const Twiddles<W> &tw) {
+    ClusterTwiddles<R, W> t;
+#pragma unroll
+    for (int k = 0; k < R; k++)
+#pragma unroll
+        for (int j = 0; j < (1 << k); j++) {
+            const int ti = (1 << (S0 + k)) + (blk << k) + j;
+            t.w[(1 << k) - 1 + j] = tw.fw[ti];
+            t.wp[(1 << k) - 1 + j] = tw.fwp[ti];
+        }
+    return t;
+}
+
+// Inverse twiddle j of stage k (j < 2^(R-k-1)): irou[N/2^(U0+k+1) + blk 2^(R-k-1) + j]
+template <int LOGN, int U0, int R, typename W>
+__device__ __forceinline__ ClusterTwiddles<R, W> inv_twiddles(int blk, const Twiddles<W> &tw) {
+    constexpr int N = 1 << LOGN;
+    ClusterTwiddles<R, W> t;
+    int o = 0;
+#pragma unroll
+    for (int k = 0; k < R; k++)
+#pragma unroll
+        for (int j = 0; j < (1 << (R - k - 1)); j++) {
+            const int ti = (N >> (U0 + k + 1)) + (blk << (R - k - 1)) + j;
+            t.w[o] = tw.iw[ti];
+            t.wp[o] = tw.iwp[ti];
+            o++;
+        }
+    return t;
+}
+
+template <int S, int R, typename W>
+__device__ __forceinline__ void fwd_cluster(W *s, int b0, bool last, const Arith<W> &A, const Twiddles<W> &tw,
+                                            const ClusterTwiddles<R, W> &t) {
+    W *base = s + pad(b0);
+    W x[1 << R];
+#pragma unroll
+    for (int e = 0; e < (1 << R); e++) x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int h = 1 << (R - 1 - k);
+#pragma unroll
+        for (int e = 0; e < (1 << R); e++) {
+            if (e & h) continue;
+            const int j = (1 << k) - 1 + (e >> (R - k));
+            const W v = A.mul_shoup(x[e + h], t.w[j], t.wp[j]);
+            const W u = x[e];
+            x[e] = u + v;
+            x[e + h] = u - v + A.Q2;
+        }
+    }
+    if (last) {
+#pragma unroll
+        for (int e = 0; e < (1 << R); e++) x[e] = Arith<W>::red(A.mul_shoup(x[e], (W)1, tw.onep), A.Q);
+    }
+#pragma unroll
+    for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
+}
+
+template <int S, int R, bool LAST, typename W>
+__device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, const Twiddles<W> &tw,
+                                            const ClusterTwiddles<R, W> &t) {
+    W *base = s + pad(b0);
+    W x[1 << R];
+#pragma unroll
+    for (int e = 0; e < (1 << R); e++) x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
+    int o = 0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int h = 1 << k;
+#pragma unroll
+        for (int e = 0; e < (1 << R); e++) {
+            if (e & h) continue;
+            const int j = o + (e >> (k + 1));
+            const W u = x[e], v = x[e + h];
+            x[e] = Arith<W>::red(u + v, A.Q2);
+            x[e + h] = A.mul_shoup(u - v + A.Q2, t.w[j], t.wp[j]);
+        }
+        o += 1 << (R - k - 1);
+    }
+    if constexpr (LAST) {
+#pragma unroll
+        for (int e = 0; e < (1 << R); e++) x[e] = Arith<W>::red(A.mul_shoup(x[e], tw.iN, tw.iNp), A.Q);
+    }
+#pragma unroll
+    for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
+}
+
 // Forward pass: stages S0 .. S0+R-1 over np polys of 2^LOGN words each.
-// When last, outputs are made canonical.
+// When last, outputs are made canonical.  When NT is a multiple of the
+// clusters per poly, a thread meets the same in-poly cluster in every poly
+// it handles, and loads that cluster's twiddles once.
 template <int LOGN, int S0, int R, int NT, typename W>
 __device__ __forceinline__ void fwd_pass(W *s, int np, bool last, const Arith<W> &A, const Twiddles<W> &tw,
                                          int tid) {
@@ -64,35 +170,19 @@ __device__ __forceinline__ void fwd_pass(W *s, int np, bool last, const Arith<W>
     constexpr int lS = LOGN - S0 - R;  // log2 of the cluster stride
     constexpr int S = 1 << lS;
     constexpr int per_poly = N >> R;
-    const int clusters = np * per_poly;
-    for (int cg = tid; cg < clusters; cg += NT) {
-        const int p = cg / per_poly, c = cg - p * per_poly;
+    if constexpr (NT % per_poly == 0) {
+        const int c = tid % per_poly;
         const int blk = c >> lS, off = c & (S - 1);
-        const int b0 = p * N + (blk << (lS + R)) + off;
-        W *base = s + pad(b0);
-        W x[1 << R];
-#pragma unroll
-        for (int e = 0; e < (1 << R); e++) x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
-#pragma unroll
-        for (int k = 0; k < R; k++) {
-            const int h = 1 << (R - 1 - k);
-            const int st = S0 + k;
-#pragma unroll
-            for (int e = 0; e < (1 << R); e++) {
-                if (e & h) continue;
-                const int ti = (1 << st) + (blk << k) + (e >> (R - k));
-                const W v = A.mul_shoup(x[e + h], tw.fw[ti], tw.fwp[ti]);
-                const W u = x[e];
-                x[e] = u + v;
-                x[e + h] = u - v + A.Q2;
-            }
+        const ClusterTwiddles<R, W> t = fwd_twiddles<S0, R>(blk, tw);
+        for (int p = tid / per_poly; p < np; p += NT / per_poly)
+            fwd_cluster<S, R>(s, p * N + (blk << (lS + R)) + off, last, A, tw, t);
+    } else {
+        const int clusters = np * per_poly;
+        for (int cg = tid; cg < clusters; cg += NT) {
+            const int p = cg / per_poly, c = cg - p * per_poly;
+            const int blk = c >> lS, off = c & (S - 1);
+            fwd_cluster<S, R>(s, p * N + (blk << (lS + R)) + off, last, A, tw, fwd_twiddles<S0, R>(blk, tw));
         }
-        if (last) {
-#pragma unroll
-            for (int e = 0; e < (1 << R); e++) x[e] = Arith<W>::red(A.mul_shoup(x[e], (W)1, tw.onep), A.Q);
-        }
-#pragma unroll
-        for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
     }
 }
 
@@ -104,33 +194,19 @@ __device__ __forceinline__ void inv_pass(W *s, int np, const Arith<W> &A, const 
     constexpr int S = 1 << U0;
     constexpr int per_poly = N >> R;
     constexpr bool last = U0 + R == LOGN;
-    const int clusters = np * per_poly;
-    for (int cg = tid; cg < clusters; cg += NT) {
-        const int p = cg / per_poly, c = cg - p * per_poly;
+    if constexpr (NT % per_poly == 0) {
+        const int c = tid % per_poly;
         const int blk = c >> U0, off = c & (S - 1);
-        const int b0 = p * N + (blk << (U0 + R)) + off;
-        W *base = s + pad(b0);
-        W x[1 << R];
-#pragma unroll
-        for (int e = 0; e < (1 << R); e++) x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
-#pragma unroll
-        for (int k = 0; k < R; k++) {
-            const int h = 1 << k;
-#pragma unroll
-            for (int e = 0; e < (1 << R); e++) {
-                if (e & h) continue;
-                const int ti = (N >> (U0 + k + 1)) + (blk << (R - k - 1)) + (e >> (k + 1));
-                const W u = x[e], v = x[e + h];
-                x[e] = Arith<W>::red(u + v, A.Q2);
-                x[e + h] = A.mul_shoup(u - v + A.Q2, tw.iw[ti], tw.iwp[ti]);
-            }
+        const ClusterTwiddles<R, W> t = inv_twiddles<LOGN, U0, R>(blk, tw);
+        for (int p = tid / per_poly; p < np; p += NT / per_poly)
+            inv_cluster<S, R, last>(s, p * N + (blk << (U0 + R)) + off, A, tw, t);
+    } else {
+        const int clusters = np * per_poly;
+        for (int cg = tid; cg < clusters; cg += NT) {
+            const int p = cg / per_poly, c = cg - p * per_poly;
+            const int blk = c >> U0, off = c & (S - 1);
+            inv_cluster<S, R, last>(s, p * N + (blk << (U0 + R)) + off, A, tw, inv_twiddles<LOGN, U0, R>(blk, tw));
         }
-        if constexpr (last) {
-#pragma unroll
-            for (int e = 0; e < (1 << R); e++) x[e] = Arith<W>::red(A.mul_shoup(x[e], tw.iN, tw.iNp), A.Q);
-        }
-#pragma unroll
-        for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
     }
 }
 
@@ -148,7 +224,7 @@ __device__ __forceinline__ void fwd_passes(W *s, int np, bool canonical, const A
 template <int LOGN, int ST, int NT, typename W>
 __device__ __forceinline__ void inv_passes(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid) {
     if constexpr (ST < LOGN) {
-        constexpr int R = pass_stages(LOGN - ST);
+        constexpr int R = last_pass_stages(LOGN - ST);
         inv_pass<LOGN, ST, R, NT>(s, np, A, tw, tid);
         __syncthreads();
         inv_passes<LOGN, ST + R, NT>(s, np, A, tw, tid);
