@@ -81,6 +81,10 @@ EXPORTS = {
                                C.c_int64, C.POINTER(_Capture)],
     "fheacc_bootstrap_batch_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_int64, C.POINTER(_Capture)],
+    "fheacc_graph_capture_bootstrap": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_int64, C.POINTER(C.c_void_p)],
+    "fheacc_graph_launch": [C.c_void_p],
+    "fheacc_graph_destroy": [C.c_void_p],
     "fheacc_ntt_dev": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int],
     "fheacc_external_product_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64],
     "fheacc_subs_keyswitch_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64],
@@ -243,6 +247,10 @@ class Context:
         _check(lib().fheacc_bootstrap_batch_dev(self.h, keys.h, _dptr(gates), _dptr(c1), _dptr(c2), _dptr(out),
                                                 B, C.byref(cap)))
 
+    def capture_bootstrap(self, keys, gates, c1, c2, out):
+        """Capture the gate path over fixed device buffers into a HIP graph."""
+        return BootstrapGraph(self, keys, gates, c1, c2, out)
+
     def ntt_dev(self, polys, inverse=False):
         _check(lib().fheacc_ntt_dev(self.h, _dptr(polys), polys.numel() // self.N, 1 if inverse else 0))
 
@@ -254,6 +262,32 @@ class Context:
     def subs_keyswitch_dev(self, keys, rlwe, sub_factor, out):
         _check(lib().fheacc_subs_keyswitch_dev(self.h, keys.h, _dptr(rlwe), sub_factor, _dptr(out),
                                                rlwe.numel() // (2 * self.N)))
+
+
+class BootstrapGraph:
+    """A captured gate-bootstrap batch; launch() replays it on the context stream."""
+
+    def __init__(self, ctx, keys, gates, c1, c2, out):
+        self.ctx, self.keys = ctx, keys
+        self._bufs = (gates, c1, c2, out)  # the graph reads and writes these
+        h = C.c_void_p()
+        _check(lib().fheacc_graph_capture_bootstrap(ctx.h, keys.h, _dptr(gates), _dptr(c1), _dptr(c2),
+                                                    _dptr(out), c1.shape[0], C.byref(h)))
+        self.h = h
+
+    def launch(self):
+        _check(lib().fheacc_graph_launch(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fheacc_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Keys:

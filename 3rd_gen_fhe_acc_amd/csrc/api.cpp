@@ -980,3 +980,63 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
     }
     return 0;
 }
+
+// ---------------------------------------------------------------- HIP graphs
+
+struct fheacc_graph {
+    fheacc_ctx *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int fheacc_graph_capture_bootstrap(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
+                                   const uint64_t *c2, uint64_t *out, int64_t B, fheacc_graph **g) {
+    if (check_ctx(c) || !k || !g || !gates || !c1 || !c2 || !out || B <= 0) return fail("invalid argument");
+    *g = nullptr;
+    HIP_TRY(hipSetDevice(c->device));
+    const HostParams &p = c->p;
+    // scratch is sized before capture (allocation is not capturable)
+    const size_t ksd_bytes = (size_t)B * p.N * p.digitKS;
+    if (int rc = ensure_scratch(c, (ksd_bytes + 255) / 256 * 256 + (size_t)B * 8)) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // capture on a stream of its own (the null stream cannot be captured)
+    hipStream_t cap = nullptr, saved = c->stream;
+    HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    const bool saved_prof = c->profile;
+    c->profile = false;
+    c->stream = cap;
+    fheacc_graph *gr = new fheacc_graph();
+    gr->ctx = c;
+    int rc = 0;
+    if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = fail("begin capture");
+    if (!rc) rc = fheacc_bootstrap_batch_dev(c, k, gates, c1, c2, out, B, nullptr);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(cap, &graph);
+    c->stream = saved;
+    c->profile = saved_prof;
+    (void)hipStreamDestroy(cap);
+    if (!rc && e != hipSuccess) rc = fail(std::string("end capture: ") + hipGetErrorString(e));
+    if (!rc && hipGraphInstantiate(&gr->exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = fail("instantiate");
+    gr->graph = graph;
+    if (rc) {
+        fheacc_graph_destroy(gr);
+        return rc;
+    }
+    *g = gr;
+    return 0;
+}
+
+int fheacc_graph_launch(fheacc_graph *g) {
+    if (!g || !g->exec) return fail("null graph");
+    HIP_TRY(hipSetDevice(g->ctx->device));
+    HIP_TRY(hipGraphLaunch(g->exec, g->ctx->stream));
+    return 0;
+}
+
+int fheacc_graph_destroy(fheacc_graph *g) {
+    if (!g) return 0;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return 0;
+}

@@ -45,9 +45,49 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", default="auto",
                     help="'auto' (run the autotuner) or THREADS,POLYS_PER_BATCH for the blind-rotation kernel")
+    ap.add_argument("--no-microbench", action="store_true", help="skip the NTT / external-product microbench")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (reference dispatch for HBM byte counters)")
     return ap.parse_args()
+
+
+def microbench(fhe, torch, B, reps=5):
+    """Batched forward+inverse negacyclic NTT plus NTT-domain RLWE x RGSW
+    external product: B RLWE ciphertexts, N=1024, Q=0x3FFFFFFFFED001 (54-bit,
+    = 1 mod 2048), coefficients uniform in [0, Q) from a seed, one shared RGSW
+    with gadget base 2^9 (d=6).  Times from HIP events per kernel."""
+    ctx = fhe.Context(n=8, q=2048, N=1024, Q=0x3FFFFFFFFED001, BGbits=9)
+    rng = np.random.default_rng(SEED + 1)
+    Q = ctx.Q
+    polys = torch.from_numpy(rng.integers(0, Q, (B, 2, 1024), dtype=np.uint64).view(np.int64)).cuda()
+    rgsw = torch.from_numpy(rng.integers(0, Q, ctx.digitG * 4 * 1024, dtype=np.uint64).view(np.int64)).cuda()
+    out = torch.zeros_like(polys)
+    work = polys.clone()
+    ctx.ntt_dev(work)
+    ctx.ntt_dev(work, inverse=True)
+    ctx.external_product_dev(polys, rgsw, out)
+    ctx.sync()
+    roundtrip_ok = bool(torch.equal(work, polys))
+    ctx.profile(True)
+    for _ in range(reps):
+        ctx.ntt_dev(work)
+        ctx.ntt_dev(work, inverse=True)
+        ctx.external_product_dev(polys, rgsw, out)
+    kms = ctx.kernel_ms()
+    ctx.profile(False)
+    ntt_ms = kms["ntt"][0] / max(kms["ntt"][1], 1)
+    ep_ms = kms["external_product"][0] / max(kms["external_product"][1], 1)
+    d, N = ctx.digitG, 1024
+    ep_modmul = 2 * d * (N // 2) * 10 + 2 * (N // 2) * 10 + 2 * N + 4 * d * N
+    return {
+        "workload": "B=%d RLWE, N=1024, Q=0x3FFFFFFFFED001, BG=2^9 d=6, shared RGSW" % B,
+        "ntt_ms_per_launch": round(ntt_ms, 4), "ntt_polys_per_launch": 2 * B,
+        "ntt_polys_per_s": round(2 * B / (ntt_ms * 1e-3), 1),
+        "external_product_ms_per_launch": round(ep_ms, 4),
+        "external_products_per_s": round(B / (ep_ms * 1e-3), 1),
+        "external_product_modmul_per_s_T": round(B * ep_modmul / (ep_ms * 1e-3) / 1e12, 4),
+        "forward_inverse_roundtrip_ok": roundtrip_ok,
+    }
 
 
 def cpu_baseline(ctx, keys, c1, c2, gates, gpu_out, sample):
@@ -133,6 +173,7 @@ def main():
         ctx.bootstrap_dev(keys, dg, d1, d2, dout)
     ctx.sync()
     torch.cuda.synchronize()
+    # eager launches, with per-kernel HIP events on the launch stream
     ctx.profile(True)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -144,9 +185,24 @@ def main():
     ctx.sync()
     kms = ctx.kernel_ms()
     ctx.profile(False)
+    eager_ms = ev0.elapsed_time(ev1) / args.steps
+    # the timed form: the batch captured once into a HIP graph and replayed
+    gout = torch.zeros_like(d1)
+    graph = ctx.capture_bootstrap(keys, dg, d1, d2, gout)
+    for _ in range(args.warmup):
+        graph.launch()
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(args.steps):
+        graph.launch()
+    ev1.record()
+    torch.cuda.synchronize()
+    ctx.sync()
     ms_step = ev0.elapsed_time(ev1) / args.steps
+    graph.close()
     value = B / (ms_step / 1e3)
-    out = dout.cpu().numpy().view(np.uint64)
+    out = gout.cpu().numpy().view(np.uint64)
+    graph_equals_eager = bool(np.array_equal(out, dout.cpu().numpy().view(np.uint64)))
     bits = ctx.lwe_decrypt(s, out)
     truth_ok = bool(np.array_equal(bits, 1 - (m1 & m2)))
 
@@ -218,9 +274,13 @@ def main():
                        "note": "single gate, idle GPU (one ciphertext per launch)"},
         "batch_latency_ms": round(ms_step, 4),
         "decrypt_truth_table_ok": truth_ok,
+        "graph_equals_eager": graph_equals_eager,
+        "eager_ms_per_step": round(eager_ms, 4),
         "keygen_s": round(keygen_s, 3),
         "tuning": tune_report,
     }
+    if not args.no_microbench:
+        rec["kernel_microbench"] = microbench(fhe, torch, B)
     if not args.no_cpu_baseline and args.cpu_sample > 0:
         rec["cpu_baseline"] = cpu_baseline(ctx, keys, c1, c2, gates, out, args.cpu_sample)
     print(json.dumps(rec))

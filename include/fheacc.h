@@ -151,6 +151,17 @@ int fheacc_bootstrap_batch(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t *ga
 int fheacc_bootstrap_batch_dev(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t *gates, const uint64_t *c1,
                                const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap);
 
+/* The gate path for fixed device buffers captured once into a HIP graph and
+ * replayed: each launch re-reads gates/c1/c2 and rewrites out (the FPGA form
+ * issues one MMIO instruction per external product instead,
+ * myfpga.cpp:1446-1492).  The buffers must stay valid while the graph
+ * exists.  Identical operands are reported by fheacc_sync. */
+typedef struct fheacc_graph fheacc_graph;
+int fheacc_graph_capture_bootstrap(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t *gates, const uint64_t *c1,
+                                   const uint64_t *c2, uint64_t *out, int64_t B, fheacc_graph **graph);
+int fheacc_graph_launch(fheacc_graph *graph);
+int fheacc_graph_destroy(fheacc_graph *graph);
+
 /* ---- kernels of the path, batched (device pointers) ----------------------
  * NTT / iNTT in place over count polys of N words (nbt.cpp:423-491). */
 int fheacc_ntt_dev(fheacc_ctx *ctx, uint64_t *polys, int64_t count, int inverse);

@@ -160,6 +160,35 @@ def test_every_launch_shape_bit_exact():
     assert np.array_equal(ctx.bootstrap(keys, gates, c1, c2), exp)
 
 
+def test_graph_replays_match_eager():
+    # replay a captured batch several times, with new inputs written into the
+    # captured buffers and eager work in between; every replay must equal the
+    # eager result for the same inputs
+    ctx, o = pair("n12_N512_q27")
+    s, z = ctx.lwe_secret(13), ctx.rlwe_secret(13)
+    keys = ctx.keys()
+    keys.generate(13, s, z)
+    B = 64
+    rng = np.random.default_rng(13)
+    g = torch.zeros(B, dtype=torch.int32, device="cuda")
+    d1 = torch.zeros((B, ctx.n + 1), dtype=torch.int64, device="cuda")
+    d2 = torch.zeros_like(d1)
+    out = torch.zeros_like(d1)
+    graph = ctx.capture_bootstrap(keys, g, d1, d2, out)
+    for rep in range(3):
+        m1, m2 = rng.integers(0, 2, B), rng.integers(0, 2, B)
+        c1, c2 = ctx.lwe_encrypt(s, m1, 100 + rep, 0), ctx.lwe_encrypt(s, m2, 100 + rep, B)
+        gates = rng.integers(0, 6, B).astype(np.int32)
+        g.copy_(torch.from_numpy(gates))
+        d1.copy_(dev(c1))
+        d2.copy_(dev(c2))
+        eager = ctx.bootstrap(keys, gates, c1, c2)
+        graph.launch()
+        ctx.sync()
+        assert np.array_equal(host(out), eager), rep
+    graph.close()
+
+
 def test_bootstrap_uploaded_keys():
     run_bootstrap_case("n12_N512_q27", B=9, seed=4, upload=True)
 
