@@ -50,6 +50,20 @@ class _Capture(C.Structure):
     _fields_ = [("acc", C.c_void_p), ("ks", C.c_void_p)]
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
+                ("min_fill", C.c_int32)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("jobs_submitted", C.c_int64), ("jobs_done", C.c_int64), ("jobs_failed", C.c_int64),
+                ("jobs_queued", C.c_int64), ("jobs_active", C.c_int64), ("bootstraps", C.c_int64),
+                ("batches", C.c_int64), ("max_batch_seen", C.c_int64), ("wall_s", C.c_double),
+                ("gpu_busy_s", C.c_double), ("gate_ms_p50", C.c_double), ("gate_ms_p99", C.c_double),
+                ("job_ms_p50", C.c_double), ("job_ms_p99", C.c_double), ("pool_bytes", C.c_uint64),
+                ("peak_bytes", C.c_uint64)]
+
+
 EXPORTS = {
     "fheacc_ctx_create": [C.POINTER(_Params), C.c_int, C.POINTER(C.c_void_p)],
     "fheacc_ctx_destroy": [C.c_void_p],
@@ -83,11 +97,21 @@ EXPORTS = {
                                    C.c_int64, C.POINTER(_Capture)],
     "fheacc_graph_capture_bootstrap": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_int64, C.POINTER(C.c_void_p)],
+    "fheacc_graph_capture_mixed": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)],
     "fheacc_graph_launch": [C.c_void_p],
     "fheacc_graph_destroy": [C.c_void_p],
     "fheacc_ntt_dev": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int],
     "fheacc_external_product_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64],
     "fheacc_subs_keyswitch_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64],
+    "fheacc_server_create": [C.c_void_p, C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)],
+    "fheacc_server_destroy": [C.c_void_p],
+    "fheacc_server_add_client": [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)],
+    "fheacc_server_submit": [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                             C.c_int64, C.c_void_p, C.POINTER(C.c_int64)],
+    "fheacc_server_wait": [C.c_void_p, C.c_int64, C.POINTER(C.c_double)],
+    "fheacc_server_get_stats": [C.c_void_p, C.POINTER(_ServerStats)],
+    "fheacc_server_reset_stats": [C.c_void_p],
 }
 
 _lib = None
@@ -290,6 +314,25 @@ class BootstrapGraph:
             pass
 
 
+class MixedGraph(BootstrapGraph):
+    """Several parameter sets' batches captured into one HIP graph, one
+    concurrent branch per set; launch() replays them as one unit.
+    sets: [(ctx, keys, gates, c1, c2, out)] with device tensors."""
+
+    def __init__(self, sets):
+        self.ctx, self.keys = sets[0][0], [s[1] for s in sets]
+        self._bufs = [s[2:] for s in sets]
+        k = len(sets)
+        arr = lambda vals: (C.c_void_p * k)(*vals)
+        ctxs = arr([s[0].h.value for s in sets])
+        keys = arr([s[1].h.value for s in sets])
+        gates, c1, c2, out = (arr([s[i].data_ptr() for s in sets]) for i in (2, 3, 4, 5))
+        B = (C.c_int64 * k)(*[s[3].shape[0] for s in sets])
+        h = C.c_void_p()
+        _check(lib().fheacc_graph_capture_mixed(k, ctxs, keys, gates, c1, c2, out, B, C.byref(h)))
+        self.h = h
+
+
 class Keys:
     """A client's device-resident keys (bootstrap, LWE key switch, subs)."""
 
@@ -350,3 +393,67 @@ class Keys:
         a, b = C.c_uint64(), C.c_uint64()
         _check(lib().fheacc_keys_digest(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+
+class Server:
+    """The serving loop (fheacc_server_*): gate-circuit jobs of many clients,
+    admitted by HBM footprint and coalesced into shared gate-path launches."""
+
+    def __init__(self, ctx, hbm_budget=1 << 30, max_batch=2048, max_inflight=2, min_fill=0):
+        self.ctx = ctx
+        cfg = _ServerConfig(int(hbm_budget), int(max_batch), int(max_inflight), int(min_fill))
+        h = C.c_void_p()
+        _check(lib().fheacc_server_create(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._clients = []
+        self._jobs = {}
+
+    def add_client(self, keys):
+        cid = C.c_int32()
+        _check(lib().fheacc_server_add_client(self.h, keys.h, C.byref(cid)))
+        self._clients.append(keys)
+        return cid.value
+
+    def submit(self, client, inputs, ops, outputs):
+        """inputs: [n_in, n+1] words mod q; ops: [n_gates, 3] (gate, src1, src2);
+        outputs: node indices.  Returns a job id for wait()."""
+        inputs, pin = _np(np.asarray(inputs).reshape(-1, self.ctx.n + 1))
+        ops = np.ascontiguousarray(np.asarray(ops, dtype=np.int32).reshape(-1, 3))
+        outputs = np.ascontiguousarray(np.asarray(outputs, dtype=np.int64).reshape(-1))
+        out = np.zeros((outputs.size, self.ctx.n + 1), dtype=np.uint64)
+        jid = C.c_int64()
+        _check(lib().fheacc_server_submit(self.h, client, pin, inputs.shape[0],
+                                          ops.ctypes.data_as(C.c_void_p), ops.shape[0],
+                                          outputs.ctypes.data_as(C.c_void_p), outputs.size,
+                                          out.ctypes.data_as(C.c_void_p), C.byref(jid)))
+        self._jobs[jid.value] = (inputs, ops, outputs, out)
+        return jid.value
+
+    def wait(self, job):
+        """Returns (output words [n_out, n+1], latency in ms)."""
+        lat = C.c_double()
+        try:
+            _check(lib().fheacc_server_wait(self.h, job, C.byref(lat)))
+        finally:
+            bufs = self._jobs.pop(job, None)
+        return bufs[3], lat.value
+
+    def stats(self):
+        st = _ServerStats()
+        _check(lib().fheacc_server_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in _ServerStats._fields_}
+
+    def reset_stats(self):
+        _check(lib().fheacc_server_reset_stats(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fheacc_server_destroy(self.h)
+            self.h = None
+            self._jobs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -13,22 +14,15 @@
 using namespace fheacc;
 
 namespace {
-
 thread_local std::string g_err;
+}
 
-int fail(const std::string &m, int rc = FHEACC_ERR) {
+int fheacc::fail(const std::string &m, int rc) {
     g_err = m;
     return rc;
 }
 
-#define HIP_TRY(expr)                                                                       \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess) {                                                             \
-            return fail(std::string(#expr) + ": " + hipGetErrorString(e_),                 \
-                        e_ == hipErrorOutOfMemory ? FHEACC_ENOMEM : FHEACC_ERR);            \
-        }                                                                                   \
-    } while (0)
+namespace {
 
 typedef unsigned __int128 u128;
 
@@ -126,15 +120,26 @@ size_t br_lds_bytes(const HostParams &p, int pb) {
 struct Prof {
     fheacc_ctx *c;
     int cls;
+    hipStream_t st;
     hipEvent_t a = nullptr, b = nullptr;
-    Prof(fheacc_ctx *c_, int cls_) : c(c_), cls(cls_) {
-        if (!c->profile) return;
-        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess ||
-            hipEventRecord(a, c->stream) != hipSuccess)
+    Prof(fheacc_ctx *c_, int cls_) : Prof(c_, cls_, c_->stream, c_->profile) {}
+    Prof(fheacc_ctx *c_, int cls_, hipStream_t st_, bool on) : c(c_), cls(cls_), st(st_) {
+        if (!on) return;
+        if (hipEventCreate(&a) != hipSuccess) {
             a = nullptr;
+            return;
+        }
+        if (hipEventCreate(&b) != hipSuccess || hipEventRecord(a, st) != hipSuccess) release();
+    }
+    void release() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        a = b = nullptr;
     }
     ~Prof() {
-        if (a && hipEventRecord(b, c->stream) == hipSuccess) c->events.push_back({cls, a, b});
+        if (!a) return;
+        if (hipEventRecord(b, st) == hipSuccess) c->events.push_back({cls, a, b});
+        else release();
     }
 };
 
@@ -669,22 +674,26 @@ int fheacc_lwe_decrypt(const fheacc_ctx *c, const uint64_t *sk, const uint64_t *
 
 // ---------------------------------------------------------------- gate path
 
+size_t fheacc::gate_scratch_bytes(const HostParams &p, int64_t B) {
+    const size_t ksd_bytes = (size_t)B * p.N * p.digitKS;
+    return (ksd_bytes + 255) / 256 * 256 + (size_t)B * 8;
+}
+
 template <typename W>
-static int bootstrap_dev_impl(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
-                              const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap) {
+static int gate_path_impl(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch &g) {
     const HostParams &p = c->p;
+    const int64_t B = g.B;
     const size_t ksd_bytes = (size_t)B * p.N * p.digitKS;
     const size_t ksd_al = (ksd_bytes + 255) / 256 * 256;
-    if (int rc = ensure_scratch(c, ksd_al + (size_t)B * 8)) return rc;
-    uint8_t *ksd = (uint8_t *)c->scratch;
-    uint64_t *ksb = (uint64_t *)((char *)c->scratch + ksd_al);
+    uint8_t *ksd = (uint8_t *)g.scratch;
+    uint64_t *ksb = (uint64_t *)((char *)g.scratch + ksd_al);
     BRArgs<W> a{};
     a.A = make_arith<W>(p);
     a.tw = make_tw<W>(c);
-    a.c1 = c1;
-    a.c2 = c2;
-    a.gates = gates;
-    a.bsk = (const W *)k->bsk;
+    a.c1 = g.c1;
+    a.c2 = g.c2;
+    a.gates = g.gates;
+    a.bsk = k ? (const W *)k->bsk : nullptr;
     a.rgsw_elems = rgsw_elems(p);
     a.n = p.n;
     a.q = (uint32_t)p.q;
@@ -695,24 +704,26 @@ static int bootstrap_dev_impl(fheacc_ctx *c, fheacc_keys *k, const int32_t *gate
     a.pb = c->tune.br_pb > 0 ? std::min(c->tune.br_pb, 2 * (int)p.digitG) : 2 * (int)p.digitG;
     if (c->tune.br_pb <= 0)
         while (a.pb > 1 && br_lds_bytes(p, a.pb) > c->lds_budget) a.pb--;
-    for (int g = 0; g < 6; g++) a.gate_constant[g] = p.gate_constant[g];
-    a.cap_acc = cap ? cap->acc : nullptr;
+    for (int i = 0; i < 6; i++) a.gate_constant[i] = p.gate_constant[i];
+    a.cap_acc = g.cap ? g.cap->acc : nullptr;
     a.ksd = ksd;
     a.ksb = ksb;
     a.dks = p.digitKS;
     a.Bks = (uint32_t)p.Bks;
     a.status = c->status;
+    a.idx1 = g.idx1;
+    a.idx2 = g.idx2;
+    a.key_of = g.key_of;
+    a.bsk_tab = (const W *const *)g.bsk_tab;
+    a.ct_status = g.ct_status;
     const size_t lds = br_lds_bytes(p, a.pb);
     if (lds > c->lds_budget) return fail("parameter set does not fit the LDS budget");
     {
-        Prof pr(c, 0);
-        HIP_TRY(launch_blind_rotate<W>(a, p.logN, c->tune.br_threads, B, lds, c->stream));
+        Prof pr(c, 0, g.stream, g.profile);
+        HIP_TRY(launch_blind_rotate<W>(a, p.logN, c->tune.br_threads, B, lds, g.stream));
     }
     KSArgs<W> s{};
     s.A = a.A;
-    s.ksk = (const W *)k->ksk;
-    s.ksd = ksd;
-    s.ksb = ksb;
     s.n = p.n;
     s.n_pad = n_pad(p);
     s.S = p.N * p.digitKS;
@@ -721,18 +732,46 @@ static int bootstrap_dev_impl(fheacc_ctx *c, fheacc_keys *k, const int32_t *gate
     int lq = 0;
     while ((1ULL << lq) < p.q) lq++;
     s.lq1 = lq + 1;
-    s.out = out;
-    s.cap_ks = cap ? cap->ks : nullptr;
-    s.B = B;
     s.KB = c->tune.ks_kb;
     s.CB = c->tune.ks_cb;
     s.SB = c->tune.ks_sb;
-    const bool gather = B <= c->tune.ks_gather_max || ks_staged_lds<W>(s.Bks) > c->lds_budget;
-    {
-        Prof pr(c, 1);
-        HIP_TRY(launch_lwe_keyswitch<W>(s, gather, c->stream));
+    // one key-switch launch per key segment (a dense batch is one segment)
+    const int nseg = g.nseg > 0 ? g.nseg : 1;
+    for (int sg = 0; sg < nseg; sg++) {
+        const int64_t o0 = g.nseg > 0 ? g.seg_off[sg] : 0, o1 = g.nseg > 0 ? g.seg_off[sg + 1] : B;
+        if (o1 <= o0) continue;
+        s.ksk = g.nseg > 0 ? (const W *)g.seg_ksk[sg] : (const W *)k->ksk;
+        s.ksd = ksd + (size_t)o0 * s.S;
+        s.ksb = ksb + o0;
+        s.oidx = g.oidx ? g.oidx + o0 : nullptr;
+        s.out = g.oidx ? g.out : g.out + o0 * (p.n + 1);
+        s.cap_ks = g.cap && g.cap->ks ? g.cap->ks + o0 * (p.n + 1) : nullptr;
+        s.B = o1 - o0;
+        const bool gather = s.B <= c->tune.ks_gather_max || ks_staged_lds<W>(s.Bks) > c->lds_budget;
+        Prof pr(c, 1, g.stream, g.profile);
+        HIP_TRY(launch_lwe_keyswitch<W>(s, gather, g.stream));
     }
     return 0;
+}
+
+int fheacc::launch_gate_path(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch &g) {
+    return c->p.narrow ? gate_path_impl<uint32_t>(c, k, g) : gate_path_impl<uint64_t>(c, k, g);
+}
+
+static int bootstrap_dense(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
+                           const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap) {
+    if (int rc = ensure_scratch(c, gate_scratch_bytes(c->p, B))) return rc;
+    GateLaunch g;
+    g.gates = gates;
+    g.c1 = c1;
+    g.c2 = c2;
+    g.out = out;
+    g.B = B;
+    g.cap = cap;
+    g.scratch = c->scratch;
+    g.stream = c->stream;
+    g.profile = c->profile;
+    return launch_gate_path(c, k, g);
 }
 
 int fheacc_bootstrap_batch_dev(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
@@ -743,8 +782,7 @@ int fheacc_bootstrap_batch_dev(fheacc_ctx *c, fheacc_keys *k, const int32_t *gat
     if (B > 0x7fffffff) return fail("batch too large");
     std::lock_guard<std::mutex> lk(c->mu);
     HIP_TRY(hipSetDevice(c->device));
-    return c->p.narrow ? bootstrap_dev_impl<uint32_t>(c, k, gates, c1, c2, out, B, cap)
-                       : bootstrap_dev_impl<uint64_t>(c, k, gates, c1, c2, out, B, cap);
+    return bootstrap_dense(c, k, gates, c1, c2, out, B, cap);
 }
 
 int fheacc_bootstrap_batch(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
@@ -984,46 +1022,93 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
 // ---------------------------------------------------------------- HIP graphs
 
 struct fheacc_graph {
-    fheacc_ctx *ctx = nullptr;
+    fheacc_ctx *ctx = nullptr;  // launches on this context's stream
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // one scratch buffer per branch, owned by the graph: later eager calls
+    // may grow (reallocate) their context's scratch
+    std::vector<void *> scratch;
 };
+
+// Captures one gate-path branch per set into one graph: a fork from a root
+// stream, one stream per set, a join back to the root.
+static int capture_sets(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *keys, const int32_t *const *gates,
+                        const uint64_t *const *c1, const uint64_t *const *c2, uint64_t *const *out,
+                        const int64_t *B, fheacc_graph **g) {
+    if (!g || nsets < 1 || nsets > 16 || !ctxs || !keys || !gates || !c1 || !c2 || !out || !B)
+        return fail("invalid argument");
+    *g = nullptr;
+    for (int i = 0; i < nsets; i++) {
+        if (check_ctx(ctxs[i]) || !keys[i] || !gates[i] || !c1[i] || !c2[i] || !out[i] || B[i] <= 0 ||
+            B[i] > 0x7fffffff)
+            return fail("invalid argument for set " + std::to_string(i));
+        if (keys[i]->ctx != ctxs[i]) return fail("keys of set " + std::to_string(i) + " belong to another context");
+        if (!keys[i]->bsk || !keys[i]->ksk) return fail("bootstrap and key-switch keys must be resident");
+        if (ctxs[i]->device != ctxs[0]->device) return fail("all sets must be on one device");
+    }
+    HIP_TRY(hipSetDevice(ctxs[0]->device));
+    std::unique_ptr<fheacc_graph, int (*)(fheacc_graph *)> gr(new fheacc_graph(), fheacc_graph_destroy);
+    gr->ctx = ctxs[0];
+    for (int i = 0; i < nsets; i++) {
+        void *p = nullptr;
+        HIP_TRY(hipMalloc(&p, gate_scratch_bytes(ctxs[i]->p, B[i])));
+        gr->scratch.push_back(p);
+    }
+    for (int i = 0; i < nsets; i++) HIP_TRY(hipStreamSynchronize(ctxs[i]->stream));
+    // capture on streams of its own (the null stream cannot be captured)
+    std::vector<hipStream_t> st(nsets + 1, nullptr);
+    std::vector<hipEvent_t> ev(nsets + 1, nullptr);
+    int rc = 0;
+    for (int i = 0; i <= nsets && !rc; i++) {
+        if (hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess)
+            rc = fail("stream or event creation");
+    }
+    hipGraph_t graph = nullptr;
+    if (!rc) {
+        hipStream_t root = st[nsets];
+        if (hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = fail("begin capture");
+        if (!rc && hipEventRecord(ev[nsets], root) != hipSuccess) rc = fail("fork event");
+        for (int i = 0; i < nsets && !rc; i++) {
+            if (hipStreamWaitEvent(st[i], ev[nsets], 0) != hipSuccess) {
+                rc = fail("fork");
+                break;
+            }
+            GateLaunch gl;
+            gl.gates = gates[i];
+            gl.c1 = c1[i];
+            gl.c2 = c2[i];
+            gl.out = out[i];
+            gl.B = B[i];
+            gl.scratch = gr->scratch[i];
+            gl.stream = st[i];
+            rc = launch_gate_path(ctxs[i], keys[i], gl);
+            if (!rc && (hipEventRecord(ev[i], st[i]) != hipSuccess || hipStreamWaitEvent(root, ev[i], 0) != hipSuccess))
+                rc = fail("join");
+        }
+        hipError_t e = hipStreamEndCapture(root, &graph);
+        if (!rc && e != hipSuccess) rc = fail(std::string("end capture: ") + hipGetErrorString(e));
+    }
+    for (int i = 0; i <= nsets; i++) {
+        if (st[i]) (void)hipStreamDestroy(st[i]);
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    }
+    gr->graph = graph;
+    if (!rc && hipGraphInstantiate(&gr->exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = fail("instantiate");
+    if (rc) return rc;
+    *g = gr.release();
+    return 0;
+}
 
 int fheacc_graph_capture_bootstrap(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
                                    const uint64_t *c2, uint64_t *out, int64_t B, fheacc_graph **g) {
-    if (check_ctx(c) || !k || !g || !gates || !c1 || !c2 || !out || B <= 0) return fail("invalid argument");
-    *g = nullptr;
-    HIP_TRY(hipSetDevice(c->device));
-    const HostParams &p = c->p;
-    // scratch is sized before capture (allocation is not capturable)
-    const size_t ksd_bytes = (size_t)B * p.N * p.digitKS;
-    if (int rc = ensure_scratch(c, (ksd_bytes + 255) / 256 * 256 + (size_t)B * 8)) return rc;
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    // capture on a stream of its own (the null stream cannot be captured)
-    hipStream_t cap = nullptr, saved = c->stream;
-    HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-    const bool saved_prof = c->profile;
-    c->profile = false;
-    c->stream = cap;
-    fheacc_graph *gr = new fheacc_graph();
-    gr->ctx = c;
-    int rc = 0;
-    if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = fail("begin capture");
-    if (!rc) rc = fheacc_bootstrap_batch_dev(c, k, gates, c1, c2, out, B, nullptr);
-    hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(cap, &graph);
-    c->stream = saved;
-    c->profile = saved_prof;
-    (void)hipStreamDestroy(cap);
-    if (!rc && e != hipSuccess) rc = fail(std::string("end capture: ") + hipGetErrorString(e));
-    if (!rc && hipGraphInstantiate(&gr->exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = fail("instantiate");
-    gr->graph = graph;
-    if (rc) {
-        fheacc_graph_destroy(gr);
-        return rc;
-    }
-    *g = gr;
-    return 0;
+    return capture_sets(1, &c, &k, &gates, &c1, &c2, &out, &B, g);
+}
+
+int fheacc_graph_capture_mixed(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *keys,
+                               const int32_t *const *gates, const uint64_t *const *c1, const uint64_t *const *c2,
+                               uint64_t *const *out, const int64_t *B, fheacc_graph **g) {
+    return capture_sets(nsets, ctxs, keys, gates, c1, c2, out, B, g);
 }
 
 int fheacc_graph_launch(fheacc_graph *g) {
@@ -1035,8 +1120,14 @@ int fheacc_graph_launch(fheacc_graph *g) {
 
 int fheacc_graph_destroy(fheacc_graph *g) {
     if (!g) return 0;
-    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    (void)hipSetDevice(g->ctx ? g->ctx->device : 0);
+    if (g->exec) {
+        // a replay may still be running on the context stream
+        if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
+        (void)hipGraphExecDestroy(g->exec);
+    }
     if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (void *p : g->scratch) (void)hipFree(p);
     delete g;
     return 0;
 }

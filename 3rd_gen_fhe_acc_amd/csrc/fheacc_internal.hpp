@@ -61,6 +61,45 @@ struct Tuning {
     int ks_gather_max = 32;   // batches up to this size use the gather key switch
 };
 
+// Error text of the calling thread (fheacc_last_error) and the C ABI's
+// failure return.
+int fail(const std::string &m, int rc = FHEACC_ERR);
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            return ::fheacc::fail(std::string(#expr) + ": " + hipGetErrorString(e_),       \
+                                  e_ == hipErrorOutOfMemory ? FHEACC_ENOMEM : FHEACC_ERR);  \
+        }                                                                                   \
+    } while (0)
+
+// One gate-path launch (blind rotation + key switch + mod switch) on an
+// explicit stream and scratch buffer.  Dense batches leave the gathered
+// fields null; the serving loop fills them.
+struct GateLaunch {
+    const int32_t *gates = nullptr;
+    const uint64_t *c1 = nullptr, *c2 = nullptr;
+    uint64_t *out = nullptr;
+    int64_t B = 0;
+    const fheacc_capture *cap = nullptr;
+    // gathered form: operand / output rows and per-ciphertext key sets
+    const int64_t *idx1 = nullptr, *idx2 = nullptr, *oidx = nullptr;
+    const int32_t *key_of = nullptr;
+    const void *const *bsk_tab = nullptr;  // device array of device key pointers
+    int *ct_status = nullptr;              // device [B]: 1 where the operands are identical
+    // key-switch segments: ciphertexts [seg_off[s], seg_off[s+1]) use seg_ksk[s]
+    int nseg = 0;
+    const int64_t *seg_off = nullptr;  // host
+    const void *const *seg_ksk = nullptr;  // host array of device pointers
+    void *scratch = nullptr;  // gate_scratch_bytes(p, B) bytes of device memory
+    hipStream_t stream = nullptr;
+    bool profile = false;
+};
+size_t gate_scratch_bytes(const HostParams &p, int64_t B);
+// Launches only: no locking, no allocation, no synchronisation.
+int launch_gate_path(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch &g);
+
 }  // namespace fheacc
 
 struct fheacc_ctx {

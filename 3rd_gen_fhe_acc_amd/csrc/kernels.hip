@@ -313,7 +313,8 @@ __global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
     const uint32_t q = a.q, qm = a.q - 1;
     const int g = a.gates[c];
     const bool xr = (g == 4 || g == 5);
-    const uint64_t *x1 = a.c1 + c * (n + 1), *x2 = a.c2 + c * (n + 1);
+    const uint64_t *x1 = a.c1 + (a.idx1 ? a.idx1[c] : c) * (n + 1);
+    const uint64_t *x2 = a.c2 + (a.idx2 ? a.idx2[c] : c) * (n + 1);
     int same = 1;
     for (int k = threadIdx.x; k <= n; k += NT) {
         uint64_t u = x1[k], v = x2[k];
@@ -322,7 +323,11 @@ __global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
         if (k < n) av[k] = (uint16_t)r;
         else s_b = r;
     }
-    if (__syncthreads_and(same) && threadIdx.x == 0) atomicOr(a.status, 1);
+    const int all_same = __syncthreads_and(same);
+    if (threadIdx.x == 0) {
+        if (a.ct_status) a.ct_status[c] = all_same;
+        else if (all_same) atomicOr(a.status, 1);
+    }
     // accumulator init: only b carries the test polynomial
     const uint64_t Q8 = a.A.Q / 8 + 1, nQ8 = a.A.Q - Q8;
     const uint32_t b = s_b;
@@ -338,7 +343,7 @@ __global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
     __syncthreads();
     ntt_forward<LOGN, NT>(acc + pad(N), 1, a.A, a.tw);
     // blind rotation: i outer, j inner, zero residues skipped
-    const W *bsk = a.bsk;
+    const W *bsk = a.key_of ? a.bsk_tab[a.key_of[c]] : a.bsk;
     for (int i = 0; i < n; i++) {
         uint32_t nega = (q - av[i]) & qm;
         for (int j = 0; j < a.digitR; j++) {
@@ -480,10 +485,11 @@ __global__ __launch_bounds__(NT) void k_lwe_keyswitch_staged(KSArgs<W> a) {
         __syncthreads();
     }
     if (!live) return;
+    const int64_t orow = a.oidx ? a.oidx[c] : c;
 #pragma unroll
     for (int j = 0; j < KB; j++) {
         const int col = col0 + j;
-        if (col <= a.n) a.out[c * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s[j], a.A));
+        if (col <= a.n) a.out[orow * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s[j], a.A));
     }
 }
 
@@ -496,7 +502,7 @@ __global__ __launch_bounds__(NT) void k_lwe_keyswitch_gather(KSArgs<W> a) {
     uint64_t s = 0;
     if (col <= a.n) {
         for (int st = 0; st < a.S; st++) ks_add<W>(s, a.ksk[((size_t)st * a.Bks + dg[st]) * a.n_pad + col], a.Q);
-        a.out[c * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s, a.A));
+        a.out[(a.oidx ? a.oidx[c] : c) * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s, a.A));
     }
 }
 

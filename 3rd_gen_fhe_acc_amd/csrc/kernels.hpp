@@ -30,6 +30,14 @@ struct BRArgs {
     int dks;
     uint32_t Bks;
     int *status;
+    // Gathered batches (the serving loop): ciphertext c reads its operands
+    // from rows idx1[c], idx2[c] of c1 / c2 and the bootstrap key
+    // bsk_tab[key_of[c]], and reports identical operands in ct_status[c].
+    // All null for a dense batch.
+    const int64_t *idx1, *idx2;
+    const int32_t *key_of;
+    const W *const *bsk_tab;
+    int *ct_status;
 };
 
 template <typename W>
@@ -43,6 +51,7 @@ struct KSArgs {
     uint64_t Q;
     int lq1;
     uint64_t *out, *cap_ks;
+    const int64_t *oidx;  // output row of ciphertext c (null: row c)
     int64_t B;
     int CB, KB, SB;
 };

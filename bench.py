@@ -10,9 +10,11 @@ already resident in HBM.  Keys come from the library's seeded device key
 generator.
 
 Also measured: the same gate bootstrap one ciphertext per launch (the plain
-form), its p50/p99 latency, per-kernel HIP-event times, and a CPU baseline on
-a bounded sample (the reference's own eval_bootstrap compiled from its sources
-when oracle/_ref was built, else the oracle's C restatement).
+form), its p50/p99 latency, per-kernel HIP-event times, the serving loop under
+Poisson load (8 clients with their own resident keys, k-bit set-equality
+jobs, p50/p99 gate and request latency), and a CPU baseline on a bounded
+sample (the reference's own eval_bootstrap compiled from its sources when
+oracle/_ref was built, else the oracle's C restatement).
 
 Prints one JSON line.
 """
@@ -41,11 +43,18 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--single", type=int, default=24, help="one-per-launch gate bootstraps to time")
-    ap.add_argument("--cpu-sample", type=int, default=6, help="gates in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=20, help="gates in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", default="auto",
                     help="'auto' (run the autotuner) or THREADS,POLYS_PER_BATCH for the blind-rotation kernel")
     ap.add_argument("--no-microbench", action="store_true", help="skip the NTT / external-product microbench")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="serving-loop arrival window (0: skip)")
+    ap.add_argument("--serve-qps", type=float, default=6.0, help="serving-loop Poisson arrival rate")
+    ap.add_argument("--serve-budget-gb", type=float, default=8.0, help="serving-loop HBM budget for ciphertexts")
+    ap.add_argument("--serve-max-batch", type=int, default=4096)
+    ap.add_argument("--serve-inflight", type=int, default=2)
+    ap.add_argument("--serve-min-fill", type=int, default=0)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-parameter-set batch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (reference dispatch for HBM byte counters)")
     return ap.parse_args()
@@ -87,6 +96,177 @@ def microbench(fhe, torch, B, reps=5):
         "external_products_per_s": round(B / (ep_ms * 1e-3), 1),
         "external_product_modmul_per_s_T": round(B * ep_modmul / (ep_ms * 1e-3) / 1e12, 4),
         "forward_inverse_roundtrip_ok": roundtrip_ok,
+    }
+
+
+def serving(fhe, ctx, args):
+    """Serving loop: Poisson arrivals at a fixed rate of k-bit set-equality
+    jobs (k from {8, 16, 32}: XNOR of each item bit with the query bit, then
+    an AND tree per item), each for one of 8 clients with their own
+    device-resident keys (benchmark parameters).  Item counts are log-uniform
+    in [1, 1024], values uniform 32-bit from a seed.  Admission by HBM budget,
+    continuous batching across jobs and clients (fheacc_server_*)."""
+    circuits = importlib.import_module("3rd_gen_fhe_acc_amd.circuits")
+    rng = np.random.default_rng(SEED + 4)
+    clients = []
+    t0 = time.perf_counter()
+    for c in range(8):
+        s = ctx.lwe_secret(SEED + 100 + c, binary=True)
+        z = ctx.rlwe_secret(SEED + 100 + c)
+        keys = ctx.keys()
+        keys.generate(SEED + 200 + c, s, z)
+        clients.append((s, keys))
+    ctx.sync()
+    keygen_s = time.perf_counter() - t0
+    arrivals = []
+    t = rng.exponential(1.0 / args.serve_qps)
+    while t < args.serve_seconds:
+        arrivals.append(t)
+        t += rng.exponential(1.0 / args.serve_qps)
+    reqs = []
+    cache = {}
+    t0 = time.perf_counter()
+    for i, ta in enumerate(arrivals):
+        c = int(rng.integers(0, 8))
+        k = int(rng.choice([8, 16, 32]))
+        m = int(min(1024, max(1, np.floor(np.exp(rng.uniform(0.0, np.log(1025.0)))))))
+        items = rng.integers(0, 2 ** 32, m, dtype=np.uint64)
+        query = int(rng.integers(0, 2 ** 32, dtype=np.uint64))
+        if (m, k) not in cache:
+            cache[(m, k)] = circuits.equality_circuit(m, k)
+        n_in, ops, outs = cache[(m, k)]
+        bits = circuits.equality_inputs(items, query, k)
+        inputs = ctx.lwe_encrypt(clients[c][0], bits, SEED + 300, i << 20)
+        mask = (1 << k) - 1
+        want = np.array([(int(v) & mask) == (query & mask) for v in items], dtype=np.uint8)
+        reqs.append(dict(t=ta, client=c, k=k, m=m, inputs=inputs, ops=ops, outs=outs, want=want))
+    encrypt_s = time.perf_counter() - t0
+    srv = fhe.Server(ctx, hbm_budget=int(args.serve_budget_gb * 2 ** 30), max_batch=args.serve_max_batch,
+                     max_inflight=args.serve_inflight, min_fill=args.serve_min_fill)
+    ids = [srv.add_client(k) for _, k in clients]
+    # warm the path once, then measure from a clean slate
+    r0 = min(reqs, key=lambda r: len(r["ops"]))
+    warm, _ = srv.wait(srv.submit(ids[r0["client"]], r0["inputs"], r0["ops"], r0["outs"]))
+    srv.reset_stats()
+    jids = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        dt = t0 + r["t"] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        jids.append(srv.submit(ids[r["client"]], r["inputs"], r["ops"], r["outs"]))
+    outs = [srv.wait(j)[0] for j in jids]
+    elapsed = time.perf_counter() - t0
+    st = srv.stats()
+    ok = all(np.array_equal(ctx.lwe_decrypt(clients[r["client"]][0], o), r["want"]) for r, o in zip(reqs, outs))
+    # the same requests alone, on an idle server: bit-identical words
+    sample = sorted(range(len(reqs)), key=lambda i: len(reqs[i]["ops"]))[:3]
+    alone_eq = True
+    for i in sample:
+        r = reqs[i]
+        o, _ = srv.wait(srv.submit(ids[r["client"]], r["inputs"], r["ops"], r["outs"]))
+        alone_eq &= bool(np.array_equal(o, outs[i]))
+    alone_eq &= bool(np.array_equal(warm, outs[reqs.index(r0)]))
+    srv.close()
+    gates = sum(len(r["ops"]) for r in reqs)
+    return {
+        "workload": "Poisson %.1f req/s for %.0f s: k-bit set equality (k in {8,16,32}, XNOR + AND tree), "
+                    "items log-uniform in [1,1024], 8 clients with resident keys, benchmark parameters"
+                    % (args.serve_qps, args.serve_seconds),
+        "requests": len(reqs), "bootstraps": int(st["bootstraps"]),
+        "offered_bootstraps_per_s": round(gates / args.serve_seconds, 1),
+        "bootstraps_per_s": round(st["bootstraps"] / elapsed, 1),
+        "gate_latency_ms_p50": round(st["gate_ms_p50"], 3), "gate_latency_ms_p99": round(st["gate_ms_p99"], 3),
+        "request_latency_ms_p50": round(st["job_ms_p50"], 3),
+        "request_latency_ms_p99": round(st["job_ms_p99"], 3),
+        "elapsed_s": round(elapsed, 3), "gpu_busy_s": round(st["gpu_busy_s"], 3),
+        "launches": int(st["batches"]), "mean_launch_size": round(st["bootstraps"] / max(st["batches"], 1), 1),
+        "max_launch_size": int(st["max_batch_seen"]),
+        "hbm_budget_bytes": int(st["pool_bytes"]), "peak_admitted_bytes": int(st["peak_bytes"]),
+        "config": {"max_batch": args.serve_max_batch, "max_inflight": args.serve_inflight,
+                   "min_fill": args.serve_min_fill},
+        "decrypt_ok": bool(ok), "alone_equals_mix": bool(alone_eq),
+        "client_keygen_s": round(keygen_s, 3), "encrypt_s": round(encrypt_s, 3),
+    }
+
+
+MIXED = {
+    "A": dict(PARAMS),
+    "B": dict(n=630, q=1024, N=2048, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=32, Bks=32),
+    "C": dict(n=500, q=1024, N=512, Q=0, nbit=27, BGbits=6, Br=32, Bks=32),
+}
+MIXED_SHARE = {"A": 0.6, "B": 0.3, "C": 0.1}
+
+
+def mixed(fhe, torch, main_ctx, main_keys, total=8192, reps=3):
+    """Uneven mixed batch in one launch: 60 % set A (the benchmark set), 30 %
+    set B (N=2048, 54-bit Q, d=3, n=630), 10 % set C (N=512, 27-bit Q, d=5),
+    8192 seeded random-bit ciphertexts uniformly interleaved, captured as one
+    HIP graph with a branch per set, each at its autotuned launch shape."""
+    rng = np.random.default_rng(SEED + 5)
+    names = list(MIXED)
+    counts = [int(round(total * MIXED_SHARE[k])) for k in names]
+    counts[0] += total - sum(counts)
+    lab = np.concatenate([np.full(c, i) for i, c in enumerate(counts)])
+    rng.shuffle(lab)
+    sets = []
+    t0 = time.perf_counter()
+    for i, nm in enumerate(names):
+        B = int(np.count_nonzero(lab == i))
+        if nm == "A":
+            ctx, keys = main_ctx, main_keys
+            s = ctx.lwe_secret(SEED, binary=True)
+        else:
+            ctx = fhe.Context(**MIXED[nm])
+            s, z = ctx.lwe_secret(SEED + 10 + i, binary=True), ctx.rlwe_secret(SEED + 10 + i)
+            keys = ctx.keys()
+            keys.generate(SEED + 20 + i, s, z)
+            ctx.autotune(keys, min(B, 1024))
+        m1, m2 = rng.integers(0, 2, B), rng.integers(0, 2, B)
+        gates = rng.integers(0, 6, B).astype(np.int32)
+        c1, c2 = ctx.lwe_encrypt(s, m1, SEED + 30 + i, 0), ctx.lwe_encrypt(s, m2, SEED + 30 + i, B)
+        d1 = torch.from_numpy(c1.view(np.int64)).cuda()
+        d2 = torch.from_numpy(c2.view(np.int64)).cuda()
+        truth = {0: m1 | m2, 1: m1 & m2, 2: 1 - (m1 | m2), 3: 1 - (m1 & m2), 4: m1 ^ m2, 5: 1 - (m1 ^ m2)}
+        exp = np.array([truth[int(g)][t] for t, g in enumerate(gates)])
+        sets.append(dict(name=nm, ctx=ctx, keys=keys, s=s, B=B, dg=torch.from_numpy(gates).cuda(), d1=d1, d2=d2,
+                         out=torch.zeros_like(d1), exp=exp))
+    setup_s = time.perf_counter() - t0
+    g = fhe.MixedGraph([(st["ctx"], st["keys"], st["dg"], st["d1"], st["d2"], st["out"]) for st in sets])
+    g.launch()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        g.launch()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    ok = True
+    for st in sets:
+        st["ctx"].sync()
+        ok &= bool(np.array_equal(st["ctx"].lwe_decrypt(st["s"], st["out"].cpu().numpy().view(np.uint64)),
+                                  st["exp"]))
+    # each set alone (its dense launch), back to back: what one launch saves
+    per_set = {}
+    dense_equal = True
+    for st in sets:
+        dout = torch.zeros_like(st["d1"])
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        st["ctx"].bootstrap_dev(st["keys"], st["dg"], st["d1"], st["d2"], dout)
+        e1.record()
+        torch.cuda.synchronize()
+        per_set[st["name"]] = {"B": st["B"], "alone_ms": round(e0.elapsed_time(e1), 3),
+                               "tuning": st["ctx"].get_tuning()}
+        dense_equal &= bool(torch.equal(dout, st["out"]))
+    g.close()
+    return {
+        "workload": "8192 ciphertexts uniformly interleaved: 60% N=1024 27-bit d=4 n=500, 30% N=2048 54-bit d=3 "
+                    "n=630, 10% N=512 27-bit d=5 n=500; one graph launch, a branch per set",
+        "ms_per_launch": round(ms, 3), "bootstraps_per_s": round(total / (ms * 1e-3), 1),
+        "sets": per_set, "sum_alone_ms": round(sum(v["alone_ms"] for v in per_set.values()), 3),
+        "decrypt_ok": bool(ok), "equals_per_set_launch": bool(dense_equal), "setup_s": round(setup_s, 2),
     }
 
 
@@ -271,7 +451,7 @@ def main():
         "speedup_vs_single_per_launch": round(value / single_rate, 2),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                        "p99": round(float(np.percentile(lat, 99)), 4),
-                       "note": "single gate, idle GPU (one ciphertext per launch)"},
+                       "note": "single gate, idle GPU (one ciphertext per launch); under load: serving"},
         "batch_latency_ms": round(ms_step, 4),
         "decrypt_truth_table_ok": truth_ok,
         "graph_equals_eager": graph_equals_eager,
@@ -279,8 +459,18 @@ def main():
         "keygen_s": round(keygen_s, 3),
         "tuning": tune_report,
     }
+    if args.serve_seconds > 0:
+        rec["serving"] = serving(fhe, ctx, args)
+        rec["latency_ms"] = {"p50": rec["serving"]["gate_latency_ms_p50"],
+                             "p99": rec["serving"]["gate_latency_ms_p99"],
+                             "request_p50": rec["serving"]["request_latency_ms_p50"],
+                             "request_p99": rec["serving"]["request_latency_ms_p99"],
+                             "note": "gate ready -> done and request submit -> result, serving loop under "
+                                     "Poisson load; idle one-per-launch figures: single_per_launch"}
     if not args.no_microbench:
         rec["kernel_microbench"] = microbench(fhe, torch, B)
+    if not args.no_mixed:
+        rec["mixed"] = mixed(fhe, torch, ctx, keys)
     if not args.no_cpu_baseline and args.cpu_sample > 0:
         rec["cpu_baseline"] = cpu_baseline(ctx, keys, c1, c2, gates, out, args.cpu_sample)
     print(json.dumps(rec))

@@ -159,6 +159,15 @@ int fheacc_bootstrap_batch_dev(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t
 typedef struct fheacc_graph fheacc_graph;
 int fheacc_graph_capture_bootstrap(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t *gates, const uint64_t *c1,
                                    const uint64_t *c2, uint64_t *out, int64_t B, fheacc_graph **graph);
+/* Mixed parameter sets in one launch: set i is a dense batch of B[i] pairs
+ * under its own context and keys (each context one parameter set, all on one
+ * device).  One graph holds a branch per set, forked from one root, so the
+ * sets run concurrently and the graph launches (on ctxs[0]'s stream) as one
+ * unit.  Each branch runs at its context's tuning.  Each set's words are
+ * those of fheacc_bootstrap_batch_dev on its own. */
+int fheacc_graph_capture_mixed(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *keys,
+                               const int32_t *const *gates, const uint64_t *const *c1, const uint64_t *const *c2,
+                               uint64_t *const *out, const int64_t *B, fheacc_graph **graph);
 int fheacc_graph_launch(fheacc_graph *graph);
 int fheacc_graph_destroy(fheacc_graph *graph);
 
@@ -173,6 +182,54 @@ int fheacc_external_product_dev(fheacc_ctx *ctx, const uint64_t *acc, const uint
  * (RLWE.cpp:444-510; the accelerator's RLWESUBS, myfpga.cpp:1662-1795). */
 int fheacc_subs_keyswitch_dev(fheacc_ctx *ctx, fheacc_keys *keys, const uint64_t *in, int sub_factor,
                               uint64_t *out, int64_t count);
+
+/* ---- serving loop -------------------------------------------------------
+ * No counterpart in the reference: its apps evaluate one gate
+ * (RLWE::eval_bootstrap, RLWE.cpp:726-786) or one fixed batch
+ * (fpga_eval_bootstrap_xN, myfpga.cpp:1003-1660) per call.  A request loop
+ * over one context for callers that serve many clients:
+ *  - clients register their resident keys once (fheacc_server_add_client);
+ *  - a job is a gate circuit: n_in input LWE ciphertexts ([n_in][n+1] words
+ *    mod q) and n_gates gates, ops[g] = {gate, src1, src2}, whose operands
+ *    are earlier nodes (node i < n_in is input i, node n_in + g is gate g's
+ *    output; src1 != src2).  outputs[] names the nodes copied to out
+ *    ([n_out][n+1] words), which must stay valid until fheacc_server_wait;
+ *  - a job is admitted, in arrival order, when its n_in + n_gates
+ *    ciphertexts fit in what is left of hbm_budget;
+ *  - a worker thread coalesces the ready gates of all admitted jobs, of all
+ *    clients, into gate-path launches of at most max_batch; up to
+ *    max_inflight launches (one stream each) run at once, so requests that
+ *    arrive while others are on the GPU join the next launch.
+ * Every gate is the same evaluation as fheacc_bootstrap_batch, word for word.
+ * Jobs must be collected with fheacc_server_wait before
+ * fheacc_server_destroy, which fails jobs that have not finished. */
+typedef struct fheacc_server fheacc_server;
+typedef struct fheacc_server_config {
+    uint64_t hbm_budget;   /* bytes of device memory for job ciphertexts */
+    int32_t max_batch;     /* most gate bootstraps per launch */
+    int32_t max_inflight;  /* launches in flight at once, 1..4 */
+    int32_t min_fill;      /* with a launch in flight, wait for this many ready gates */
+} fheacc_server_config;
+typedef struct fheacc_server_stats {
+    int64_t jobs_submitted, jobs_done, jobs_failed, jobs_queued, jobs_active;
+    int64_t bootstraps, batches, max_batch_seen;
+    double wall_s;                 /* since creation or the last reset */
+    double gpu_busy_s;             /* wall time with a launch in flight */
+    double gate_ms_p50, gate_ms_p99;  /* gate ready -> its launch complete */
+    double job_ms_p50, job_ms_p99;    /* submit -> outputs in host memory */
+    uint64_t pool_bytes, peak_bytes;  /* pool size; peak admitted footprint */
+} fheacc_server_stats;
+int fheacc_server_create(fheacc_ctx *ctx, const fheacc_server_config *cfg, fheacc_server **server);
+int fheacc_server_destroy(fheacc_server *server);
+int fheacc_server_add_client(fheacc_server *server, fheacc_keys *keys, int32_t *client);
+int fheacc_server_submit(fheacc_server *server, int32_t client, const uint64_t *inputs, int64_t n_in,
+                         const int32_t *ops, int64_t n_gates, const int64_t *outputs, int64_t n_out,
+                         uint64_t *out, int64_t *job);
+/* Blocks until the job is done; returns its status (identical operands:
+ * -1, RLWE.cpp:727-730).  latency_ms: submit -> outputs in host memory. */
+int fheacc_server_wait(fheacc_server *server, int64_t job, double *latency_ms);
+int fheacc_server_get_stats(fheacc_server *server, fheacc_server_stats *stats);
+int fheacc_server_reset_stats(fheacc_server *server);
 
 #ifdef __cplusplus
 }

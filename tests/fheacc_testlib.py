@@ -20,8 +20,19 @@ SETS = {
     "n10_N128_q54": dict(n=10, q=256, N=128, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=16, Bks=32),
     # ring degree 2048, 54-bit Q (mixed-set B shape, n reduced)
     "n8_N2048_q54": dict(n=8, q=1024, N=2048, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=32, Bks=32),
+    # ring degree 512, 27-bit Q, BG=2^6 d=5 (mixed-set C shape, n reduced)
+    "n8_N512_q27_bg6": dict(n=8, q=1024, N=512, Q=0, nbit=27, BGbits=6, Br=32, Bks=32),
 }
 
 # full-size configurations
 CFG0 = dict(n=500, q=1024, N=1024, Q=0, nbit=27, BGbits=7, Br=32, Bks=32)
 CFG1 = dict(n=8, q=2048, N=1024, Q=0x3FFFFFFFFED001, nbit=54, BGbits=9, Br=32, Bks=32)
+
+# the mixed batch: A = the benchmark set; B = N=2048, 54-bit Q, d=3, n=630;
+# C = N=512, 27-bit Q, d=5
+MIXED = {
+    "A": CFG0,
+    "B": dict(n=630, q=1024, N=2048, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=32, Bks=32),
+    "C": dict(n=500, q=1024, N=512, Q=0, nbit=27, BGbits=6, Br=32, Bks=32),
+}
+MIXED_SHARE = {"A": 0.6, "B": 0.3, "C": 0.1}

@@ -135,7 +135,8 @@ def run_bootstrap_case(name, B, seed, upload=False):
     assert np.array_equal(ctx.lwe_decrypt(s, out), exp_bits)
 
 
-@pytest.mark.parametrize("name", ["n16_N256_q27", "n12_N512_q27", "n10_N128_q54", "n8_N2048_q54"])
+@pytest.mark.parametrize("name", ["n16_N256_q27", "n12_N512_q27", "n10_N128_q54", "n8_N2048_q54",
+                                  "n8_N512_q27_bg6"])
 def test_bootstrap_matches_oracle(name):
     run_bootstrap_case(name, B=24, seed=3)
 

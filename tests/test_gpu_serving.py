@@ -1,0 +1,150 @@
+"""Serving loop (fheacc_server_*) on the GPU, benchmark parameters (n=500,
+N=1024, 27-bit Q, Br=32).
+
+Each job's output words are the same when it runs alone and when it shares
+launches with other clients' jobs. They also equal the dense gate path
+(fheacc_bootstrap_batch, itself pinned against the oracle) evaluated level by
+level. Every decrypted output is the plaintext equality. Exact modular
+arithmetic, so "the same" is bit-exact: the tolerance is zero."""
+import importlib
+
+import numpy as np
+import pytest
+
+from fheacc_testlib import CFG0, fhe
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+circuits = importlib.import_module("3rd_gen_fhe_acc_amd.circuits")
+
+
+@pytest.fixture(scope="module")
+def serving():
+    ctx = fhe.Context(**CFG0)
+    clients = []
+    for c in range(3):
+        s, z = ctx.lwe_secret(500 + c), ctx.rlwe_secret(500 + c)
+        keys = ctx.keys()
+        keys.generate(600 + c, s, z)
+        clients.append((s, keys))
+    return ctx, clients
+
+
+def make_jobs(ctx, clients, specs, seed):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j, (c, m, k) in enumerate(specs):
+        items = rng.integers(0, 2 ** 32, m, dtype=np.uint64)
+        query = int(rng.integers(0, 2 ** 32, dtype=np.uint64))
+        mask = (1 << k) - 1
+        items[0] = (int(items[0]) & ~mask) | (query & mask)  # one planted match
+        n_in, ops, outs = circuits.equality_circuit(m, k)
+        bits = circuits.equality_inputs(items, query, k)
+        inputs = ctx.lwe_encrypt(clients[c][0], bits, seed + 1, 4096 * j)
+        want = np.array([(int(v) & mask) == (query & mask) for v in items], dtype=np.uint8)
+        jobs.append(dict(client=c, n_in=n_in, inputs=inputs, ops=ops, outs=outs, want=want))
+    return jobs
+
+
+def dense_eval(ctx, keys, job):
+    """The same circuit through fheacc_bootstrap_batch, one launch per level."""
+    n_in, ops = job["n_in"], job["ops"]
+    words = [row for row in job["inputs"]] + [None] * len(ops)
+    depth = [0] * n_in
+    for g, a, b in ops:
+        depth.append(1 + max(depth[a], depth[b]))
+    for lvl in range(1, max(depth) + 1):
+        gs = [g for g in range(len(ops)) if depth[n_in + g] == lvl]
+        c1 = np.stack([words[ops[g][1]] for g in gs])
+        c2 = np.stack([words[ops[g][2]] for g in gs])
+        out = ctx.bootstrap(keys, ops[gs, 0].astype(np.int32), c1, c2)
+        for t, g in enumerate(gs):
+            words[n_in + g] = out[t]
+    return np.stack([words[o] for o in job["outs"]])
+
+
+SPECS = [(0, 3, 8), (1, 2, 16), (2, 5, 8), (0, 1, 32), (1, 4, 8), (2, 2, 16), (0, 6, 8)]
+
+
+def test_alone_equals_mix_equals_dense(serving):
+    ctx, clients = serving
+    jobs = make_jobs(ctx, clients, SPECS, 71)
+    srv = fhe.Server(ctx, hbm_budget=1 << 28, max_batch=256, max_inflight=2)
+    ids = [srv.add_client(k) for _, k in clients]
+    try:
+        alone = []
+        for jb in jobs:
+            out, lat = srv.wait(srv.submit(ids[jb["client"]], jb["inputs"], jb["ops"], jb["outs"]))
+            assert lat > 0
+            alone.append(out)
+        jids = [srv.submit(ids[jb["client"]], jb["inputs"], jb["ops"], jb["outs"]) for jb in jobs]
+        mix = [srv.wait(j)[0] for j in jids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    for a, b, jb in zip(alone, mix, jobs):
+        assert np.array_equal(a, b)
+        assert np.array_equal(ctx.lwe_decrypt(clients[jb["client"]][0], b), jb["want"])
+    for j in (1, 3):
+        jb = jobs[j]
+        assert np.array_equal(dense_eval(ctx, clients[jb["client"]][1], jb), alone[j])
+    total_gates = sum(len(jb["ops"]) for jb in jobs)
+    assert st["jobs_done"] == 2 * len(jobs) and st["jobs_failed"] == 0
+    assert st["bootstraps"] == 2 * total_gates
+    assert st["gate_ms_p50"] > 0 and st["job_ms_p99"] >= st["job_ms_p50"]
+
+
+def test_hbm_budget_queues_jobs(serving):
+    # a budget that admits roughly one job at a time: the rest wait their turn
+    ctx, clients = serving
+    specs = [(c % 3, 4, 8) for c in range(6)]
+    jobs = make_jobs(ctx, clients, specs, 72)
+    rows = max(jb["n_in"] + len(jb["ops"]) for jb in jobs)
+    budget = (rows + 10) * (ctx.n + 1) * 8
+    srv = fhe.Server(ctx, hbm_budget=budget, max_batch=64, max_inflight=2)
+    ids = [srv.add_client(k) for _, k in clients]
+    try:
+        jids = [srv.submit(ids[jb["client"]], jb["inputs"], jb["ops"], jb["outs"]) for jb in jobs]
+        outs = [srv.wait(j)[0] for j in jids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    for o, jb in zip(outs, jobs):
+        assert np.array_equal(ctx.lwe_decrypt(clients[jb["client"]][0], o), jb["want"])
+    assert st["peak_bytes"] <= st["pool_bytes"] <= budget
+    assert st["jobs_done"] == len(jobs)
+
+
+def test_errors(serving):
+    ctx, clients = serving
+    srv = fhe.Server(ctx, hbm_budget=1 << 24, max_batch=64, max_inflight=1)
+    cid = srv.add_client(clients[0][1])
+    s = clients[0][0]
+    try:
+        x = ctx.lwe_encrypt(s, np.array([1, 0, 1], dtype=np.uint8), 90, 0)
+        with pytest.raises(fhe.FheAccError, match="identical operands"):
+            srv.submit(cid, x, [[circuits.AND, 1, 1]], [3])
+        with pytest.raises(fhe.FheAccError, match="earlier nodes"):
+            srv.submit(cid, x, [[circuits.AND, 0, 3]], [3])
+        with pytest.raises(fhe.FheAccError, match="unknown client"):
+            srv.submit(cid + 5, x, [[circuits.AND, 0, 1]], [3])
+        big = np.zeros((5000, ctx.n + 1), dtype=np.uint64)
+        with pytest.raises(fhe.FheAccError, match="HBM budget"):
+            srv.submit(cid, big, np.zeros((0, 3), dtype=np.int32), [0])
+        # two input ciphertexts with the same words: the kernel flags the gate,
+        # the job fails (the reference exits, RLWE.cpp:727-730), other jobs run
+        dup = np.stack([x[0], x[0], x[1]])
+        bad = srv.submit(cid, dup, [[circuits.NAND, 0, 1]], [3])
+        good = srv.submit(cid, x, [[circuits.NAND, 0, 1], [circuits.AND, 3, 2]], [3, 4])
+        with pytest.raises(fhe.FheAccError, match="identical operands"):
+            srv.wait(bad)
+        out, _ = srv.wait(good)
+        assert list(ctx.lwe_decrypt(s, out)) == [1, 1]
+        # a circuit with no gates returns its inputs
+        out, _ = srv.wait(srv.submit(cid, x, np.zeros((0, 3), dtype=np.int32), [2, 0]))
+        assert np.array_equal(out, x[[2, 0]])
+        st = srv.stats()
+        assert st["jobs_failed"] == 1
+    finally:
+        srv.close()

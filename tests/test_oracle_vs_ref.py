@@ -21,6 +21,7 @@ SETS = [
     dict(n=16, q=512, N=256, Q=0, nbit=27, BGbits=9, Br=23, Bks=25),
     dict(n=12, q=1024, N=512, Q=0, nbit=27, BGbits=7, Br=32, Bks=32),
     dict(n=10, q=256, N=128, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=16, Bks=32),
+    dict(n=8, q=1024, N=512, Q=0, nbit=27, BGbits=6, Br=32, Bks=32),
 ]
 
 
