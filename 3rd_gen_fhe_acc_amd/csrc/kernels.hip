@@ -116,19 +116,29 @@ template <typename W>
 struct KeyQuad {
     W v[4], p[4];
 };
+// Key pointers may come from a table in memory (per-client keys), where the
+// compiler cannot see that they are global and would emit flat loads, which
+// also count against the LDS wait counter; these go through global loads.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ T ld_global(const void *p, size_t i) {
+    typedef const __attribute__((address_space(1))) T global_t;
+    return ((global_t *)p)[i];
+}
+
 __device__ __forceinline__ KeyQuad<uint32_t> load_quad(const uint32_t *key, int idx) {
-    uint4 q = reinterpret_cast<const uint4 *>(key)[idx];
+    const u32x4 q = ld_global<u32x4>(key, (size_t)idx);
     KeyQuad<uint32_t> r;
     r.v[0] = q.x; r.v[1] = q.y; r.v[2] = q.z; r.v[3] = q.w;
     r.p[0] = r.p[1] = r.p[2] = r.p[3] = 0;
     return r;
 }
 __device__ __forceinline__ KeyQuad<uint64_t> load_quad(const uint64_t *key, int idx) {
-    const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(key) + 4 * (size_t)idx;
     KeyQuad<uint64_t> r;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        ulonglong2 t = p[i];
+        const u64x2 t = ld_global<u64x2>(key, 4 * (size_t)idx + i);
         r.v[i] = t.x;
         r.p[i] = t.y;
     }
@@ -246,45 +256,84 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     constexpr int CPT = (N + NT - 1) / NT;
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));  // see ext_prod
+    // with N a multiple of NT every k = tid + c NT is a coefficient; the
+    // opaque tid hides that from the compiler, so say it here
+    constexpr bool kFull = N % NT == 0;
     ntt_inverse<LOGN, NT>(buf, 2, A, tw, tid);
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
     W xa[CPT], xb[CPT];
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
         const int k = tid + c * NT;
-        if (k < N) {
+        if (kFull || k < N) {
             xa[c] = buf[pad(k)];
             xb[c] = buf[pad(N + k)];
         }
     }
     __syncthreads();
+    // digit h of a goes to poly 2h, of b to poly 2h + 1: a poly is a multiple
+    // of 8 words, so pad(2hN + k) = pad(k) + 2h padded-poly strides, and the
+    // unrolled writes become immediate LDS offsets
+    constexpr int PS = N + N / 8;  // padded words per poly
+    constexpr int DU = 8;          // digit count unrolled (larger d loops)
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
         const int k = tid + c * NT;
-        if (k < N) {
-            for (int h = 0; h < d; h++) {
-                buf[pad(((2 * h) << LOGN) + k)] = (xa[c] >> (h * bits)) & mask;
-                buf[pad(((2 * h + 1) << LOGN) + k)] = (xb[c] >> (h * bits)) & mask;
+        if (kFull || k < N) {
+            W *col = buf + pad(k);
+            if (d <= DU) {
+#pragma unroll
+                for (int h = 0; h < DU; h++) {
+                    if (h < d) {
+                        col[2 * h * PS] = (xa[c] >> (h * bits)) & mask;
+                        col[(2 * h + 1) * PS] = (xb[c] >> (h * bits)) & mask;
+                    }
+                }
+            } else {
+                for (int h = 0; h < d; h++) {
+                    col[2 * h * PS] = (xa[c] >> (h * bits)) & mask;
+                    col[(2 * h + 1) * PS] = (xb[c] >> (h * bits)) & mask;
+                }
             }
         }
     }
     __syncthreads();
     ntt_forward<LOGN, NT>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
+    // gadget rows outer, the thread's coefficients inner: row h+1's key quads
+    // are in flight while row h is accumulated
+    MacSum<W> sa[CPT], sb[CPT];
+    KeyQuad<W> cur[CPT], nxt[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++)
+        if (kFull || tid + c * NT < N) cur[c] = load_quad(key, tid + c * NT);
+#pragma unroll 1
+    for (int h = 0; h < d; h++) {
+        if (h + 1 < d) {
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+                if (kFull || tid + c * NT < N) nxt[c] = load_quad(key, ((h + 1) << LOGN) + tid + c * NT);
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int k = tid + c * NT;
+            if (kFull || k < N) {
+                const W da = buf[pad(((2 * h) << LOGN) + k)], db = buf[pad(((2 * h + 1) << LOGN) + k)];
+                sa[c].add(A, da, cur[c].v[0], cur[c].p[0]);
+                sb[c].add(A, da, cur[c].v[1], cur[c].p[1]);
+                sa[c].add(A, db, cur[c].v[2], cur[c].p[2]);
+                sb[c].add(A, db, cur[c].v[3], cur[c].p[3]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++) cur[c] = nxt[c];
+    }
+    // every digit of column k has been read by its owner: overwrite in place
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
         const int k = tid + c * NT;
-        if (k < N) {
-            MacSum<W> sa, sb;
-            for (int h = 0; h < d; h++) {
-                const KeyQuad<W> kq = load_quad(key, (h << LOGN) + k);
-                const W da = buf[pad(((2 * h) << LOGN) + k)], db = buf[pad(((2 * h + 1) << LOGN) + k)];
-                sa.add(A, da, kq.v[0], kq.p[0]);
-                sb.add(A, da, kq.v[1], kq.p[1]);
-                sa.add(A, db, kq.v[2], kq.p[2]);
-                sb.add(A, db, kq.v[3], kq.p[3]);
-            }
-            buf[pad(k)] = sa.get(A);
-            buf[pad(N + k)] = sb.get(A);
+        if (kFull || k < N) {
+            buf[pad(k)] = sa[c].get(A);
+            buf[pad(N + k)] = sb[c].get(A);
         }
     }
     __syncthreads();

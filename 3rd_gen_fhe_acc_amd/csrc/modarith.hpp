@@ -35,6 +35,17 @@ struct Arith<uint32_t> {
         W qh = __umulhi(x, wp);
         return x * w - qh * Q;
     }
+    // Lazy Cooley-Tukey butterfly (a, b) -> (a + wb, a - wb + 2Q), wb = b w
+    // mod Q in [0, 2Q).  Written so the VALU needs three multiplies and three
+    // adds: a' = a + b w - qh Q in one 3-input add (the -Q factor folded into
+    // the multiply), b' = 2a + 2Q - a' in one shift-add and one subtract.
+    // Sums wrap mod 2^32 on the way but the true results fit in 32 bits.
+    __device__ __forceinline__ void ct_butterfly(W &a, W &b, W w, W wp) const {
+        const W qh = __umulhi(b, wp);
+        const W a0 = a;
+        a = a0 + b * w + qh * (0u - Q);
+        b = (a0 << 1) + Q2 - a;
+    }
     // 64-bit sum -> [0, Q)
     __device__ __forceinline__ W reduce64(uint64_t x) const {
         uint64_t qh = __umul64hi(x, mu);
@@ -59,6 +70,12 @@ struct Arith<uint64_t> {
     __device__ __forceinline__ W mul_shoup(W x, W w, W wp) const {
         W qh = __umul64hi(x, wp);
         return x * w - qh * Q;
+    }
+    __device__ __forceinline__ void ct_butterfly(W &a, W &b, W w, W wp) const {
+        const W v = mul_shoup(b, w, wp);
+        const W a0 = a;
+        a = a0 + v;
+        b = a0 - v + Q2;
     }
     // x < 2^62, any multiple of Q below that -> [0, Q)
     __device__ __forceinline__ W reduce_small(uint64_t x) const {

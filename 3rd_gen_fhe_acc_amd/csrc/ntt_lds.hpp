@@ -116,10 +116,7 @@ __device__ __forceinline__ void fwd_cluster(W *s, int b0, bool last, const Arith
         for (int e = 0; e < (1 << R); e++) {
             if (e & h) continue;
             const int j = (1 << k) - 1 + (e >> (R - k));
-            const W v = A.mul_shoup(x[e + h], t.w[j], t.wp[j]);
-            const W u = x[e];
-            x[e] = u + v;
-            x[e + h] = u - v + A.Q2;
+            A.ct_butterfly(x[e], x[e + h], t.w[j], t.wp[j]);
         }
     }
     if (last) {
