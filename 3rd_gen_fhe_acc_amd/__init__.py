@@ -104,6 +104,11 @@ EXPORTS = {
     "fheacc_ntt_dev": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int],
     "fheacc_external_product_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64],
     "fheacc_subs_keyswitch_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64],
+    "fheacc_expand_rlwe_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64],
+    "fheacc_cmux_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64],
+    "fheacc_blind_rotate_rlwe_dev": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int64],
+    "fheacc_blind_rotate_full_dev": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                     C.c_int64],
     "fheacc_server_create": [C.c_void_p, C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)],
     "fheacc_server_destroy": [C.c_void_p],
     "fheacc_server_add_client": [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)],
@@ -282,6 +287,28 @@ class Context:
         count = acc.numel() // (2 * self.N)
         stride = self.digitG * 4 * self.N if per_ciphertext_key else 0
         _check(lib().fheacc_external_product_dev(self.h, _dptr(acc), _dptr(rgsw), stride, _dptr(out), count))
+
+    def expand_rlwe_dev(self, keys, rlwe, out):
+        """expand_RLWE (app.cpp:26-97): rlwe [B][2][N] -> out [B][N][2][N]."""
+        B = rlwe.numel() // (2 * self.N)
+        _check(lib().fheacc_expand_rlwe_dev(self.h, keys.h, _dptr(rlwe), _dptr(out), B))
+
+    def cmux_dev(self, c0, c1, rgsw, out, per_ciphertext_key=False):
+        count = c0.numel() // (2 * self.N)
+        stride = self.digitG * 4 * self.N if per_ciphertext_key else 0
+        _check(lib().fheacc_cmux_dev(self.h, _dptr(c0), _dptr(c1), _dptr(rgsw), stride, _dptr(out), count))
+
+    def blind_rotate_rlwe_dev(self, rlwe, forward, rgsws, out):
+        count = rlwe.numel() // (2 * self.N)
+        steps = rgsws.numel() // (self.digitG * 4 * self.N)
+        _check(lib().fheacc_blind_rotate_rlwe_dev(self.h, _dptr(rlwe), 1 if forward else 0, _dptr(rgsws), steps,
+                                                  _dptr(out), count))
+
+    def blind_rotate_full_dev(self, rlwe, b, a, rgsws, out):
+        count = rlwe.numel() // (2 * self.N)
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1))
+        _check(lib().fheacc_blind_rotate_full_dev(self.h, _dptr(rlwe), int(b), a.ctypes.data_as(C.c_void_p),
+                                                  _dptr(rgsws) if a.size else None, a.size, _dptr(out), count))
 
     def subs_keyswitch_dev(self, keys, rlwe, sub_factor, out):
         _check(lib().fheacc_subs_keyswitch_dev(self.h, keys.h, _dptr(rlwe), sub_factor, _dptr(out),

@@ -905,6 +905,132 @@ int fheacc_subs_keyswitch_dev(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in,
 }
 
 
+// ------------------------------------------- RLWE expansion and CMUX chains
+
+template <typename W>
+static int expand_impl(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, uint64_t *out, int64_t B) {
+    const HostParams &p = c->p;
+    const size_t rw = 2 * (size_t)p.N;
+    // slot 0 of every expansion starts as its input
+    HIP_TRY(hipMemcpy2DAsync(out, (size_t)p.N * rw * 8, in, rw * 8, rw * 8, (size_t)B, hipMemcpyDeviceToDevice,
+                             c->stream));
+    SubsArgs<W> a{};
+    a.A = make_arith<W>(p);
+    a.tw = make_tw<W>(c);
+    a.in = out;
+    a.out = out;
+    a.key = (const W *)k->subs;
+    a.key_stride = (int64_t)p.digitKS_R * p.N * (p.narrow ? 2 : 4);
+    a.d = p.digitKS_R;
+    a.bits = p.Bks_R_bits;
+    a.dch = pick_pb(p, p.digitKS_R, 2 * (size_t)padded(2 * p.N) * sizeof(W), c->lds_budget, 0);
+    if ((2 * (size_t)padded(2 * p.N) + padded(a.dch * p.N)) * sizeof(W) > c->lds_budget)
+        return fail("subs key switch exceeds the LDS budget");
+    for (int i = 0; i < p.logN; i++) {
+        a.sub_factor = i;
+        Prof pr(c, 2);
+        HIP_TRY(launch_expand_level<W>(a, p.logN, B, c->stream));
+    }
+    return 0;
+}
+
+int fheacc_expand_rlwe_dev(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !k || !in || !out) return fail("null argument");
+    if (!k->subs) return fail("no subs key resident");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    if ((count << c->p.logN) > 0x7fffffff) return fail("batch too large");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return c->p.narrow ? expand_impl<uint32_t>(c, k, in, out, count) : expand_impl<uint64_t>(c, k, in, out, count);
+}
+
+// Rotation factor of poly_rotate_time (RLWE.cpp:49-68): reduced mod N with
+// the sign of C++ '%', then X^rf as the exponent in [0, 2N).
+static int rotate_exponent(int64_t rf, int N) {
+    rf %= N;
+    return (int)(rf >= 0 ? rf : 2 * N + rf);
+}
+
+template <typename W>
+static int cmux_impl(fheacc_ctx *c, const uint64_t *c0, const uint64_t *c1, const uint64_t *rgsw, int64_t nk,
+                     int64_t rgsw_stride, bool per_ct, const std::vector<int32_t> &rot, int pre_rot, uint64_t *out,
+                     int64_t count) {
+    const HostParams &p = c->p;
+    const size_t ke = rgsw_elems(p);
+    const size_t kbytes = (nk * ke * sizeof(W) + 255) / 256 * 256;
+    if (int rc = ensure_scratch(c, kbytes + rot.size() * 4 + 4)) return rc;
+    const size_t rw = (size_t)p.digitG * 4 * p.N;
+    HIP_TRY(launch_rgsw_to_device<W>(rgsw, rgsw_stride ? rgsw_stride : (int64_t)rw, (W *)c->scratch, ke, p.digitG,
+                                     p.N, make_sg(p.Q), nullptr, nk, c->stream));
+    int32_t *drot = (int32_t *)((char *)c->scratch + kbytes);
+    if (!rot.empty())
+        HIP_TRY(hipMemcpyAsync(drot, rot.data(), rot.size() * 4, hipMemcpyHostToDevice, c->stream));
+    CmuxArgs<W> a{};
+    a.A = make_arith<W>(p);
+    a.tw = make_tw<W>(c);
+    a.c0 = c0;
+    a.c1 = c1;
+    a.out = out;
+    a.key = (const W *)c->scratch;
+    a.key_stride = per_ct ? 0 : (int64_t)ke;
+    a.key_ct_stride = per_ct ? (int64_t)ke : 0;
+    a.rot = drot;
+    a.steps = c1 ? 1 : (int)rot.size();
+    a.pre_rot = pre_rot;
+    a.d = p.digitG;
+    a.bits = p.BGbits;
+    a.pb = c->tune.br_pb > 0 ? std::min(c->tune.br_pb, 2 * (int)p.digitG) : 2 * (int)p.digitG;
+    while (a.pb > 1 && br_lds_bytes(p, a.pb) > c->lds_budget) a.pb--;
+    const size_t lds = br_lds_bytes(p, a.pb);
+    if (lds > c->lds_budget) return fail("parameter set does not fit the LDS budget");
+    Prof pr(c, 2);
+    HIP_TRY(launch_cmux_chain<W>(a, p.logN, c->tune.br_threads, count, lds, c->stream));
+    return 0;
+}
+
+static int cmux_dispatch(fheacc_ctx *c, const uint64_t *c0, const uint64_t *c1, const uint64_t *rgsw, int64_t nk,
+                         int64_t stride, bool per_ct, const std::vector<int32_t> &rot, int pre_rot, uint64_t *out,
+                         int64_t count) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return c->p.narrow ? cmux_impl<uint32_t>(c, c0, c1, rgsw, nk, stride, per_ct, rot, pre_rot, out, count)
+                       : cmux_impl<uint64_t>(c, c0, c1, rgsw, nk, stride, per_ct, rot, pre_rot, out, count);
+}
+
+int fheacc_cmux_dev(fheacc_ctx *c, const uint64_t *c0, const uint64_t *c1, const uint64_t *rgsw, int64_t stride,
+                    uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !c0 || !c1 || !rgsw || !out) return fail("null argument");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    if (count > 0x7fffffff) return fail("batch too large");
+    return cmux_dispatch(c, c0, c1, rgsw, stride ? count : 1, stride, stride != 0, {}, 0, out, count);
+}
+
+int fheacc_blind_rotate_rlwe_dev(fheacc_ctx *c, const uint64_t *in, int forward, const uint64_t *rgsw, int steps,
+                                 uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !in || !rgsw || !out) return fail("null argument");
+    if (steps < 1 || steps > c->p.logN) return fail("steps must be in [1, log2 N]");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    if (count > 0x7fffffff) return fail("batch too large");
+    // rotate_poly_forward[i] = NTT(X^(2^i)), rotate_poly_backward[i] =
+    // NTT((Q - 1) X^(N - 2^i)) = NTT(X^(2N - 2^i)) (params.cpp:177-210)
+    std::vector<int32_t> rot(steps);
+    for (int i = 0; i < steps; i++) rot[i] = forward ? (1 << i) : 2 * c->p.N - (1 << i);
+    return cmux_dispatch(c, in, nullptr, rgsw, steps, 0, false, rot, 0, out, count);
+}
+
+int fheacc_blind_rotate_full_dev(fheacc_ctx *c, const uint64_t *in, int b, const int32_t *a, const uint64_t *rgsw,
+                                 int steps, uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !in || !out || (steps > 0 && (!a || !rgsw))) return fail("null argument");
+    if (steps < 0 || steps > 4096) return fail("steps must be in [0, 4096]");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    if (count > 0x7fffffff) return fail("batch too large");
+    std::vector<int32_t> rot(steps);
+    for (int i = 0; i < steps; i++) rot[i] = rotate_exponent(a[i], c->p.N);
+    // steps == 0 leaves only the public rotation (a copy when b reduces to 0)
+    return cmux_dispatch(c, in, nullptr, steps ? rgsw : in, steps, 0, false, rot, rotate_exponent(b, c->p.N), out,
+                         count);
+}
+
 // ---------------------------------------------------------------- tuning
 
 int fheacc_ctx_set_tuning(fheacc_ctx *c, int threads, int pb) {

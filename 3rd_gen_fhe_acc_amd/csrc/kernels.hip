@@ -83,6 +83,19 @@ __device__ __forceinline__ uint64_t mulmod_generic(const Arith<uint64_t> &A, uin
     return Arith<uint64_t>::red(A.mul_shoup(x, y, yp), A.Q);
 }
 
+// x X^e in the NTT domain at output index idx, e in [0, 2N): NTT(X^e)[idx] =
+// psi^((2 brev(idx) + 1) e) (bit-reversed output order, nbt.cpp:423-454), and
+// psi^t = rou[brev(t)] for t < N, -psi^(t - N) above (psi^N = -1).
+template <typename W, int LOGN>
+__device__ __forceinline__ W mul_monomial(const Arith<W> &A, const Twiddles<W> &tw, W x, int idx, int e) {
+    constexpr int N = 1 << LOGN;
+    const int br = (int)(__brev((unsigned)idx) >> (32 - LOGN));
+    const int t = (int)(((int64_t)(2 * br + 1) * e) & (2 * N - 1));
+    const int j = (int)(__brev((unsigned)(t & (N - 1))) >> (32 - LOGN));
+    const W v = Arith<W>::red(A.mul_shoup(x, tw.fw[j], tw.fwp[j]), A.Q);
+    return t >= N ? (v ? A.Q - v : (W)0) : v;
+}
+
 // ---------------------------------------------------------------------------
 // external product (RLWE.cpp:623-704) on an accumulator held in LDS
 // ---------------------------------------------------------------------------
@@ -593,7 +606,13 @@ __global__ __launch_bounds__(NT) void k_external_product(EPArgs<W> a) {
 // RLWE_keyswitch (RLWE.cpp:464-510).  The keyswitch's own iNTT of the
 // substituted ciphertext is the identity on the time-domain words already
 // in LDS, so the digits are taken from them directly.
-template <typename W, int LOGN, int NT>
+//
+// EXPAND: one level of expand_RLWE (app.cpp:26-97) over [B][N] slots of
+// RLWEs: workgroup (b, j), j < 2^i, substitutes and key-switches slot j with
+// factor i, then writes slot j = c + subs and slot j + 2^i =
+// X^(-2^i) (c - subs) (RLWE_rotate_freq backward, RLWE.cpp:411-430; the
+// rotation polynomial (Q - 1) X^(N - 2^i) is X^(2N - 2^i)).
+template <typename W, int LOGN, int NT, bool EXPAND>
 __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
     constexpr int N = 1 << LOGN;
     constexpr int CPT = (N + NT - 1) / NT;
@@ -602,7 +621,9 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
     W *y = x + padded(2 * N);            // substituted (2 padded polys)
     W *dig = y + padded(2 * N);          // dch padded digit polys
     const int64_t c = blockIdx.x;
-    const uint64_t *src = a.in + c * 2 * N;
+    int64_t row = c;
+    if constexpr (EXPAND) row = ((c >> a.sub_factor) << LOGN) + (c & ((1 << a.sub_factor) - 1));
+    const uint64_t *src = a.in + row * 2 * N;
     for (int i = threadIdx.x; i < 2 * N; i += NT) x[pad(i)] = (W)src[i];
     __syncthreads();
     ntt_inverse<LOGN, NT>(x, 2, a.A, a.tw);
@@ -653,16 +674,113 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
         }
         __syncthreads();
     }
-    uint64_t *dst = a.out + c * 2 * N;
     const uint64_t Q = a.A.Q;
+    if constexpr (!EXPAND) {
+        uint64_t *dst = a.out + c * 2 * N;
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) {
+            const int k = threadIdx.x + cc * NT;
+            if (k < N) {
+                const uint64_t ma = sa[cc].get(a.A), mb = sb[cc].get(a.A);
+                dst[k] = ma ? Q - ma : 0;
+                const uint64_t bv = x[pad(N + k)];
+                dst[N + k] = bv >= mb ? bv - mb : bv + Q - mb;
+            }
+        }
+    } else {
+        // slot j is re-read from memory (this workgroup alone writes it, and
+        // only after every thread has read its own words)
+        uint64_t *lo = a.out + row * 2 * N, *hi = lo + ((int64_t)2 * N << a.sub_factor);
+        const int e = 2 * N - (1 << a.sub_factor);
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) {
+            const int k = threadIdx.x + cc * NT;
+            if (k < N) {
+                const uint64_t ma = sa[cc].get(a.A), mb = sb[cc].get(a.A);
+                const uint64_t bv = x[pad(N + k)];
+                const uint64_t s2[2] = {ma ? Q - ma : 0, bv >= mb ? bv - mb : bv + Q - mb};
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const uint64_t v = src[h * N + k], sv = s2[h];
+                    const uint64_t sum = v + sv;
+                    lo[h * N + k] = sum >= Q ? sum - Q : sum;
+                    const uint64_t df = v >= sv ? v - sv : v + Q - sv;
+                    hi[h * N + k] = mul_monomial<W, LOGN>(a.A, a.tw, (W)df, k, e);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CMUX chains: CMUX (RLWE.cpp:788-793), blind_rotate (RLWE.cpp:822-838) and
+// blind_rotate_full (RLWE.cpp:795-820), one ciphertext per workgroup.  The
+// running accumulator of the thread's coefficients stays in registers; the
+// difference X(acc) - acc goes through the external product in LDS.
+// ---------------------------------------------------------------------------
+
+template <typename W, int LOGN, int NT, bool INPLACE>
+__global__ __launch_bounds__(NT) void k_cmux_chain(CmuxArgs<W> a) {
+    constexpr int N = 1 << LOGN;
+    constexpr int CPT = (N + NT - 1) / NT;
+    constexpr bool kFull = N % NT == 0;
+    extern __shared__ __align__(16) unsigned char smem[];
+    W *acc = reinterpret_cast<W *>(smem);
+    W *dig = acc + padded(2 * N);
+    const int64_t c = blockIdx.x;
+    const uint64_t *src = a.c0 + c * 2 * N;
+    const uint64_t Q = a.A.Q;
+    W sav[2][CPT];
 #pragma unroll
     for (int cc = 0; cc < CPT; cc++) {
         const int k = threadIdx.x + cc * NT;
-        if (k < N) {
-            const uint64_t ma = sa[cc].get(a.A), mb = sb[cc].get(a.A);
-            dst[k] = ma ? Q - ma : 0;
-            const uint64_t bv = x[pad(N + k)];
-            dst[N + k] = bv >= mb ? bv - mb : bv + Q - mb;
+        if (kFull || k < N) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const W v = (W)src[h * N + k];
+                sav[h][cc] = a.pre_rot ? mul_monomial<W, LOGN>(a.A, a.tw, v, k, a.pre_rot) : v;
+            }
+        }
+    }
+    for (int st = 0; st < a.steps; st++) {
+        const int e = a.c1 ? 0 : a.rot[st];
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) {
+            const int k = threadIdx.x + cc * NT;
+            if (kFull || k < N) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const W v = sav[h][cc];
+                    const W o = a.c1 ? (W)a.c1[c * 2 * N + h * N + k] : mul_monomial<W, LOGN>(a.A, a.tw, v, k, e);
+                    acc[pad(h * N + k)] = o >= v ? o - v : o + (W)Q - v;
+                }
+            }
+        }
+        __syncthreads();
+        const W *key = a.key + c * a.key_ct_stride + st * a.key_stride;
+        if constexpr (INPLACE) ext_prod_inplace<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
+        else ext_prod<W, LOGN, NT>(acc, dig, key, a.A, a.tw, a.d, a.bits, a.pb);
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) {
+            const int k = threadIdx.x + cc * NT;
+            if (kFull || k < N) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const W s = acc[pad(h * N + k)] + sav[h][cc];
+                    sav[h][cc] = s >= (W)Q ? s - (W)Q : s;
+                }
+            }
+        }
+        // the next step rewrites acc: every thread has read its words
+        __syncthreads();
+    }
+    uint64_t *dst = a.out + c * 2 * N;
+#pragma unroll
+    for (int cc = 0; cc < CPT; cc++) {
+        const int k = threadIdx.x + cc * NT;
+        if (kFull || k < N) {
+            dst[k] = sav[0][cc];
+            dst[N + k] = sav[1][cc];
         }
     }
 }
@@ -1043,10 +1161,44 @@ template <typename W>
 hipError_t launch_subs_keyswitch(const SubsArgs<W> &a, int logN, int64_t count, hipStream_t st) {
     const size_t lds = (size_t)(2 * padded(2 << logN) + padded(a.dch << logN)) * sizeof(W);
     FHEACC_LOGN_SWITCH(logN, {
-        auto k = k_subs_keyswitch<W, LOGN, 256>;
+        auto k = k_subs_keyswitch<W, LOGN, 256, false>;
         hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(256), lds, st, a);
+    })
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_expand_level(const SubsArgs<W> &a, int logN, int64_t B, hipStream_t st) {
+    const size_t lds = (size_t)(2 * padded(2 << logN) + padded(a.dch << logN)) * sizeof(W);
+    const int64_t count = B << a.sub_factor;
+    if (count > 0x7fffffff) return hipErrorInvalidValue;
+    FHEACC_LOGN_SWITCH(logN, {
+        auto k = k_subs_keyswitch<W, LOGN, 256, true>;
+        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(256), lds, st, a);
+    })
+    return hipGetLastError();
+}
+
+template <typename W, int LOGN, int NT>
+static hipError_t launch_cmux_nt(const CmuxArgs<W> &a, int64_t count, size_t lds, hipStream_t st) {
+    auto k = a.pb >= 2 * a.d ? k_cmux_chain<W, LOGN, NT, true> : k_cmux_chain<W, LOGN, NT, false>;
+    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(NT), lds, st, a);
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_cmux_chain(const CmuxArgs<W> &a, int logN, int nt, int64_t count, size_t lds, hipStream_t st) {
+    if (count > 0x7fffffff) return hipErrorInvalidValue;
+    FHEACC_LOGN_SWITCH(logN, {
+        if (nt == 128 && LOGN >= 7) return launch_cmux_nt<W, LOGN, 128>(a, count, lds, st);
+        if (nt == 512 && LOGN >= 9) return launch_cmux_nt<W, LOGN, 512>(a, count, lds, st);
+        return launch_cmux_nt<W, LOGN, 256>(a, count, lds, st);
     })
     return hipGetLastError();
 }
@@ -1129,6 +1281,8 @@ hipError_t launch_digest(const uint64_t *w, int64_t count, uint64_t *partials, i
     template hipError_t launch_ntt<W>(const NTTArgs<W> &, int, hipStream_t);                                      \
     template hipError_t launch_external_product<W>(const EPArgs<W> &, int, int64_t, hipStream_t);                 \
     template hipError_t launch_subs_keyswitch<W>(const SubsArgs<W> &, int, int64_t, hipStream_t);                 \
+    template hipError_t launch_expand_level<W>(const SubsArgs<W> &, int, int64_t, hipStream_t);                   \
+    template hipError_t launch_cmux_chain<W>(const CmuxArgs<W> &, int, int, int64_t, size_t, hipStream_t);        \
     template hipError_t launch_rgsw_to_device<W>(const uint64_t *, int64_t, W *, int64_t, int, int, ShoupGen,     \
                                                  const int64_t *, int64_t, hipStream_t);                          \
     template hipError_t launch_rgsw_to_ref<W>(const W *, int64_t, uint64_t *, int64_t, int, int, const int64_t *, \

@@ -89,6 +89,24 @@ struct SubsArgs {
     int d, bits, dch;
 };
 
+// CMUX chains (RLWE.cpp:788-838): per ciphertext acc = c0; for each step
+// acc = (X(acc) - acc) (x) C[step] + acc, where X(acc) is c1 (one plain
+// CMUX) or acc times X^rot[step] (blind rotation).
+template <typename W>
+struct CmuxArgs {
+    Arith<W> A;
+    Twiddles<W> tw;
+    const uint64_t *c0, *c1;  // [B][2][N]; c1 null: rotation steps
+    uint64_t *out;            // [B][2][N]
+    const W *key;             // device-layout RGSW of (ciphertext c, step s) at
+    int64_t key_stride;       //   key + c key_ct_stride + s key_stride
+    int64_t key_ct_stride;
+    const int32_t *rot;       // [steps] exponents in [0, 2N) (device)
+    int steps;
+    int pre_rot;              // public rotation applied first (exponent in [0, 2N)), 0 = none
+    int d, bits, pb;
+};
+
 struct ShoupGen {
     uint64_t Q, a, b, bp;
     __host__ __device__ ShoupGen() : Q(0), a(0), b(0), bp(0) {}
@@ -148,6 +166,11 @@ template <typename W>
 hipError_t launch_external_product(const EPArgs<W> &a, int logN, int64_t count, hipStream_t st);
 template <typename W>
 hipError_t launch_subs_keyswitch(const SubsArgs<W> &a, int logN, int64_t count, hipStream_t st);
+// expand_RLWE level sub_factor over the [B][N][2][N] slots in a.out (== a.in)
+template <typename W>
+hipError_t launch_expand_level(const SubsArgs<W> &a, int logN, int64_t B, hipStream_t st);
+template <typename W>
+hipError_t launch_cmux_chain(const CmuxArgs<W> &a, int logN, int nt, int64_t count, size_t lds, hipStream_t st);
 template <typename W>
 hipError_t launch_rgsw_to_device(const uint64_t *src, int64_t src_stride, W *dst, int64_t dst_stride, int d, int N,
                                  ShoupGen sg, const int64_t *map, int64_t count, hipStream_t st);

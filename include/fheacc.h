@@ -183,6 +183,28 @@ int fheacc_external_product_dev(fheacc_ctx *ctx, const uint64_t *acc, const uint
 int fheacc_subs_keyswitch_dev(fheacc_ctx *ctx, fheacc_keys *keys, const uint64_t *in, int sub_factor,
                               uint64_t *out, int64_t count);
 
+/* ---- RLWE expansion and leveled operations (device pointers) -------------
+ * application::expand_RLWE (app.cpp:26-97; the accelerator's batched
+ * expansion, myfpga.cpp:1662-2452): in [count][2][N] (NTT domain), out
+ * [count][N][2][N], slot k of input c at out + (c N + k) 2N.  Needs the subs
+ * key.  in and out must not overlap. */
+int fheacc_expand_rlwe_dev(fheacc_ctx *ctx, fheacc_keys *keys, const uint64_t *in, uint64_t *out, int64_t count);
+/* RLWE::CMUX (RLWE.cpp:788-793): out[i] = (c1[i] - c0[i]) (x) rgsw[i] + c0[i];
+ * rgsw in the reference RGSW layout, rgsw_stride words apart (0: one shared
+ * RGSW). */
+int fheacc_cmux_dev(fheacc_ctx *ctx, const uint64_t *c0, const uint64_t *c1, const uint64_t *rgsw,
+                    int64_t rgsw_stride, uint64_t *out, int64_t count);
+/* RLWE::blind_rotate (RLWE.cpp:822-838): acc = CMUX(acc, acc X^(+-2^i), C[i])
+ * for i < steps (<= log2 N; the reference fixes 11 at N = 2048); rgsw:
+ * [steps] reference-layout RGSWs shared by all count inputs. */
+int fheacc_blind_rotate_rlwe_dev(fheacc_ctx *ctx, const uint64_t *in, int forward, const uint64_t *rgsw, int steps,
+                                 uint64_t *out, int64_t count);
+/* RLWE::blind_rotate_full (RLWE.cpp:795-820): rotation by X^b, then
+ * acc = CMUX(acc, acc X^a[i], C[i]); a is a host array, rotation factors are
+ * reduced mod N as poly_rotate_time does (RLWE.cpp:49-68). */
+int fheacc_blind_rotate_full_dev(fheacc_ctx *ctx, const uint64_t *in, int b, const int32_t *a, const uint64_t *rgsw,
+                                 int steps, uint64_t *out, int64_t count);
+
 /* ---- serving loop -------------------------------------------------------
  * No counterpart in the reference: its apps evaluate one gate
  * (RLWE::eval_bootstrap, RLWE.cpp:726-786) or one fixed batch

@@ -71,6 +71,12 @@ def lib():
             "orc_acc_init": (None, [P, C.c_uint64, C.c_uint64, C.c_uint64, U]),
             "orc_rlwe_substitute": (None, [P, U, C.c_int, U]),
             "orc_rlwe_keyswitch": (None, [P, U, U, U]),
+            "orc_rotate_poly": (None, [P, C.c_int, C.c_int, U]),
+            "orc_rlwe_rotate_freq": (None, [P, U, C.c_int, C.c_int, U]),
+            "orc_expand_rlwe": (None, [P, U, U, U]),
+            "orc_cmux": (None, [P, U, U, U, U]),
+            "orc_blind_rotate_rlwe": (None, [P, U, C.c_int, U, C.c_int, U]),
+            "orc_blind_rotate_full": (None, [P, U, C.c_int, I32, C.c_int, U, U]),
             "orc_lwe_evaluate": (C.c_int, [P, U, U, C.c_int, U]),
             "orc_lwe_eval_not": (None, [P, U, U]),
             "orc_lwe_keyswitch": (None, [P, U, U, U]),
@@ -197,6 +203,58 @@ class Oracle:
         out = np.zeros((2, self.N), dtype=np.uint64)
         _, po = _u64(out)
         lib().orc_rlwe_keyswitch(self.ref, pr, pk, po)
+        return out
+
+    def rotate_poly(self, log_factor, forward):
+        out = np.zeros(self.N, dtype=np.uint64)
+        _, po = _u64(out)
+        lib().orc_rotate_poly(self.ref, log_factor, 1 if forward else 0, po)
+        return out
+
+    def rotate_freq(self, rlwe, log_factor, forward):
+        rlwe, pr = _u64(rlwe)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        _, po = _u64(out)
+        lib().orc_rlwe_rotate_freq(self.ref, pr, log_factor, 1 if forward else 0, po)
+        return out
+
+    def expand_rlwe(self, rlwe, subs):
+        """expand_RLWE (app.cpp:26-97): [N][2][N] from one RLWE (NTT domain)."""
+        rlwe, pr = _u64(rlwe)
+        subs, pk = _u64(subs)
+        out = np.zeros((self.N, 2, self.N), dtype=np.uint64)
+        _, po = _u64(out)
+        lib().orc_expand_rlwe(self.ref, pr, pk, po)
+        return out
+
+    def cmux(self, c0, c1, rgsw):
+        c0, p0 = _u64(c0)
+        c1, p1 = _u64(c1)
+        rgsw, pg = _u64(rgsw)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        _, po = _u64(out)
+        lib().orc_cmux(self.ref, p0, p1, pg, po)
+        return out
+
+    def blind_rotate_rlwe(self, rlwe, forward, rgsws):
+        """blind_rotate (RLWE.cpp:822-838) over len(rgsws) RGSWs."""
+        rlwe, pr = _u64(rlwe)
+        rgsws = np.ascontiguousarray(rgsws, dtype=np.uint64).reshape(-1, self.rgsw_words)
+        _, pg = _u64(rgsws)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        _, po = _u64(out)
+        lib().orc_blind_rotate_rlwe(self.ref, pr, 1 if forward else 0, pg, rgsws.shape[0], po)
+        return out
+
+    def blind_rotate_full(self, rlwe, b, a, rgsws):
+        """blind_rotate_full (RLWE.cpp:795-820)."""
+        rlwe, pr = _u64(rlwe)
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1))
+        rgsws = np.ascontiguousarray(rgsws, dtype=np.uint64).reshape(-1)
+        _, pg = _u64(rgsws)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        _, po = _u64(out)
+        lib().orc_blind_rotate_full(self.ref, pr, int(b), a.ctypes.data_as(C.POINTER(C.c_int32)), a.size, pg, po)
         return out
 
     def lwe_keyswitch(self, acc, ksk):

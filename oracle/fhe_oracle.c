@@ -20,6 +20,10 @@
  *           oracle/_ref for every gate and b.
  *   orc_poly_substitute / orc_rlwe_substitute / orc_rlwe_keyswitch ->
  *           RLWESUBS_output.txt; expansion invariant (test.cpp:1889-1912).
+ *   orc_rotate_poly / orc_rlwe_rotate_freq / orc_expand_rlwe / orc_cmux /
+ *   orc_blind_rotate_rlwe -> oracle/_ref (the reference's expand_RLWE, CMUX,
+ *           blind_rotate compiled here); NTT(X^k) closed form; an expanded
+ *           encryption of m decrypts to N m_k in slot k; CMUX selects.
  *   orc_lwe_evaluate / orc_lwe_keyswitch / orc_scale_round / orc_lwe_modswitch /
  *   orc_blind_rotate / orc_eval_bootstrap -> oracle/_ref (reference
  *           LWE.cpp/RLWE.cpp compiled here, explicit keys), scale_round closed
@@ -366,6 +370,142 @@ void orc_rlwe_keyswitch(const orc_params *p, const uint64_t *in, const uint64_t 
         }
     }
     free(dec);
+}
+
+/* ------------------------------------------------------------------ */
+/* RLWE expansion and leveled operations                               */
+/* ------------------------------------------------------------------ */
+
+/* params.cpp:177-210 (rotate_vector_gen): the monomial X^(2^i), or
+ * (Q - 1) X^(N - 2^i), transformed to the NTT domain */
+void orc_rotate_poly(const orc_params *p, int log_factor, int forward, uint64_t *out) {
+    int N = p->N;
+    memset(out, 0, sizeof(uint64_t) * N);
+    int pw = 1 << log_factor;
+    if (forward) out[pw] = 1;
+    else out[N - pw] = p->Q - 1;
+    orc_ntt(out, N, p->Q, p->rou);
+}
+
+/* RLWE.cpp:377-400 (poly_add / poly_subtraction, RLWE.cpp:11-21) */
+void orc_rlwe_add(const orc_params *p, const uint64_t *c1, const uint64_t *c2, uint64_t *out) {
+    for (int i = 0; i < 2 * p->N; i++) out[i] = modu((u128)c1[i] + c2[i], p->Q);
+}
+void orc_rlwe_sub(const orc_params *p, const uint64_t *c1, const uint64_t *c2, uint64_t *out) {
+    for (int i = 0; i < 2 * p->N; i++) out[i] = mods((s128)c1[i] - (s128)c2[i], p->Q);
+}
+
+/* RLWE.cpp:411-430 */
+void orc_rlwe_rotate_freq(const orc_params *p, const uint64_t *in, int log_factor, int forward, uint64_t *out) {
+    int N = p->N;
+    uint64_t *r = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    orc_rotate_poly(p, log_factor, forward, r);
+    for (int i = 0; i < N; i++) {
+        out[i] = modu((u128)in[i] * r[i], p->Q);
+        out[N + i] = modu((u128)in[N + i] * r[i], p->Q);
+    }
+    free(r);
+}
+
+/* app.cpp:26-97 (expand_RLWE): level i doubles the filled slots; slot j
+ * becomes c + subs and slot j + 2^i becomes rotate(c - subs, i, backward),
+ * with subs = RLWE_keyswitch(RLWE_substitute(c, i), subs_key[i]). */
+void orc_expand_rlwe(const orc_params *p, const uint64_t *in, const uint64_t *subs, uint64_t *out) {
+    int N = p->N, logN = 0;
+    while ((1 << logN) < N) logN++;
+    size_t rw = (size_t)2 * N;
+    size_t kw = (size_t)p->digitKS_R * rw;
+    uint64_t *t = (uint64_t *)malloc(sizeof(uint64_t) * rw * 3);
+    uint64_t *sb = t + rw, *df = t + 2 * rw;
+    memcpy(out, in, sizeof(uint64_t) * rw);
+    int inner = 1;
+    for (int i = 0; i < logN; i++) {
+        for (int j = 0; j < inner; j++) {
+            uint64_t *c = out + (size_t)j * rw;
+            orc_rlwe_substitute(p, c, i, t);
+            orc_rlwe_keyswitch(p, t, subs + (size_t)i * kw, sb);
+            orc_rlwe_sub(p, c, sb, df);
+            orc_rlwe_add(p, c, sb, c);
+            orc_rlwe_rotate_freq(p, df, i, 0, out + (size_t)(j + inner) * rw);
+        }
+        inner *= 2;
+    }
+    free(t);
+}
+
+/* RLWE.cpp:788-793 */
+void orc_cmux(const orc_params *p, const uint64_t *c0, const uint64_t *c1, const uint64_t *rgsw, uint64_t *out) {
+    size_t rw = (size_t)2 * p->N;
+    uint64_t *d = (uint64_t *)malloc(sizeof(uint64_t) * rw * 2);
+    orc_rlwe_sub(p, c1, c0, d);
+    orc_rlwe_mult_rgsw(p, d, rgsw, d + rw);
+    orc_rlwe_add(p, d + rw, c0, out);
+    free(d);
+}
+
+/* RLWE.cpp:49-68: right rotation by rf with the wrapped words negated
+ * (rf > 0), left rotation by -rf with the wrapped words negated (rf < 0) */
+void orc_poly_rotate_time(uint64_t *a, int rotate_factor, int N, uint64_t Q) {
+    int rf = rotate_factor % N;
+    if (rf == 0) return;
+    uint64_t *t = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    memcpy(t, a, sizeof(uint64_t) * N);
+    if (rf > 0) {
+        for (int i = 0; i < N; i++) a[i] = i >= rf ? t[i - rf] : mods(-(s128)t[N - rf + i], Q);
+    } else {
+        int m = -rf;
+        for (int i = 0; i < N; i++) a[i] = i < N - m ? t[i + m] : mods(-(s128)t[i + m - N], Q);
+    }
+    free(t);
+}
+
+/* RLWE.cpp:795-820: the accumulator lives in the time domain between
+ * CMUXes, as in the reference */
+void orc_blind_rotate_full(const orc_params *p, const uint64_t *in, int b, const int32_t *a, int steps,
+                           const uint64_t *rgsw, uint64_t *out) {
+    int N = p->N;
+    size_t rw = (size_t)2 * N;
+    size_t gw = (size_t)p->digitG * 4 * N;
+    uint64_t *acc = (uint64_t *)malloc(sizeof(uint64_t) * rw * 2);
+    uint64_t *tmp = acc + rw;
+    memcpy(acc, in, sizeof(uint64_t) * rw);
+    orc_intt(acc, N, p->Q, p->irou, p->iN);
+    orc_intt(acc + N, N, p->Q, p->irou, p->iN);
+    if (b != 0) {
+        orc_poly_rotate_time(acc, b, N, p->Q);
+        orc_poly_rotate_time(acc + N, b, N, p->Q);
+    }
+    for (int i = 0; i < steps; i++) {
+        memcpy(tmp, acc, sizeof(uint64_t) * rw);
+        orc_poly_rotate_time(tmp, a[i], N, p->Q);
+        orc_poly_rotate_time(tmp + N, a[i], N, p->Q);
+        for (int h = 0; h < 2; h++) {
+            orc_ntt(tmp + h * N, N, p->Q, p->rou);
+            orc_ntt(acc + h * N, N, p->Q, p->rou);
+        }
+        orc_cmux(p, acc, tmp, rgsw + (size_t)i * gw, acc);
+        orc_intt(acc, N, p->Q, p->irou, p->iN);
+        orc_intt(acc + N, N, p->Q, p->irou, p->iN);
+    }
+    orc_ntt(acc, N, p->Q, p->rou);
+    orc_ntt(acc + N, N, p->Q, p->rou);
+    memcpy(out, acc, sizeof(uint64_t) * rw);
+    free(acc);
+}
+
+/* RLWE.cpp:822-838 (blind_rotate; the reference fixes the count at
+ * num_rotate_poly = 11 = log2 2048, params.h:114) */
+void orc_blind_rotate_rlwe(const orc_params *p, const uint64_t *in, int forward, const uint64_t *rgsw,
+                           int count, uint64_t *out) {
+    size_t rw = (size_t)2 * p->N;
+    size_t gw = (size_t)p->digitG * 4 * p->N;
+    uint64_t *tmp = (uint64_t *)malloc(sizeof(uint64_t) * rw);
+    memcpy(out, in, sizeof(uint64_t) * rw);
+    for (int i = 0; i < count; i++) {
+        orc_rlwe_rotate_freq(p, out, i, forward, tmp);
+        orc_cmux(p, out, tmp, rgsw + (size_t)i * gw, out);
+    }
+    free(tmp);
 }
 
 /* ------------------------------------------------------------------ */

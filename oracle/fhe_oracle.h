@@ -94,6 +94,26 @@ void orc_acc_init(const orc_params *p, uint64_t bound1, uint64_t bound2, uint64_
 void orc_rlwe_substitute(const orc_params *p, const uint64_t *in, int sub_factor, uint64_t *out);
 void orc_rlwe_keyswitch(const orc_params *p, const uint64_t *in, const uint64_t *kskey, uint64_t *out);
 
+/* ---------- RLWE expansion and leveled ops (params.cpp:177-210, RLWE.cpp:377-400,411-430,788-838,
+ *            app.cpp:26-97) ---------- */
+/* NTT of X^(2^i) (forward) or of (Q-1) X^(N - 2^i) (backward), i < log2 N */
+void orc_rotate_poly(const orc_params *p, int log_factor, int forward, uint64_t *out);
+void orc_rlwe_add(const orc_params *p, const uint64_t *c1, const uint64_t *c2, uint64_t *out);
+void orc_rlwe_sub(const orc_params *p, const uint64_t *c1, const uint64_t *c2, uint64_t *out);
+void orc_rlwe_rotate_freq(const orc_params *p, const uint64_t *in, int log_factor, int forward, uint64_t *out);
+/* expand_RLWE: out [N][2][N]; subs key [log2 N][digitKS_R][2][N] */
+void orc_expand_rlwe(const orc_params *p, const uint64_t *in, const uint64_t *subs, uint64_t *out);
+/* CMUX(c0, c1, SEL) = (c1 - c0) (x) SEL + c0 */
+void orc_cmux(const orc_params *p, const uint64_t *c0, const uint64_t *c1, const uint64_t *rgsw, uint64_t *out);
+/* poly_rotate_time (RLWE.cpp:49-68): a <- a X^(rf), rf = rotate_factor % N (C '%' sign) */
+void orc_poly_rotate_time(uint64_t *a, int rotate_factor, int N, uint64_t Q);
+/* blind_rotate_full (RLWE.cpp:795-820): X^b, then CMUX(acc, acc X^a[i], C[i]) in the time domain */
+void orc_blind_rotate_full(const orc_params *p, const uint64_t *in, int b, const int32_t *a, int steps,
+                           const uint64_t *rgsw, uint64_t *out);
+/* blind_rotate: count RGSWs (count <= log2 N), acc = CMUX(acc, rotate_freq(acc, i), C[i]) */
+void orc_blind_rotate_rlwe(const orc_params *p, const uint64_t *in, int forward, const uint64_t *rgsw,
+                           int count, uint64_t *out);
+
 /* ---------- LWE (LWE.cpp) ---------- */
 int  orc_lwe_evaluate(const orc_params *p, const uint64_t *c1, const uint64_t *c2, int gate, uint64_t *out);
 void orc_lwe_eval_not(const orc_params *p, const uint64_t *c, uint64_t *out);

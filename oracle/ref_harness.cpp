@@ -100,6 +100,18 @@ int ref_set_params(int n_, uint64_t q_, int N_, uint64_t Q_, uint32_t BGbits_, u
     gate_constant[3] = 3 * (q >> 3);
     gate_constant[4] = 5 * (q >> 3);
     gate_constant[5] = 1 * (q >> 3);
+    // rotate_poly_forward/backward (params.cpp:130-134), built as
+    // rotate_vector_gen (params.cpp:177-210) builds them but for the
+    // log2 N factors that exist at this N (the reference fixes
+    // num_rotate_poly = 11, which only fits N = 2048)
+    rotate_poly_forward.assign(digitN, std::vector<ciphertext_t>(N, 0));
+    rotate_poly_backward.assign(digitN, std::vector<ciphertext_t>(N, 0));
+    for (int i = 0; i < digitN; i++) {
+        rotate_poly_forward[i][1 << i] = 1;
+        rotate_poly_backward[i][N - (1 << i)] = Q - 1;
+        NTT(rotate_poly_forward[i].data(), N, Q, ROU_table);
+        NTT(rotate_poly_backward[i].data(), N, Q, ROU_table);
+    }
     g_bsk.reset();
     g_ksk.reset();
     return 0;
@@ -215,6 +227,58 @@ void ref_rlwe_keyswitch(const uint64_t *in, const uint64_t *ks, uint64_t *out) {
     k.RLWE_kskey.resize(digitKS_R);
     for (uint64_t l = 0; l < digitKS_R; l++) fill_rlwe(k.RLWE_kskey[l], ks + l * 2 * N);
     dump_rlwe(RLWE::RLWE_keyswitch(a, k), out);
+}
+
+// application::expand_RLWE (app.cpp:26-97); subs: [digitN][digitKS_R][2][N]
+void ref_expand_rlwe(const uint64_t *in, const uint64_t *subs, uint64_t *out) {
+    RLWE::RLWE_ciphertext c;
+    fill_rlwe(c, in);
+    application::substitute_key sk;
+    sk.subs_key.resize(digitN);
+    for (int i = 0; i < digitN; i++) {
+        sk.subs_key[i].RLWE_kskey.resize(digitKS_R);
+        for (uint64_t l = 0; l < digitKS_R; l++)
+            fill_rlwe(sk.subs_key[i].RLWE_kskey[l], subs + ((size_t)i * digitKS_R + l) * 2 * N);
+    }
+    auto r = application::expand_RLWE(c, sk);
+    for (int k = 0; k < N; k++) dump_rlwe((*r)[k], out + (size_t)k * 2 * N);
+}
+
+// RLWE::RLWE_rotate_freq (RLWE.cpp:411-430)
+void ref_rotate_freq(const uint64_t *in, int log_factor, int forward, uint64_t *out) {
+    RLWE::RLWE_ciphertext c;
+    fill_rlwe(c, in);
+    dump_rlwe(RLWE::RLWE_rotate_freq(c, log_factor, forward != 0), out);
+}
+
+// RLWE::CMUX (RLWE.cpp:788-793)
+void ref_cmux(const uint64_t *c0, const uint64_t *c1, const uint64_t *rgsw, uint64_t *out) {
+    RLWE::RLWE_ciphertext a, b;
+    fill_rlwe(a, c0);
+    fill_rlwe(b, c1);
+    dump_rlwe(RLWE::CMUX(a, b, make_rgsw(rgsw)), out);
+}
+
+// RLWE::blind_rotate (RLWE.cpp:822-838): num_rotate_poly (11) RGSWs, N = 2048
+int ref_blind_rotate_rlwe(const uint64_t *in, int forward, const uint64_t *rgsw, uint64_t *out) {
+    if (digitN != num_rotate_poly) return -1;
+    RLWE::RLWE_ciphertext c;
+    fill_rlwe(c, in);
+    std::vector<RLWE::RGSW_ciphertext> C;
+    for (int i = 0; i < num_rotate_poly; i++) C.push_back(make_rgsw(rgsw + (size_t)i * digitG * 4 * N));
+    dump_rlwe(RLWE::blind_rotate(c, forward != 0, C), out);
+    return 0;
+}
+
+// RLWE::blind_rotate_full (RLWE.cpp:795-820)
+void ref_blind_rotate_full(const uint64_t *in, int b, const int32_t *a, int steps, const uint64_t *rgsw,
+                           uint64_t *out) {
+    RLWE::RLWE_ciphertext c;
+    fill_rlwe(c, in);
+    std::vector<int> av(a, a + steps);
+    std::vector<RLWE::RGSW_ciphertext> C;
+    for (int i = 0; i < steps; i++) C.push_back(make_rgsw(rgsw + (size_t)i * digitG * 4 * N));
+    dump_rlwe(RLWE::blind_rotate_full(c, b, av, C), out);
 }
 
 uint64_t ref_root_of_unity(int order, uint64_t mod) { return root_of_unity(order, mod); }

@@ -40,6 +40,11 @@ def lib():
             "ref_eval_bootstrap": (C.c_double, [U, U, I32, C.c_int64, U]),
             "ref_substitute": (None, [U, C.c_int, U]),
             "ref_rlwe_keyswitch": (None, [U, U, U]),
+            "ref_expand_rlwe": (None, [U, U, U]),
+            "ref_rotate_freq": (None, [U, C.c_int, C.c_int, U]),
+            "ref_cmux": (None, [U, U, U, U]),
+            "ref_blind_rotate_rlwe": (C.c_int, [U, C.c_int, U, U]),
+            "ref_blind_rotate_full": (None, [U, C.c_int, I32, C.c_int, U, U]),
             "ref_root_of_unity": (C.c_uint64, [C.c_int, C.c_uint64]),
             "ref_first_prime": (C.c_uint64, [C.c_int, C.c_int]),
             "ref_previous_prime": (C.c_uint64, [C.c_uint64, C.c_int]),
@@ -133,4 +138,39 @@ class Reference:
         ks = np.ascontiguousarray(ks, dtype=np.uint64)
         out = np.zeros((2, self.N), dtype=np.uint64)
         lib().ref_rlwe_keyswitch(_p(rlwe), _p(ks), _p(out))
+        return out
+
+    def expand_rlwe(self, rlwe, subs):
+        rlwe = np.ascontiguousarray(rlwe, dtype=np.uint64)
+        subs = np.ascontiguousarray(subs, dtype=np.uint64)
+        out = np.zeros((self.N, 2, self.N), dtype=np.uint64)
+        lib().ref_expand_rlwe(_p(rlwe), _p(subs), _p(out))
+        return out
+
+    def rotate_freq(self, rlwe, log_factor, forward):
+        rlwe = np.ascontiguousarray(rlwe, dtype=np.uint64)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        lib().ref_rotate_freq(_p(rlwe), log_factor, 1 if forward else 0, _p(out))
+        return out
+
+    def cmux(self, c0, c1, rgsw):
+        c0, c1, rgsw = (np.ascontiguousarray(x, dtype=np.uint64) for x in (c0, c1, rgsw))
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        lib().ref_cmux(_p(c0), _p(c1), _p(rgsw), _p(out))
+        return out
+
+    def blind_rotate_full(self, rlwe, b, a, rgsws):
+        rlwe = np.ascontiguousarray(rlwe, dtype=np.uint64)
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1))
+        rgsws = np.ascontiguousarray(rgsws, dtype=np.uint64).reshape(-1)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        lib().ref_blind_rotate_full(_p(rlwe), int(b), a.ctypes.data_as(C.POINTER(C.c_int32)), a.size, _p(rgsws),
+                                    _p(out))
+        return out
+
+    def blind_rotate_rlwe(self, rlwe, forward, rgsws):
+        rlwe, rgsws = (np.ascontiguousarray(x, dtype=np.uint64) for x in (rlwe, rgsws))
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        if lib().ref_blind_rotate_rlwe(_p(rlwe), 1 if forward else 0, _p(rgsws), _p(out)) != 0:
+            raise ValueError("the reference's blind_rotate needs N = 2048 (num_rotate_poly = 11)")
         return out

@@ -218,3 +218,75 @@ def test_fresh_encryptions_decrypt():
     et = o.intt(e.astype(np.uint64)).astype(object)
     et = [x if x < o.Q // 2 else x - o.Q for x in et]
     assert max(abs(x) for x in et) < 18
+
+
+def _rlwe_decrypt_phase(o, c, s_ntt):
+    """Time-domain phase b - a s of an RLWE ciphertext (NTT domain in)."""
+    Q = o.Q
+    ph = (c[1].astype(object) - c[0].astype(object) * s_ntt.astype(object)) % Q
+    return [int(v) for v in o.intt(np.array(ph, dtype=np.uint64))]
+
+
+def test_rotate_freq_inverse_pair():
+    # X^(2^i) X^(-2^i) = 1: forward then backward rotation is the identity
+    o = Oracle(n=4, q=64, N=32, nbit=27)
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, o.Q, (2, 32), dtype=np.uint64)
+    for i in range(5):
+        assert np.array_equal(o.rotate_freq(o.rotate_freq(x, i, True), i, False), x)
+    # rotating by X^1 shifts the time-domain coefficients negacyclically
+    t = o.intt(o.rotate_freq(x, 0, True)[0])
+    a = o.intt(x[0])
+    assert int(t[0]) == (o.Q - int(a[31])) % o.Q and np.array_equal(t[1:], a[:31])
+
+
+@pytest.mark.parametrize("N", [32, 64])
+def test_expand_rlwe_decrypts_to_scaled_coefficients(N):
+    # an encryption of m(X) expands into N ciphertexts; slot k decrypts to
+    # N m_k in its constant term (and ~0 elsewhere) -- the PSI query
+    # expansion app.cpp:26-97 exists for
+    o = Oracle(n=4, q=2 * N, N=N, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Bks_R_bits=9)
+    z = o.rlwe_secret(77)
+    s_ntt = o.ntt(z)
+    subs = o.subs_key_gen(z, 78)
+    rng = np.random.default_rng(N)
+    m = rng.integers(0, 2, N)
+    delta = o.Q // (4 * N)
+    c = o.rlwe_encrypt_zero(s_ntt, 79, 1)
+    mt = np.array([(int(v) * delta) % o.Q for v in m], dtype=np.uint64)
+    c[1] = ((c[1].astype(object) + o.ntt(mt).astype(object)) % o.Q).astype(np.uint64)
+    out = o.expand_rlwe(c, subs)
+    scale = N * delta
+    for k in range(N):
+        ph = _rlwe_decrypt_phase(o, out[k], s_ntt)
+        centered = [v if v < o.Q // 2 else v - o.Q for v in ph]
+        assert round(centered[0] / scale) == m[k], k
+        assert max(abs(v) for v in centered[1:]) < scale // 4, k
+
+
+def test_cmux_selects():
+    # CMUX(c0, c1, RGSW(b)) decrypts to c_b's message
+    N = 64
+    o = Oracle(n=4, q=128, N=N, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18)
+    z = o.rlwe_secret(81)
+    s_ntt = o.ntt(z)
+    rng = np.random.default_rng(9)
+    delta = o.Q // 8
+    msgs = [rng.integers(0, 2, N) for _ in range(2)]
+    cts = []
+    for t, mm in enumerate(msgs):
+        c = o.rlwe_encrypt_zero(s_ntt, 82, 10 + t)
+        mt = np.array([(int(v) * delta) % o.Q for v in mm], dtype=np.uint64)
+        c[1] = ((c[1].astype(object) + o.ntt(mt).astype(object)) % o.Q).astype(np.uint64)
+        cts.append(c)
+    import ctypes as C
+    P = C.POINTER(C.c_uint64)
+    # RGSW(1): the monomial X^0; RGSW(0): every row an encryption of zero
+    one = np.zeros(o.rgsw_words, dtype=np.uint64)
+    lib().orc_rgsw_encrypt_monomial(o.ref, s_ntt.ctypes.data_as(P), 0, 83, 20, one.ctypes.data_as(P))
+    zero = np.concatenate([o.rlwe_encrypt_zero(s_ntt, 84, 200 + r).reshape(-1) for r in range(2 * o.digitG)])
+    for bit, sel in ((0, zero), (1, one)):
+        out = o.cmux(cts[0], cts[1], sel)
+        ph = _rlwe_decrypt_phase(o, out, s_ntt)
+        got = [round((v if v < o.Q // 2 else v - o.Q) / delta) % 2 for v in ph]
+        assert got == list(msgs[bit]), bit

@@ -109,3 +109,47 @@ def test_subs_and_rlwe_keyswitch(cfg):
         assert np.array_equal(o.substitute(x, f), r.substitute(x, f))
     ks = rng.integers(0, o.Q, o.digitKS_R * 2 * o.N, dtype=np.uint64)
     assert np.array_equal(o.rlwe_keyswitch(x, ks), r.rlwe_keyswitch(x, ks))
+
+
+@pytest.mark.parametrize("N,Q,Bks_R_bits", [(64, 0x3FFFFFFFFED001, 9), (128, 0, 7)])
+def test_expand_rlwe(N, Q, Bks_R_bits):
+    # application::expand_RLWE (app.cpp:26-97) on random words and a random
+    # substitution key: every word of all N outputs
+    cfg = dict(n=4, q=2 * N, N=N, Q=Q, nbit=27, BGbits=9, Br=16, Bks=32)
+    o = Oracle(**cfg, Bks_R_bits=Bks_R_bits)
+    r = refbind.Reference(cfg["n"], cfg["q"], N, o.Q, cfg["BGbits"], cfg["Br"], cfg["Bks"], Bks_R_bits)
+    rng = np.random.default_rng(N)
+    x = rng.integers(0, o.Q, (2, N), dtype=np.uint64)
+    logN = N.bit_length() - 1
+    subs = rng.integers(0, o.Q, logN * o.digitKS_R * 2 * N, dtype=np.uint64)
+    assert np.array_equal(o.expand_rlwe(x, subs), r.expand_rlwe(x, subs))
+    for i in range(logN):
+        for fwd in (True, False):
+            assert np.array_equal(o.rotate_freq(x, i, fwd), r.rotate_freq(x, i, fwd))
+
+
+def test_cmux_and_leveled_blind_rotate():
+    # RLWE::CMUX and RLWE::blind_rotate (RLWE.cpp:788-838); the reference's
+    # blind_rotate takes num_rotate_poly = 11 RGSWs, so N = 2048
+    cfg = dict(n=4, q=1024, N=2048, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=32, Bks=32)
+    o, r = pair(cfg)
+    rng = np.random.default_rng(11)
+    c0, c1 = (rng.integers(0, o.Q, (2, o.N), dtype=np.uint64) for _ in range(2))
+    rgsw = rng.integers(0, o.Q, (11, o.rgsw_words), dtype=np.uint64)
+    assert np.array_equal(o.cmux(c0, c1, rgsw[0]), r.cmux(c0, c1, rgsw[0]))
+    for fwd in (True, False):
+        assert np.array_equal(o.blind_rotate_rlwe(c0, fwd, rgsw), r.blind_rotate_rlwe(c0, fwd, rgsw))
+
+
+@pytest.mark.parametrize("N,Q,BGbits", [(64, 0x3FFFFFFFFED001, 18), (128, 0, 7)])
+def test_blind_rotate_full(N, Q, BGbits):
+    # RLWE::blind_rotate_full (RLWE.cpp:795-820): public and private rotation
+    # factors of both signs and beyond N (poly_rotate_time reduces mod N)
+    cfg = dict(n=4, q=2 * N, N=N, Q=Q, nbit=27, BGbits=BGbits, Br=16, Bks=32)
+    o, r = pair(cfg)
+    rng = np.random.default_rng(N + BGbits)
+    x = rng.integers(0, o.Q, (2, N), dtype=np.uint64)
+    a = np.array([3, -5, N + 7, -(N + 1), 0, 2 * N - 1], dtype=np.int32)
+    rgsw = rng.integers(0, o.Q, (a.size, o.rgsw_words), dtype=np.uint64)
+    for b in (0, 9, -4, N):
+        assert np.array_equal(o.blind_rotate_full(x, b, a, rgsw), r.blind_rotate_full(x, b, a, rgsw)), b
