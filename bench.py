@@ -55,6 +55,7 @@ def parse():
     ap.add_argument("--serve-inflight", type=int, default=2)
     ap.add_argument("--serve-min-fill", type=int, default=0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-parameter-set batch")
+    ap.add_argument("--no-leveled", action="store_true", help="skip the RLWE expansion / blind-rotate microbench")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (reference dispatch for HBM byte counters)")
     return ap.parse_args()
@@ -270,6 +271,51 @@ def mixed(fhe, torch, main_ctx, main_keys, total=8192, reps=3):
     }
 
 
+def leveled(fhe, torch, B=8, reps=3):
+    """RLWE expansion (expand_RLWE) at the reference's expansion app size --
+    N=2048, Q=0x3FFFFFFFFED001, Bks_R=2^9 (digitKS_R=6) -- B inputs per call,
+    N-1 substitution + key switch steps each; and the leveled blind_rotate
+    (11 CMUXes, BG=2^9 d=6) over 256 RLWEs.  Seeded random words and keys."""
+    kw = dict(n=512, q=512, N=2048, Q=0x3FFFFFFFFED001, nbit=54, BGbits=9, Br=23, Bks=25, Bks_R_bits=9)
+    ctx = fhe.Context(**kw)
+    z = ctx.rlwe_secret(SEED + 6)
+    keys = ctx.keys()
+    keys.generate_subs(SEED + 7, z)
+    rng = np.random.default_rng(SEED + 6)
+    N = 2048
+    x = torch.from_numpy(rng.integers(0, ctx.Q, (B, 2, N), dtype=np.uint64).view(np.int64)).cuda()
+    out = torch.zeros((B, N, 2, N), dtype=torch.int64, device="cuda")
+    ctx.expand_rlwe_dev(keys, x, out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        ctx.expand_rlwe_dev(keys, x, out)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del out
+    R = 256
+    rl = torch.from_numpy(rng.integers(0, ctx.Q, (R, 2, N), dtype=np.uint64).view(np.int64)).cuda()
+    rg = torch.from_numpy(rng.integers(0, ctx.Q, (11, ctx.digitG * 4 * N), dtype=np.uint64).view(np.int64)).cuda()
+    ro = torch.zeros_like(rl)
+    ctx.blind_rotate_rlwe_dev(rl, True, rg, ro)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        ctx.blind_rotate_rlwe_dev(rl, True, rg, ro)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_br = e0.elapsed_time(e1) / reps
+    return {
+        "workload": "expand_RLWE, N=2048 54-bit Q, Bks_R=2^9 (6 digits), %d inputs per call; blind_rotate (11 CMUX, "
+                    "BG=2^9 d=6) over %d RLWEs" % (B, R),
+        "expansion_ms_per_call": round(ms, 3), "expansions_per_s": round(B / (ms * 1e-3), 1),
+        "subs_keyswitch_per_s": round(B * (N - 1) / (ms * 1e-3), 1),
+        "blind_rotate_ms_per_call": round(ms_br, 3), "cmux_per_s": round(R * 11 / (ms_br * 1e-3), 1),
+    }
+
+
 def cpu_baseline(ctx, keys, c1, c2, gates, gpu_out, sample):
     """Time the CPU path on `sample` gates (single thread), bit-compare with the GPU."""
     import oracle
@@ -418,6 +464,15 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    # integer-VALU roofline: the kernel's VALU instructions per launch (from
+    # the committed rocprofv3 SQ counter summary of this workload) over the
+    # live launch time, against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+    valu_peak = 256 * 4 * 16 * 2.4e9
+    valu_instr = None
+    spath = os.path.join(ROOT, "profiles", "sq_counters_blind_rotate_n1024_b4096.json")
+    if os.path.exists(spath):
+        with open(spath) as f:
+            valu_instr = json.load(f)["counters_per_launch"].get("SQ_INSTS_VALU")
     rec = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -439,10 +494,13 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_blind_rotate", "avg_launch_ms": round(br_avg, 4),
                      "algorithmic_bytes_per_launch": br_bytes},
-        "valu": {"kernel": "k_blind_rotate", "external_products_per_launch": ext_products,
-                 "modmul_per_external_product": modmul_per_ep,
-                 "modmul_per_s": round(ext_products * modmul_per_ep / (br_avg * 1e-3) / 1e12, 4),
-                 "unit": "T modmul/s"},
+        "valu": {"kernel": "k_blind_rotate", "bound": "integer VALU (the kernel's roofline; no MFMA applies)",
+                 "achieved": round(valu_instr * 64 / (br_avg * 1e-3) / 1e12, 3) if valu_instr else None,
+                 "peak": round(valu_peak / 1e12, 3), "unit": "T lane-ops/s",
+                 "frac": round(valu_instr * 64 / (br_avg * 1e-3) / valu_peak, 4) if valu_instr else None,
+                 "valu_instr_per_launch": valu_instr,
+                 "external_products_per_launch": ext_products, "modmul_per_external_product": modmul_per_ep,
+                 "modmul_per_s_T": round(ext_products * modmul_per_ep / (br_avg * 1e-3) / 1e12, 4)},
         "kernels_ms_avg": {"blind_rotate": round(br_avg, 4), "lwe_keyswitch": round(ks_avg, 4)},
         "single_per_launch": {"bootstraps_per_s": round(single_rate, 2),
                               "latency_ms_p50": round(float(np.percentile(lat, 50)), 4),
@@ -471,6 +529,8 @@ def main():
         rec["kernel_microbench"] = microbench(fhe, torch, B)
     if not args.no_mixed:
         rec["mixed"] = mixed(fhe, torch, ctx, keys)
+    if not args.no_leveled:
+        rec["leveled"] = leveled(fhe, torch)
     if not args.no_cpu_baseline and args.cpu_sample > 0:
         rec["cpu_baseline"] = cpu_baseline(ctx, keys, c1, c2, gates, out, args.cpu_sample)
     print(json.dumps(rec))
