@@ -355,7 +355,7 @@ def cpu_baseline(ctx, keys, c1, c2, gates, gpu_out, sample):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus %d)" % args.gpus,
+        print("bench.py measures one GPU: this library has no multi-GPU path (asked for --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     import torch
@@ -486,7 +486,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": "batched NAND gate bootstrap, cfg-0 parameters (n=500 q=1024 N=1024 "
+        "config": {"workload": "batched NAND gate bootstrap, benchmark parameters (n=500 q=1024 N=1024 "
                                "Q=0x7FFF801 BG=2^7 d=4 Br=32 Bks=2^5), batch %d" % B,
                    "global_batch": B, "parallelism": "single GPU", "Q": ctx.Q, "digitG": d,
                    "digitR": ctx.digitR, "digitKS": dks},

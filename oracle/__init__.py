@@ -1,6 +1,6 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
+TEST INFRASTRUCTURE ONLY: the parity reference for the HIP library.  Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
 The C sources (fhe_oracle.c) restate the reference's software path
 (FHE_ACC_runtime/src/{nbt,params,LWE,RLWE,app}.cpp); each C function cites the

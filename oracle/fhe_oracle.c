@@ -1,7 +1,7 @@
 /*
  * fhe_oracle.c -- CPU oracle for the FHEW/TFHE gate-bootstrapping path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker for the HIP library, never the product).
+ * TEST INFRASTRUCTURE ONLY (the parity reference for the HIP library, never the product).
  *
  * Each function restates the reference routine it cites.  Arithmetic is the
  * reference's own: canonical results from unsigned/signed __int128 '%'
@@ -28,7 +28,7 @@
  *   orc_blind_rotate / orc_eval_bootstrap -> oracle/_ref (reference
  *           LWE.cpp/RLWE.cpp compiled here, explicit keys), scale_round closed
  *           form, transpose invariant, decrypted truth tables.
- *   orc_rng_word / orc_gauss / keygen -> this build's own sampler (the
+ *   orc_rng_word / orc_gauss / keygen -> this library's own sampler (the
  *           reference seeds from std::random_device, RNG.cpp:5-6, so it has no
  *           reproducible keys).  Pinned by decryption of fresh encryptions and
  *           by the device key generator agreeing word for word.
@@ -680,7 +680,7 @@ int orc_eval_bootstrap_batch(const orc_params *p, const uint64_t *c1, const uint
 }
 
 /* ------------------------------------------------------------------ */
-/* seeded sampler and key generation (this build's own; the reference  */
+/* seeded sampler and key generation (this library's own; the reference  */
 /* draws from std::random_device, RNG.cpp:5-6,20-21)                   */
 /* ------------------------------------------------------------------ */
 

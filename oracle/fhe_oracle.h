@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's
  * software path (FHE_ACC_runtime/src/{nbt,params,LWE,RLWE,app}.cpp) used as the
- * checker for the HIP library.  Only tests/, __graft_entry__.smoke() and the
+ * parity reference for the HIP library.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it.  The product never links it.
  *
  * Arithmetic follows the reference literally: every modular product is an
@@ -138,7 +138,7 @@ int  orc_eval_bootstrap_batch(const orc_params *p, const uint64_t *c1, const uin
                               const uint64_t *ksk, uint64_t *out, uint64_t *cap_acc,
                               uint64_t *cap_ks, int threads);
 
-/* ---------- seeded key generation (this build's own sampler) ---------- */
+/* ---------- seeded key generation (this library's own sampler) ---------- */
 uint64_t orc_rng_word(uint64_t seed, uint64_t stream, uint64_t ctr);
 int64_t  orc_gauss(const orc_params *p, uint64_t word);
 void orc_lwe_secret(const orc_params *p, uint64_t seed, int binary, uint64_t *sk /* [n] mod q */);

@@ -56,7 +56,7 @@ def test_external_product_golden(golden_dir):
 
 
 @pytest.mark.parametrize("N,Q,BGbits", [(1024, 0x3FFFFFFFFED001, 9),   # microbench shape, d=6
-                                        (1024, 0, 7),                   # cfg-0 shape, d=4
+                                        (1024, 0, 7),                   # benchmark shape, d=4
                                         (2048, 0x3FFFFFFFFED001, 18),   # set B, d=3
                                         (512, 0, 6)])                   # set C, d=5
 @pytest.mark.parametrize("per_ct", [False, True])

@@ -6,8 +6,7 @@ word stream (blind rotation with the zero-digit skip, LWE_keyswitch with its
 Q/8+1 offset and d=0 rows, LWE_modswitch), acc_initialization for every gate
 and b, and external products at other (BG, d, Q, N).  Keys are explicit
 inputs from the oracle's seeded generator (the reference's own keygen draws
-from std::random_device, RNG.cpp:5-6).  Skipped when _ref was not built (the
-reference sources are not present on the GPU box).
+from std::random_device, RNG.cpp:5-6).  Skipped when _ref was not built (where the reference sources are absent).
 """
 import numpy as np
 import pytest
