@@ -87,6 +87,8 @@ EXPORTS = {
     "fheacc_keys_generate": [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p],
     "fheacc_keys_generate_subs": [C.c_void_p, C.c_uint64, C.c_void_p],
     "fheacc_keys_digest": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "fheacc_keys_save": [C.c_void_p, C.c_char_p],
+    "fheacc_keys_load": [C.c_void_p, C.c_char_p],
     "fheacc_lwe_secret": [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p],
     "fheacc_rlwe_secret": [C.c_void_p, C.c_uint64, C.c_void_p],
     "fheacc_lwe_encrypt": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.c_void_p],
@@ -415,6 +417,14 @@ class Keys:
     def generate_subs(self, seed, rlwe_sk):
         rlwe_sk, p = _np(rlwe_sk)
         _check(lib().fheacc_keys_generate_subs(self.h, seed, p))
+
+    def save(self, path):
+        """Checkpoint the resident keys to a file (fheacc_keys_save)."""
+        _check(lib().fheacc_keys_save(self.h, os.fsencode(path)))
+
+    def load(self, path):
+        """Restore keys written by save() for this parameter set (fheacc_keys_load)."""
+        _check(lib().fheacc_keys_load(self.h, os.fsencode(path)))
 
     def digest(self):
         a, b = C.c_uint64(), C.c_uint64()

@@ -1,6 +1,7 @@
 // api.cpp -- the C ABI (include/fheacc.h) over the HIP kernels.
 #include <errno.h>
 #include <hip/hip_runtime.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
@@ -209,6 +210,9 @@ int fheacc_ctx_create(const fheacc_params *params, int device, fheacc_ctx **out)
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
         lds_max > 0)
         c->lds_budget = std::min<size_t>((size_t)lds_max, 160 * 1024);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->tune.small_max = cus;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(fail("stream"));
     c->own_stream = true;
     const HostParams &p = c->p;
@@ -616,6 +620,127 @@ int fheacc_keys_digest(fheacc_keys *k, uint64_t *bd, uint64_t *kd) {
     return 0;
 }
 
+// ------------------------------------------------------- key checkpoints
+//
+// File: 17 little-endian uint64 header fields, then the resident keys in the
+// device layout, bsk, ksk, subs (absent ones have 0 bytes):
+//   magic "FHEACCK1", version, device word bytes, n, N, q, Q, BGbits, Br,
+//   Bks, Bks_R_bits, bsk_bytes, ksk_bytes, subs_bytes, and a digest of each.
+// Keys stream through a pinned 64 MiB buffer; a multi-GB key never needs a
+// host copy of its own.
+
+namespace {
+constexpr uint64_t kKeyMagic = 0x314B434341454846ULL;  // "FHEACCK1"
+constexpr uint64_t kKeyVersion = 1;
+constexpr size_t kKeyChunk = 64 << 20;
+
+struct PinnedBuf {
+    void *p = nullptr;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+struct File {
+    FILE *f = nullptr;
+    ~File() {
+        if (f) fclose(f);
+    }
+};
+}  // namespace
+
+int fheacc_keys_save(fheacc_keys *k, const char *path) {
+    if (!k || !path) return fail("null argument");
+    fheacc_ctx *c = k->ctx;
+    const HostParams &p = c->p;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    uint64_t dg[3] = {0, 0, 0};
+    const void *sec[3] = {k->bsk, k->ksk, k->subs};
+    const size_t len[3] = {k->bsk ? k->bsk_bytes : 0, k->ksk ? k->ksk_bytes : 0, k->subs ? k->subs_bytes : 0};
+    for (int i = 0; i < 3; i++)
+        if (int rc = digest_of(c, sec[i], len[i], &dg[i])) return rc;
+    const uint64_t hdr[17] = {kKeyMagic, kKeyVersion, (uint64_t)wsize(c), (uint64_t)p.n, (uint64_t)p.N, p.q, p.Q,
+                              p.BGbits, p.Br, p.Bks, p.Bks_R_bits, len[0], len[1], len[2], dg[0], dg[1], dg[2]};
+    File f;
+    f.f = fopen(path, "wb");
+    if (!f.f) return fail(std::string("cannot open ") + path + " for writing");
+    if (fwrite(hdr, 8, 17, f.f) != 17) return fail("write failed");
+    PinnedBuf pb;
+    HIP_TRY(hipHostMalloc(&pb.p, kKeyChunk, hipHostMallocDefault));
+    for (int i = 0; i < 3; i++) {
+        for (size_t off = 0; off < len[i]; off += kKeyChunk) {
+            const size_t m = std::min(kKeyChunk, len[i] - off);
+            HIP_TRY(hipMemcpyAsync(pb.p, (const char *)sec[i] + off, m, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            if (fwrite(pb.p, 1, m, f.f) != m) return fail("write failed");
+        }
+    }
+    if (fclose(f.f) != 0) {
+        f.f = nullptr;
+        return fail("write failed");
+    }
+    f.f = nullptr;
+    return 0;
+}
+
+int fheacc_keys_load(fheacc_keys *k, const char *path) {
+    if (!k || !path) return fail("null argument");
+    fheacc_ctx *c = k->ctx;
+    const HostParams &p = c->p;
+    File f;
+    f.f = fopen(path, "rb");
+    if (!f.f) return fail(std::string("cannot open ") + path);
+    uint64_t hdr[17];
+    if (fread(hdr, 8, 17, f.f) != 17 || hdr[0] != kKeyMagic) return fail("not a key file");
+    if (hdr[1] != kKeyVersion) return fail("unsupported key file version " + std::to_string(hdr[1]));
+    if (hdr[2] != wsize(c) || hdr[3] != (uint64_t)p.n || hdr[4] != (uint64_t)p.N || hdr[5] != p.q || hdr[6] != p.Q ||
+        hdr[7] != p.BGbits || hdr[8] != p.Br || hdr[9] != p.Bks || hdr[10] != p.Bks_R_bits)
+        return fail("key file belongs to another parameter set");
+    const size_t want[3] = {bsk_dev_bytes(p), ksk_dev_bytes(p), subs_dev_bytes(p)};
+    for (int i = 0; i < 3; i++)
+        if (hdr[11 + i] != 0 && hdr[11 + i] != want[i]) return fail("key file section size mismatch");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // load into fresh buffers; the key set changes only when all of it checks out
+    void *buf[3] = {nullptr, nullptr, nullptr};
+    auto release = [&](int rc) {
+        for (void *b : buf)
+            if (b) (void)hipFree(b);
+        return rc;
+    };
+    PinnedBuf pb;
+    HIP_TRY(hipHostMalloc(&pb.p, kKeyChunk, hipHostMallocDefault));
+    for (int i = 0; i < 3; i++) {
+        const size_t n = hdr[11 + i];
+        if (!n) continue;
+        hipError_t e = hipMalloc(&buf[i], n);
+        if (e != hipSuccess) return release(fail("key allocation", FHEACC_ENOMEM));
+        for (size_t off = 0; off < n; off += kKeyChunk) {
+            const size_t m = std::min(kKeyChunk, n - off);
+            if (fread(pb.p, 1, m, f.f) != m) return release(fail("key file truncated"));
+            e = hipMemcpyAsync((char *)buf[i] + off, pb.p, m, hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) return release(fail(std::string("key upload: ") + hipGetErrorString(e)));
+        }
+        uint64_t d = 0;
+        if (int rc = digest_of(c, buf[i], n, &d)) return release(rc);
+        if (d != hdr[14 + i]) return release(fail("key file digest mismatch (corrupted file)"));
+    }
+    void **dst[3] = {&k->bsk, &k->ksk, &k->subs};
+    size_t *dlen[3] = {&k->bsk_bytes, &k->ksk_bytes, &k->subs_bytes};
+    for (int i = 0; i < 3; i++) {
+        if (!buf[i]) continue;
+        if (*dst[i]) (void)hipFree(*dst[i]);
+        *dst[i] = buf[i];
+        *dlen[i] = hdr[11 + i];
+        buf[i] = nullptr;
+    }
+    k->bsk_digest = hdr[14];
+    k->ksk_digest = hdr[15];
+    return 0;
+}
+
 // ---------------------------------------------------------------- host API
 
 int fheacc_lwe_secret(const fheacc_ctx *c, uint64_t seed, int binary, uint64_t *sk) {
@@ -718,9 +843,10 @@ static int gate_path_impl(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch 
     a.ct_status = g.ct_status;
     const size_t lds = br_lds_bytes(p, a.pb);
     if (lds > c->lds_budget) return fail("parameter set does not fit the LDS budget");
+    const int nt = B <= c->tune.small_max && c->tune.br_threads_small ? c->tune.br_threads_small : c->tune.br_threads;
     {
         Prof pr(c, 0, g.stream, g.profile);
-        HIP_TRY(launch_blind_rotate<W>(a, p.logN, c->tune.br_threads, B, lds, g.stream));
+        HIP_TRY(launch_blind_rotate<W>(a, p.logN, nt, B, lds, g.stream));
     }
     KSArgs<W> s{};
     s.A = a.A;
@@ -1040,6 +1166,7 @@ int fheacc_ctx_set_tuning(fheacc_ctx *c, int threads, int pb) {
     std::lock_guard<std::mutex> lk(c->mu);
     c->tune.br_threads = threads;
     c->tune.br_pb = pb;
+    c->tune.br_threads_small = 0;
     return 0;
 }
 
@@ -1079,44 +1206,47 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
     HIP_TRY(hipGetLastError());
     const Tuning saved = c->tune;
     const bool saved_prof = c->profile;
+    // blind-rotation time of one shape on the first Bt synthetic ciphertexts
+    auto time_shape = [&](int threads, int pb, int64_t Bt, double *br) -> int {
+        c->tune.br_threads = threads;
+        c->tune.br_pb = pb;
+        c->tune.br_threads_small = 0;
+        {
+            std::lock_guard<std::mutex> lk(c->mu);
+            c->profile = false;
+        }
+        int rc = fheacc_bootstrap_batch_dev(c, k, g, c1, c2, out, Bt, nullptr);  // warm
+        if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail("synchronize");
+        {
+            std::lock_guard<std::mutex> lk(c->mu);
+            drop_events(c);
+            c->profile = true;
+        }
+        for (int r = 0; r < 2 && rc == 0; r++) rc = fheacc_bootstrap_batch_dev(c, k, g, c1, c2, out, Bt, nullptr);
+        double ms[4];
+        int64_t cnt[4];
+        if (rc == 0) rc = fheacc_ctx_kernel_ms(c, ms, cnt);
+        if (rc == 0) *br = ms[0] / std::max<int64_t>(cnt[0], 1);
+        return rc;
+    };
+    auto restore = [&](int rc) {
+        c->tune = saved;
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->profile = saved_prof;
+        drop_events(c);
+        return rc;
+    };
     std::string rep = "{\"batch\": " + std::to_string(B) + ", \"candidates\": [";
     double best = 1e30;
     int best_t = saved.br_threads, best_pb = saved.br_pb;
-    const int wsz = p.narrow ? 4 : 8;
     bool first = true;
     for (int threads : {128, 256, 512}) {
         if (threads > p.N) continue;
         for (int pb = 2; pb <= 2 * (int)p.digitG; pb += 2) {
             const size_t lds = br_lds_bytes(p, pb);
-            (void)wsz;
             if (lds > c->lds_budget) continue;
-            c->tune.br_threads = threads;
-            c->tune.br_pb = pb;
-            {
-                std::lock_guard<std::mutex> lk(c->mu);
-                c->profile = false;
-            }
-            int rc = fheacc_bootstrap_batch_dev(c, k, g, c1, c2, out, B, nullptr);  // warm
-            if (rc) {
-                c->tune = saved;
-                return rc;
-            }
-            HIP_TRY(hipStreamSynchronize(c->stream));
-            {
-                std::lock_guard<std::mutex> lk(c->mu);
-                drop_events(c);
-                c->profile = true;
-            }
-            for (int r = 0; r < 2 && rc == 0; r++) rc = fheacc_bootstrap_batch_dev(c, k, g, c1, c2, out, B, nullptr);
-            double ms[4];
-            int64_t cnt[4];
-            if (rc == 0) rc = fheacc_ctx_kernel_ms(c, ms, cnt);
-            if (rc) {
-                c->tune = saved;
-                c->profile = saved_prof;
-                return rc;
-            }
-            const double br = ms[0] / std::max<int64_t>(cnt[0], 1);
+            double br = 0;
+            if (int rc = time_shape(threads, pb, B, &br)) return restore(rc);
             if (!first) rep += ", ";
             first = false;
             rep += "{\"threads\": " + std::to_string(threads) + ", \"polys_per_batch\": " + std::to_string(pb) +
@@ -1128,16 +1258,33 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
             }
         }
     }
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        c->profile = saved_prof;
-        drop_events(c);
+    // the latency shape for batches up to one workgroup per CU, at the same
+    // digit batching (every shape gives the same words)
+    const int64_t Bs = std::min<int64_t>(c->tune.small_max, B);
+    double best_s = 1e30;
+    int best_ts = best_t;
+    std::string rep_s;
+    for (int threads : {128, 256, 512}) {
+        if (threads > p.N || Bs <= 0) continue;
+        double br = 0;
+        if (int rc = time_shape(threads, best_pb, Bs, &br)) return restore(rc);
+        rep_s += std::string(rep_s.empty() ? "" : ", ") + "{\"threads\": " + std::to_string(threads) +
+                 ", \"blind_rotate_ms\": " + std::to_string(br) + "}";
+        if (br < best_s) {
+            best_s = br;
+            best_ts = threads;
+        }
     }
+    restore(0);
     c->tune.br_threads = best_t;
     c->tune.br_pb = best_pb;
+    c->tune.br_threads_small = Bs > 0 ? best_ts : 0;
     HIP_TRY(hipMemset(c->status, 0, sizeof(int)));
     rep += "], \"chosen\": {\"threads\": " + std::to_string(best_t) + ", \"polys_per_batch\": " +
-           std::to_string(best_pb) + ", \"blind_rotate_ms\": " + std::to_string(best) + "}}";
+           std::to_string(best_pb) + ", \"blind_rotate_ms\": " + std::to_string(best) + "}" +
+           ", \"small_batch\": {\"max_batch\": " + std::to_string(c->tune.small_max) + ", \"tuned_at\": " +
+           std::to_string(Bs) + ", \"candidates\": [" + rep_s + "], \"threads\": " + std::to_string(best_ts) +
+           "}}";
     if (report && report_len) {
         strncpy(report, rep.c_str(), report_len - 1);
         report[report_len - 1] = 0;

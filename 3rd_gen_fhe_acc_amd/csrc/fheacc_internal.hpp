@@ -59,6 +59,11 @@ struct Tuning {
     int ks_kb = 8;            // output columns per key-switch workgroup
     int ks_sb = 32;           // key-switch steps staged per LDS fill
     int ks_gather_max = 32;   // batches up to this size use the gather key switch
+    // Batches of at most small_max ciphertexts (one workgroup per CU) leave
+    // most of the chip idle; they run at br_threads_small threads per gate,
+    // which shortens each gate (0: br_threads).  Set by the autotuner.
+    int br_threads_small = 0;
+    int64_t small_max = 0;
 };
 
 // Error text of the calling thread (fheacc_last_error) and the C ABI's

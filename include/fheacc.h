@@ -129,6 +129,15 @@ int fheacc_keys_generate_subs(fheacc_keys *keys, uint64_t seed, const uint64_t *
  * FPGA::dma_read_compare_bootstrap_key, myfpga.cpp:772-850). */
 int fheacc_keys_digest(fheacc_keys *keys, uint64_t *bsk_digest, uint64_t *ksk_digest);
 
+/* Checkpoint / restore of a resident key set: a header naming the parameter
+ * set and a digest of each key, then the keys in the device layout.  No
+ * counterpart in the reference, whose keys live only in process memory (the
+ * accelerator's key DMA, myfpga.cpp:596-712, is the nearest).  Load checks
+ * the parameter set and every digest and leaves the key set unchanged on any
+ * failure. */
+int fheacc_keys_save(fheacc_keys *keys, const char *path);
+int fheacc_keys_load(fheacc_keys *keys, const char *path);
+
 /* ---- host-side helpers of the reference host API ------------------------
  * LWE::LWE_secretkey, RLWE::RLWE_secretkey, LWE::LWE_encrypt,
  * LWE::LWE_decrypt (LWE.cpp:22-53,108-190; RLWE.cpp:243-279), with this

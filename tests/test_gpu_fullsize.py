@@ -69,3 +69,52 @@ def test_host_api_gate_demo():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ALL GATES PASSED" in r.stdout
+
+
+# the reference's own application parameter sets, full size
+APP_SETS = {
+    # apps/fhe_wo_fpga.cpp:35 and apps/bootstrap_fpga_128_ap.cpp:43 (binary LWE secret here)
+    "app_128": dict(n=512, q=512, N=1024, Q=0, nbit=27, BGbits=9, Br=23, Bks=25),
+    # apps/bootstrap_fpga_medium.cpp:43
+    "app_medium": dict(n=256, q=512, N=1024, Q=0, nbit=27, BGbits=9, Br=23, Bks=25),
+}
+
+
+@pytest.mark.parametrize("name", list(APP_SETS))
+def test_reference_app_sets(name):
+    # every gate's truth table over a batch, and every word of a few
+    # bootstraps against the oracle, at the apps' own sizes
+    kw = APP_SETS[name]
+    ctx = fhe.Context(**kw)
+    s, z = ctx.lwe_secret(321), ctx.rlwe_secret(321)
+    keys = ctx.keys()
+    keys.generate(322, s, z)
+    B = 1024
+    rng = np.random.default_rng(323)
+    m1, m2 = rng.integers(0, 2, B), rng.integers(0, 2, B)
+    gates = rng.integers(0, 6, B).astype(np.int32)
+    c1, c2 = ctx.lwe_encrypt(s, m1, 324, 0), ctx.lwe_encrypt(s, m2, 324, B)
+    out, acc, ks = ctx.bootstrap(keys, gates, c1, c2, capture=True)
+    truth = {0: m1 | m2, 1: m1 & m2, 2: 1 - (m1 | m2), 3: 1 - (m1 & m2), 4: m1 ^ m2, 5: 1 - (m1 ^ m2)}
+    exp = np.array([truth[int(g)][i] for i, g in enumerate(gates)])
+    assert np.array_equal(ctx.lwe_decrypt(s, out), exp)
+    o = Oracle(**kw)
+    idx = np.array([0, 511, 1023])
+    eo, ea, ek = o.bootstrap(c1[idx], c2[idx], gates[idx], keys.download_bsk(), keys.download_ksk(), capture=True)
+    assert np.array_equal(acc[idx], ea) and np.array_equal(ks[idx], ek) and np.array_equal(out[idx], eo)
+
+
+def test_empty_and_ragged_batches(cfg0):
+    # B = 0 is a no-op; odd sizes around the key-switch staging group (256)
+    # and the gather threshold (32) give the words of the full batch
+    ctx, keys, s = cfg0
+    assert ctx.bootstrap(keys, np.zeros(0, dtype=np.int32), np.zeros((0, ctx.n + 1), dtype=np.uint64),
+                         np.zeros((0, ctx.n + 1), dtype=np.uint64)).shape == (0, ctx.n + 1)
+    B = 515
+    rng = np.random.default_rng(9)
+    m1, m2 = rng.integers(0, 2, B), rng.integers(0, 2, B)
+    c1, c2 = ctx.lwe_encrypt(s, m1, 11, 0), ctx.lwe_encrypt(s, m2, 11, B)
+    gates = rng.integers(0, 6, B).astype(np.int32)
+    full = ctx.bootstrap(keys, gates, c1, c2)
+    for lo, hi in ((0, 1), (1, 33), (33, 290), (290, 515)):
+        assert np.array_equal(ctx.bootstrap(keys, gates[lo:hi], c1[lo:hi], c2[lo:hi]), full[lo:hi]), (lo, hi)

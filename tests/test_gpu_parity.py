@@ -254,3 +254,40 @@ def test_subs_keyswitch_matches_oracle(name):
         for i in range(3):
             exp = o.rlwe_keyswitch(o.substitute(x[i], f), subs[f * per:(f + 1) * per])
             assert np.array_equal(got[i], exp)
+
+
+def test_key_checkpoint_roundtrip(tmp_path):
+    # fheacc_keys_save / fheacc_keys_load: the restored key set is word for
+    # word the saved one and gives the same gate outputs; a corrupted or
+    # truncated file, or one of another parameter set, is refused
+    ctx, o = pair("n12_N512_q27")
+    s, z = ctx.lwe_secret(51), ctx.rlwe_secret(51)
+    keys = ctx.keys()
+    keys.generate(51, s, z)
+    keys.generate_subs(52, z)
+    path = tmp_path / "keys.bin"
+    keys.save(path)
+    k2 = ctx.keys()
+    k2.load(path)
+    assert k2.digest() == keys.digest()
+    assert np.array_equal(k2.download_bsk(), keys.download_bsk())
+    assert np.array_equal(k2.download_ksk(), keys.download_ksk())
+    assert np.array_equal(k2.download_subs(), keys.download_subs())
+    rng = np.random.default_rng(53)
+    m1, m2 = rng.integers(0, 2, 16), rng.integers(0, 2, 16)
+    c1, c2 = ctx.lwe_encrypt(s, m1, 54, 0), ctx.lwe_encrypt(s, m2, 54, 16)
+    gates = rng.integers(0, 6, 16).astype(np.int32)
+    assert np.array_equal(ctx.bootstrap(k2, gates, c1, c2), ctx.bootstrap(keys, gates, c1, c2))
+    data = bytearray(path.read_bytes())
+    data[len(data) // 2] ^= 1
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(bytes(data))
+    k3 = ctx.keys()
+    with pytest.raises(fhe.FheAccError, match="digest"):
+        k3.load(bad)
+    (tmp_path / "short.bin").write_bytes(bytes(data[: len(data) // 3]))
+    with pytest.raises(fhe.FheAccError, match="truncated"):
+        k3.load(tmp_path / "short.bin")
+    other = fhe.Context(**SETS["n16_N256_q27"])
+    with pytest.raises(fhe.FheAccError, match="parameter set"):
+        other.keys().load(path)
