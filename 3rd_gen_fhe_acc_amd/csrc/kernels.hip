@@ -582,8 +582,11 @@ __global__ __launch_bounds__(NT) void k_ntt(NTTArgs<W> a) {
     uint64_t *g = a.polys + p0 * N;
     for (int i = threadIdx.x; i < np * N; i += NT) s[pad(i)] = (W)g[i];
     __syncthreads();
-    if (a.inverse) ntt_inverse<LOGN, NT>(s, np, a.A, a.tw);
-    else ntt_forward<LOGN, NT>(s, np, a.A, a.tw);
+    // 3-stage passes for 64-bit words here (fewer registers, more resident
+    // waves; measured 15 % faster at N=1024); the fused kernels keep 4
+    constexpr int MAXR = sizeof(W) == 8 ? 3 : 4;
+    if (a.inverse) ntt_inverse<LOGN, NT, MAXR>(s, np, a.A, a.tw);
+    else ntt_forward<LOGN, NT, MAXR>(s, np, a.A, a.tw);
     for (int i = threadIdx.x; i < np * N; i += NT) g[i] = s[pad(i)];
 }
 

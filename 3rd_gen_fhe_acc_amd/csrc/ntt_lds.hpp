@@ -39,16 +39,24 @@ __host__ __device__ __forceinline__ constexpr int pad(int i) { return i + (i >> 
 // LDS words occupied by `words` logical words
 __host__ __device__ __forceinline__ constexpr int padded(int words) { return words + (words >> 3) + 1; }
 
-// stages per pass for `rem` remaining stages: 4s, never a lone 1-stage pass
-// unless the transform has a single stage
-__host__ __device__ constexpr int pass_stages(int rem) { return rem <= 4 ? rem : (rem <= 6 ? 3 : 4); }
+// stages per pass for `rem` remaining stages, at most MAXR (4 by default;
+// the standalone 64-bit NTT uses 3, where 16-word clusters and 15 twiddle
+// pairs of 64-bit words cost more registers than the saved LDS round trip
+// is worth): MAXR-stage passes, never a lone 1-stage pass unless the
+// transform has a single stage
+template <int MAXR>
+__host__ __device__ constexpr int pass_stages(int rem) {
+    return rem <= MAXR ? rem : (rem <= MAXR + 2 ? MAXR - 1 : MAXR);
+}
 // the last pass of that schedule over `rem` stages; the inverse transform
 // runs the forward schedule mirrored (3,3,4 for 1024 points), so that its
 // stride-1 first pass has 8-word clusters, which the padding keeps
 // conflict-free
+template <int MAXR>
 __host__ __device__ constexpr int last_pass_stages(int rem) {
-    return rem <= pass_stages(rem) ? rem : last_pass_stages(rem - pass_stages(rem));
+    return rem <= pass_stages<MAXR>(rem) ? rem : last_pass_stages<MAXR>(rem - pass_stages<MAXR>(rem));
 }
+
 
 // Physical offset of cluster element e (stride S) from the cluster's first
 // word b0: pad(b0 + e S) - pad(b0).  When S 2^R >= 8, b0 = 8m + off with
@@ -207,41 +215,41 @@ __device__ __forceinline__ void inv_pass(W *s, int np, const Arith<W> &A, const 
     }
 }
 
-template <int LOGN, int ST, int NT, typename W>
+template <int LOGN, int ST, int NT, int MAXR, typename W>
 __device__ __forceinline__ void fwd_passes(W *s, int np, bool canonical, const Arith<W> &A, const Twiddles<W> &tw,
                                            int tid) {
     if constexpr (ST < LOGN) {
-        constexpr int R = pass_stages(LOGN - ST);
+        constexpr int R = pass_stages<MAXR>(LOGN - ST);
         fwd_pass<LOGN, ST, R, NT>(s, np, canonical && ST + R == LOGN, A, tw, tid);
         __syncthreads();
-        fwd_passes<LOGN, ST + R, NT>(s, np, canonical, A, tw, tid);
+        fwd_passes<LOGN, ST + R, NT, MAXR>(s, np, canonical, A, tw, tid);
     }
 }
 
-template <int LOGN, int ST, int NT, typename W>
+template <int LOGN, int ST, int NT, int MAXR, typename W>
 __device__ __forceinline__ void inv_passes(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid) {
     if constexpr (ST < LOGN) {
-        constexpr int R = last_pass_stages(LOGN - ST);
+        constexpr int R = last_pass_stages<MAXR>(LOGN - ST);
         inv_pass<LOGN, ST, R, NT>(s, np, A, tw, tid);
         __syncthreads();
-        inv_passes<LOGN, ST + R, NT>(s, np, A, tw, tid);
+        inv_passes<LOGN, ST + R, NT, MAXR>(s, np, A, tw, tid);
     }
 }
 
 // Forward transform of np padded polys in LDS (canonical inputs).  Output
 // canonical in [0, Q), or, when canonical is false, below (2 logN + 1) Q and
 // congruent to the canonical word (for consumers that reduce anyway).
-template <int LOGN, int NT, typename W>
+template <int LOGN, int NT, int MAXR = 4, typename W>
 __device__ void ntt_forward(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, bool canonical = true,
                             int tid = -1) {
-    fwd_passes<LOGN, 0, NT>(s, np, canonical, A, tw, tid < 0 ? (int)threadIdx.x : tid);
+    fwd_passes<LOGN, 0, NT, MAXR>(s, np, canonical, A, tw, tid < 0 ? (int)threadIdx.x : tid);
 }
 
 // Inverse transform of np padded polys in LDS (inputs in [0, 2Q)); output
 // canonical in [0, Q).
-template <int LOGN, int NT, typename W>
+template <int LOGN, int NT, int MAXR = 4, typename W>
 __device__ void ntt_inverse(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid = -1) {
-    inv_passes<LOGN, 0, NT>(s, np, A, tw, tid < 0 ? (int)threadIdx.x : tid);
+    inv_passes<LOGN, 0, NT, MAXR>(s, np, A, tw, tid < 0 ? (int)threadIdx.x : tid);
 }
 
 }  // namespace fheacc
