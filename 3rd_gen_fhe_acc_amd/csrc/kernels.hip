@@ -1108,6 +1108,9 @@ hipError_t launch_blind_rotate(const BRArgs<W> &a, int logN, int nt, int64_t B, 
     FHEACC_LOGN_SWITCH(logN, {
         if (nt == 128 && LOGN >= 7) return launch_br_nt<W, LOGN, 128>(a, B, lds, st);
         if (nt == 512 && LOGN >= 9) return launch_br_nt<W, LOGN, 512>(a, B, lds, st);
+        // 1024 threads: 32-bit words only (64-bit ones spill at 128 registers)
+        if constexpr (sizeof(W) == 4 && LOGN >= 10)
+            if (nt == 1024) return launch_br_nt<W, LOGN, 1024>(a, B, lds, st);
         return launch_br_nt<W, LOGN, 256>(a, B, lds, st);
     })
     return hipSuccess;
@@ -1201,6 +1204,8 @@ hipError_t launch_cmux_chain(const CmuxArgs<W> &a, int logN, int nt, int64_t cou
     FHEACC_LOGN_SWITCH(logN, {
         if (nt == 128 && LOGN >= 7) return launch_cmux_nt<W, LOGN, 128>(a, count, lds, st);
         if (nt == 512 && LOGN >= 9) return launch_cmux_nt<W, LOGN, 512>(a, count, lds, st);
+        if constexpr (sizeof(W) == 4 && LOGN >= 10)
+            if (nt == 1024) return launch_cmux_nt<W, LOGN, 1024>(a, count, lds, st);
         return launch_cmux_nt<W, LOGN, 256>(a, count, lds, st);
     })
     return hipGetLastError();

@@ -20,6 +20,8 @@ SETS = {
     "n10_N128_q54": dict(n=10, q=256, N=128, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=16, Bks=32),
     # ring degree 2048, 54-bit Q (mixed-set B shape, n reduced)
     "n8_N2048_q54": dict(n=8, q=1024, N=2048, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=32, Bks=32),
+    # the benchmark's N=1024 shape, n reduced (every launch shape, 1024 threads included)
+    "n12_N1024_q27": dict(n=12, q=1024, N=1024, Q=0, nbit=27, BGbits=7, Br=32, Bks=32),
     # ring degree 512, 27-bit Q, BG=2^6 d=5 (mixed-set C shape, n reduced)
     "n8_N512_q27_bg6": dict(n=8, q=1024, N=512, Q=0, nbit=27, BGbits=6, Br=32, Bks=32),
 }

@@ -141,9 +141,10 @@ def test_bootstrap_matches_oracle(name):
     run_bootstrap_case(name, B=24, seed=3)
 
 
-def test_every_launch_shape_bit_exact():
+@pytest.mark.parametrize("name,shapes", [("n12_N512_q27", (128, 256, 512)), ("n12_N1024_q27", (256, 1024))])
+def test_every_launch_shape_bit_exact(name, shapes):
     # the autotuner may pick any shape: all must give the same words
-    ctx, o = pair("n12_N512_q27")
+    ctx, o = pair(name)
     s, z = ctx.lwe_secret(12), ctx.rlwe_secret(12)
     keys = ctx.keys()
     keys.generate(12, s, z)
@@ -152,12 +153,19 @@ def test_every_launch_shape_bit_exact():
     c1, c2 = ctx.lwe_encrypt(s, m1, 12, 0), ctx.lwe_encrypt(s, m2, 12, 40)
     gates = rng.integers(0, 6, 40).astype(np.int32)
     exp = o.bootstrap(c1, c2, gates, keys.download_bsk(), keys.download_ksk())
-    for threads in (128, 256, 512):
+    # the CMUX chains share the launch shape
+    x0, x1 = (rng.integers(0, o.Q, (3, 2, o.N), dtype=np.uint64) for _ in range(2))
+    rg = rng.integers(0, o.Q, o.rgsw_words, dtype=np.uint64)
+    cm = np.stack([o.cmux(x0[i], x1[i], rg) for i in range(3)])
+    for threads in shapes:
         for pb in (1, 2, 3, 4, 8):
             ctx.set_tuning(threads, pb)
             assert np.array_equal(ctx.bootstrap(keys, gates, c1, c2), exp), (threads, pb)
+            out = torch.zeros((3, 2, o.N), dtype=torch.int64, device="cuda")
+            ctx.cmux_dev(dev(x0), dev(x1), dev(rg), out)
+            assert np.array_equal(host(out), cm), ("cmux", threads, pb)
     rep = ctx.autotune(keys, 256)
-    assert rep["chosen"]["threads"] in (128, 256, 512)
+    assert rep["chosen"]["threads"] in (128, 256, 512, 1024)
     assert np.array_equal(ctx.bootstrap(keys, gates, c1, c2), exp)
 
 
