@@ -111,6 +111,12 @@ EXPORTS = {
     "fheacc_blind_rotate_rlwe_dev": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int64],
     "fheacc_blind_rotate_full_dev": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                      C.c_int64],
+    "fheacc_expand_rlwe": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64],
+    "fheacc_external_product": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64],
+    "fheacc_cmux": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64],
+    "fheacc_blind_rotate_rlwe": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int64],
+    "fheacc_blind_rotate_full": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                 C.c_int64],
     "fheacc_server_create": [C.c_void_p, C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)],
     "fheacc_server_destroy": [C.c_void_p],
     "fheacc_server_add_client": [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)],
@@ -311,6 +317,22 @@ class Context:
         a = np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1))
         _check(lib().fheacc_blind_rotate_full_dev(self.h, _dptr(rlwe), int(b), a.ctypes.data_as(C.c_void_p),
                                                   _dptr(rgsws) if a.size else None, a.size, _dptr(out), count))
+
+    def expand_rlwe(self, keys, rlwe):
+        """Host arrays: rlwe [B][2][N] -> [B][N][2][N] (fheacc_expand_rlwe)."""
+        rlwe, p = _np(np.asarray(rlwe).reshape(-1, 2, self.N))
+        out = np.zeros((rlwe.shape[0], self.N, 2, self.N), dtype=np.uint64)
+        _check(lib().fheacc_expand_rlwe(self.h, keys.h, p, _np(out)[1], rlwe.shape[0]))
+        return out
+
+    def cmux(self, c0, c1, rgsw):
+        c0, p0 = _np(np.asarray(c0).reshape(-1, 2, self.N))
+        c1, p1 = _np(np.asarray(c1).reshape(-1, 2, self.N))
+        rgsw, pg = _np(rgsw)
+        out = np.zeros(c0.shape, dtype=np.uint64)
+        stride = 0 if rgsw.size == self.digitG * 4 * self.N else self.digitG * 4 * self.N
+        _check(lib().fheacc_cmux(self.h, p0, p1, pg, stride, _np(out)[1], c0.shape[0]))
+        return out
 
     def subs_keyswitch_dev(self, keys, rlwe, sub_factor, out):
         _check(lib().fheacc_subs_keyswitch_dev(self.h, keys.h, _dptr(rlwe), sub_factor, _dptr(out),

@@ -1157,6 +1157,115 @@ int fheacc_blind_rotate_full_dev(fheacc_ctx *c, const uint64_t *in, int b, const
                          count);
 }
 
+// ------------------------------------- host-buffer forms of the RLWE calls
+
+namespace {
+// Device copies of host inputs, device outputs, and the copy back, around
+// one *_dev call on the context stream.
+struct HostCall {
+    fheacc_ctx *c;
+    std::vector<void *> bufs;
+    explicit HostCall(fheacc_ctx *c_) : c(c_) {}
+    ~HostCall() {
+        for (void *p : bufs) (void)hipFree(p);
+    }
+    int in(const void *h, size_t bytes, uint64_t **d) {
+        *d = nullptr;
+        if (!bytes) return 0;
+        HIP_TRY(hipMalloc((void **)d, bytes));
+        bufs.push_back(*d);
+        HIP_TRY(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
+        return 0;
+    }
+    int out(size_t bytes, uint64_t **d) {
+        HIP_TRY(hipMalloc((void **)d, bytes));
+        bufs.push_back(*d);
+        return 0;
+    }
+    int back(void *h, const uint64_t *d, size_t bytes) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
+        return 0;
+    }
+};
+}  // namespace
+
+int fheacc_expand_rlwe(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !k || !in || !out) return fail("null argument");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    const size_t rb = 2 * (size_t)c->p.N * 8;
+    HIP_TRY(hipSetDevice(c->device));
+    HostCall h(c);
+    uint64_t *di, *dout;
+    if (int rc = h.in(in, count * rb, &di)) return rc;
+    if (int rc = h.out(count * rb * c->p.N, &dout)) return rc;
+    if (int rc = fheacc_expand_rlwe_dev(c, k, di, dout, count)) return rc;
+    return h.back(out, dout, count * rb * c->p.N);
+}
+
+int fheacc_external_product(fheacc_ctx *c, const uint64_t *acc, const uint64_t *rgsw, int64_t stride, uint64_t *out,
+                            int64_t count) {
+    if (check_ctx(c) || !acc || !rgsw || !out) return fail("null argument");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    const size_t rb = 2 * (size_t)c->p.N * 8, gb = (size_t)c->p.digitG * 4 * c->p.N * 8;
+    HIP_TRY(hipSetDevice(c->device));
+    HostCall h(c);
+    uint64_t *da, *dg, *dout;
+    if (int rc = h.in(acc, count * rb, &da)) return rc;
+    if (int rc = h.in(rgsw, stride ? count * stride * 8 : gb, &dg)) return rc;
+    if (int rc = h.out(count * rb, &dout)) return rc;
+    if (int rc = fheacc_external_product_dev(c, da, dg, stride, dout, count)) return rc;
+    return h.back(out, dout, count * rb);
+}
+
+int fheacc_cmux(fheacc_ctx *c, const uint64_t *c0, const uint64_t *c1, const uint64_t *rgsw, int64_t stride,
+                uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !c0 || !c1 || !rgsw || !out) return fail("null argument");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    const size_t rb = 2 * (size_t)c->p.N * 8, gb = (size_t)c->p.digitG * 4 * c->p.N * 8;
+    HIP_TRY(hipSetDevice(c->device));
+    HostCall h(c);
+    uint64_t *d0, *d1, *dg, *dout;
+    if (int rc = h.in(c0, count * rb, &d0)) return rc;
+    if (int rc = h.in(c1, count * rb, &d1)) return rc;
+    if (int rc = h.in(rgsw, stride ? count * stride * 8 : gb, &dg)) return rc;
+    if (int rc = h.out(count * rb, &dout)) return rc;
+    if (int rc = fheacc_cmux_dev(c, d0, d1, dg, stride, dout, count)) return rc;
+    return h.back(out, dout, count * rb);
+}
+
+int fheacc_blind_rotate_rlwe(fheacc_ctx *c, const uint64_t *in, int forward, const uint64_t *rgsw, int steps,
+                             uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !in || !rgsw || !out) return fail("null argument");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    if (steps < 1 || steps > c->p.logN) return fail("steps must be in [1, log2 N]");
+    const size_t rb = 2 * (size_t)c->p.N * 8, gb = (size_t)c->p.digitG * 4 * c->p.N * 8;
+    HIP_TRY(hipSetDevice(c->device));
+    HostCall h(c);
+    uint64_t *di, *dg, *dout;
+    if (int rc = h.in(in, count * rb, &di)) return rc;
+    if (int rc = h.in(rgsw, steps * gb, &dg)) return rc;
+    if (int rc = h.out(count * rb, &dout)) return rc;
+    if (int rc = fheacc_blind_rotate_rlwe_dev(c, di, forward, dg, steps, dout, count)) return rc;
+    return h.back(out, dout, count * rb);
+}
+
+int fheacc_blind_rotate_full(fheacc_ctx *c, const uint64_t *in, int b, const int32_t *a, const uint64_t *rgsw,
+                             int steps, uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !in || !out || (steps > 0 && (!a || !rgsw))) return fail("null argument");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    if (steps < 0 || steps > 4096) return fail("steps must be in [0, 4096]");
+    const size_t rb = 2 * (size_t)c->p.N * 8, gb = (size_t)c->p.digitG * 4 * c->p.N * 8;
+    HIP_TRY(hipSetDevice(c->device));
+    HostCall h(c);
+    uint64_t *di, *dg = nullptr, *dout;
+    if (int rc = h.in(in, count * rb, &di)) return rc;
+    if (int rc = h.in(rgsw, steps * gb, &dg)) return rc;
+    if (int rc = h.out(count * rb, &dout)) return rc;
+    if (int rc = fheacc_blind_rotate_full_dev(c, di, b, a, dg, steps, dout, count)) return rc;
+    return h.back(out, dout, count * rb);
+}
+
 // ---------------------------------------------------------------- tuning
 
 int fheacc_ctx_set_tuning(fheacc_ctx *c, int threads, int pb) {

@@ -111,7 +111,96 @@ LWE::LWE_ciphertext eval_bootstrap(const LWE::LWE_ciphertext &c1, const LWE::LWE
     return r[0];
 }
 
+static std::vector<ciphertext_t> flat(const fheacc::Context &ctx, const RLWE_ciphertext &c) {
+    const int N = ctx.N();
+    if ((int)c.a.size() != N || (int)c.b.size() != N || !c.NTT_form)
+        throw fheacc::Error(-1, "RLWE ciphertext must hold N words per polynomial, NTT domain");
+    std::vector<ciphertext_t> w(c.a);
+    w.insert(w.end(), c.b.begin(), c.b.end());
+    return w;
+}
+
+static RLWE_ciphertext unflat(int N, const ciphertext_t *w) {
+    RLWE_ciphertext c;
+    c.a.assign(w, w + N);
+    c.b.assign(w + N, w + 2 * N);
+    return c;
+}
+
+// the reference RGSW word order [h][g][a, b] (RLWE.h:313-329)
+static void flat_rgsw(const fheacc::Context &ctx, const RGSW_ciphertext &C, std::vector<ciphertext_t> &w) {
+    const uint32_t d = ctx.info().digitG;
+    if (C.c_text.size() != d) throw fheacc::Error(-1, "RGSW ciphertext must hold digitG rows");
+    for (uint32_t h = 0; h < d; h++) {
+        if (C.c_text[h].size() != 2) throw fheacc::Error(-1, "RGSW rows hold two RLWE ciphertexts");
+        for (int g = 0; g < 2; g++) {
+            std::vector<ciphertext_t> r = flat(ctx, C.c_text[h][g]);
+            w.insert(w.end(), r.begin(), r.end());
+        }
+    }
+}
+
+RLWE_ciphertext RLWE_mult_RGSW(const fheacc::Context &ctx, const RLWE_ciphertext &c, const RGSW_ciphertext &C) {
+    std::vector<ciphertext_t> in = flat(ctx, c), g, out(2 * (size_t)ctx.N());
+    flat_rgsw(ctx, C, g);
+    fheacc::check(fheacc_external_product(ctx.handle(), in.data(), g.data(), 0, out.data(), 1));
+    return unflat(ctx.N(), out.data());
+}
+
+RLWE_ciphertext CMUX(const fheacc::Context &ctx, const RLWE_ciphertext &c0, const RLWE_ciphertext &c1,
+                     const RGSW_ciphertext &SEL) {
+    std::vector<ciphertext_t> x0 = flat(ctx, c0), x1 = flat(ctx, c1), g, out(2 * (size_t)ctx.N());
+    flat_rgsw(ctx, SEL, g);
+    fheacc::check(fheacc_cmux(ctx.handle(), x0.data(), x1.data(), g.data(), 0, out.data(), 1));
+    return unflat(ctx.N(), out.data());
+}
+
+RLWE_ciphertext blind_rotate(const fheacc::Context &ctx, const RLWE_ciphertext &c_in, const bool forward,
+                             const std::vector<RGSW_ciphertext> &C) {
+    std::vector<ciphertext_t> x = flat(ctx, c_in), g, out(2 * (size_t)ctx.N());
+    for (const RGSW_ciphertext &r : C) flat_rgsw(ctx, r, g);
+    fheacc::check(fheacc_blind_rotate_rlwe(ctx.handle(), x.data(), forward ? 1 : 0, g.data(), (int)C.size(),
+                                           out.data(), 1));
+    return unflat(ctx.N(), out.data());
+}
+
+RLWE_ciphertext blind_rotate_full(const fheacc::Context &ctx, const RLWE_ciphertext &c_in, const int b,
+                                  const std::vector<int> &a, const std::vector<RGSW_ciphertext> &C) {
+    if (a.size() != C.size()) {
+        // RLWE.cpp:796-799
+        printf("The sizes of rotation vector and control vector are not equal!!!\n");
+        exit(1);
+    }
+    std::vector<ciphertext_t> x = flat(ctx, c_in), g, out(2 * (size_t)ctx.N());
+    for (const RGSW_ciphertext &r : C) flat_rgsw(ctx, r, g);
+    std::vector<int32_t> av(a.begin(), a.end());
+    fheacc::check(fheacc_blind_rotate_full(ctx.handle(), x.data(), b, av.data(), g.empty() ? nullptr : g.data(),
+                                           (int)C.size(), out.data(), 1));
+    return unflat(ctx.N(), out.data());
+}
+
 }  // namespace RLWE
+
+namespace application {
+
+substitute_key::substitute_key(const fheacc::Context &ctx, const RLWE::RLWE_secretkey &sk, uint64_t seed)
+    : keys(std::make_shared<fheacc::KeySet>(ctx)) {
+    fheacc::check(fheacc_keys_generate_subs(keys->handle(), seed, sk.a.data()));
+}
+
+std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> expand_RLWE(const RLWE::RLWE_ciphertext &c1,
+                                                                const substitute_key &subs_key) {
+    if (!subs_key.keys) throw fheacc::Error(-1, "substitute key not generated");
+    const fheacc::Context &ctx = subs_key.keys->context();
+    const int N = ctx.N();
+    std::vector<ciphertext_t> in = RLWE::flat(ctx, c1), out((size_t)N * 2 * N);
+    fheacc::check(fheacc_expand_rlwe(ctx.handle(), subs_key.keys->handle(), in.data(), out.data(), 1));
+    auto res = std::make_unique<std::vector<RLWE::RLWE_ciphertext>>(N);
+    for (int k = 0; k < N; k++) (*res)[k] = RLWE::unflat(N, out.data() + (size_t)k * 2 * N);
+    return res;
+}
+
+}  // namespace application
 
 namespace fheacc {
 

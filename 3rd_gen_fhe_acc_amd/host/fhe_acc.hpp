@@ -14,6 +14,11 @@
 //   LWE::LWE_evaluate / LWE_eval_NOT            same names (host, LWE.cpp:203-231)
 //   RLWE::eval_bootstrap(c1, c2, bt, ks, gate)  same name and arguments
 //   FPGA::fpga_eval_bootstrap_x{4,8,12,16}      fheacc::eval_bootstrap_batch (any B, int* rc)
+//   RLWE::RLWE_ciphertext / RGSW_ciphertext     same members (a, b, NTT_form; c_text[h][g])
+//   RLWE::RLWE_mult_RGSW(c, C)                  RLWE::RLWE_mult_RGSW(ctx, c, C)
+//   RLWE::CMUX / blind_rotate / blind_rotate_full  same names, ctx first
+//   application::substitute_key(sk, uni, norm)  application::substitute_key(ctx, sk, seed)
+//   application::expand_RLWE(c, subs_key)       same name and arguments
 //
 // Keys live on the device; the classes hold shared handles.  eval_bootstrap
 // on identical operands prints the reference's message and exits the process
@@ -167,7 +172,54 @@ class bootstrap_key {
 LWE::LWE_ciphertext eval_bootstrap(const LWE::LWE_ciphertext &c1, const LWE::LWE_ciphertext &c2,
                                    const bootstrap_key &btkey, const LWE::keyswitch_key &kskey, const GATES gate);
 
+// RLWE.h:127-159: the two polynomials, NTT domain (the GPU calls take and
+// return the NTT form)
+class RLWE_ciphertext {
+   public:
+    RLWE_ciphertext() = default;
+    explicit RLWE_ciphertext(int N) : a(N, 0), b(N, 0) {}
+    std::vector<ciphertext_t> a, b;
+    bool NTT_form = true;
+};
+
+// RLWE.h:313-329: c_text[h][g], h < digitG, g in {0, 1}
+class RGSW_ciphertext {
+   public:
+    RGSW_ciphertext() = default;
+    explicit RGSW_ciphertext(std::vector<std::vector<RLWE_ciphertext>> rows) : c_text(std::move(rows)) {}
+    std::vector<std::vector<RLWE_ciphertext>> c_text;
+};
+
+// RLWE.cpp:642-704
+RLWE_ciphertext RLWE_mult_RGSW(const fheacc::Context &ctx, const RLWE_ciphertext &c, const RGSW_ciphertext &C);
+// RLWE.cpp:788-793
+RLWE_ciphertext CMUX(const fheacc::Context &ctx, const RLWE_ciphertext &c0, const RLWE_ciphertext &c1,
+                     const RGSW_ciphertext &SEL);
+// RLWE.cpp:822-838 (any number of RGSWs up to log2 N; the reference fixes 11)
+RLWE_ciphertext blind_rotate(const fheacc::Context &ctx, const RLWE_ciphertext &c_in, const bool forward,
+                             const std::vector<RGSW_ciphertext> &C);
+// RLWE.cpp:795-820
+RLWE_ciphertext blind_rotate_full(const fheacc::Context &ctx, const RLWE_ciphertext &c_in, const int b,
+                                  const std::vector<int> &a, const std::vector<RGSW_ciphertext> &C);
+
 }  // namespace RLWE
+
+namespace application {
+
+// app.h substitute_key: the RLWE key switch keys from sk(X^(N/2^i + 1)) to
+// sk(X), i < log2 N, generated on the device from a seed and kept resident
+class substitute_key {
+   public:
+    substitute_key() = default;
+    substitute_key(const fheacc::Context &ctx, const RLWE::RLWE_secretkey &sk, uint64_t seed);
+    std::shared_ptr<fheacc::KeySet> keys;
+};
+
+// app.cpp:26-97
+std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> expand_RLWE(const RLWE::RLWE_ciphertext &c1,
+                                                                const substitute_key &subs_key);
+
+}  // namespace application
 
 namespace fheacc {
 

@@ -214,6 +214,19 @@ int fheacc_blind_rotate_rlwe_dev(fheacc_ctx *ctx, const uint64_t *in, int forwar
 int fheacc_blind_rotate_full_dev(fheacc_ctx *ctx, const uint64_t *in, int b, const int32_t *a, const uint64_t *rgsw,
                                  int steps, uint64_t *out, int64_t count);
 
+/* Host-buffer forms of the same calls (the reference-shaped C++ layer uses
+ * them): copies in, the device call, copies out; they return when out holds
+ * the result. */
+int fheacc_expand_rlwe(fheacc_ctx *ctx, fheacc_keys *keys, const uint64_t *in, uint64_t *out, int64_t count);
+int fheacc_external_product(fheacc_ctx *ctx, const uint64_t *acc, const uint64_t *rgsw, int64_t rgsw_stride,
+                            uint64_t *out, int64_t count);
+int fheacc_cmux(fheacc_ctx *ctx, const uint64_t *c0, const uint64_t *c1, const uint64_t *rgsw, int64_t rgsw_stride,
+                uint64_t *out, int64_t count);
+int fheacc_blind_rotate_rlwe(fheacc_ctx *ctx, const uint64_t *in, int forward, const uint64_t *rgsw, int steps,
+                             uint64_t *out, int64_t count);
+int fheacc_blind_rotate_full(fheacc_ctx *ctx, const uint64_t *in, int b, const int32_t *a, const uint64_t *rgsw,
+                             int steps, uint64_t *out, int64_t count);
+
 /* ---- serving loop -------------------------------------------------------
  * No counterpart in the reference: its apps evaluate one gate
  * (RLWE::eval_bootstrap, RLWE.cpp:726-786) or one fixed batch

@@ -128,3 +128,56 @@ def test_leveled_argument_errors():
     keys = ctx.keys()
     with pytest.raises(fhe.FheAccError, match="subs key"):
         ctx.expand_rlwe_dev(keys, x, torch.zeros((1, 256, 2, 256), dtype=torch.int64, device="cuda"))
+
+
+@pytest.mark.parametrize("N,Q,BGbits,BksR", [(256, 0, 7, 7), (128, 0x3FFFFFFFFED001, 18, 9)])
+def test_cpp_rlwe_api_demo(tmp_path, N, Q, BGbits, BksR):
+    # the reference-shaped C++ layer (host/fhe_acc.hpp: application::
+    # expand_RLWE, RLWE::RLWE_mult_RGSW, CMUX, blind_rotate, blind_rotate_full)
+    # through lib/rlwe_demo on words from files, against the oracle
+    import os
+    import subprocess
+    from fheacc_testlib import ROOT
+    kw = dict(n=8, q=min(2 * N, 1024), N=N, Q=Q, nbit=54 if Q else 27, BGbits=BGbits, Br=32, Bks=32,
+              Bks_R_bits=BksR)
+    o = Oracle(**kw)
+    ctx = fhe.Context(**kw)
+    z = ctx.rlwe_secret(61)
+    keys = ctx.keys()
+    keys.generate_subs(62, z)
+    subs = keys.download_subs()
+    rng = np.random.default_rng(N + BGbits)
+    logN = N.bit_length() - 1
+    x = rng.integers(0, o.Q, (2, 2, N), dtype=np.uint64)
+    rg = rng.integers(0, o.Q, (logN, o.rgsw_words), dtype=np.uint64)
+    a = np.array([(-1) ** i * (3 * i + 1) for i in range(logN)], dtype=np.int32)
+    b = -7
+    x.tofile(tmp_path / "in.bin")
+    rg.tofile(tmp_path / "rgsw.bin")
+    a.tofile(tmp_path / "a.bin")
+    exe = os.path.join(ROOT, "3rd_gen_fhe_acc_amd", "lib", "rlwe_demo")
+    r = subprocess.run([exe, str(tmp_path), str(N), hex(o.Q), str(BGbits), str(BksR), "61", "62", str(b)],
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "RLWE DEMO DONE" in r.stdout
+    rd = lambda name, cnt: np.fromfile(tmp_path / name, dtype=np.uint64).reshape(cnt, 2, N)
+    assert np.array_equal(rd("expand.bin", N), o.expand_rlwe(x[0], subs))
+    assert np.array_equal(rd("mult.bin", 1)[0], o.mult_rgsw(x[0][None], rg[0])[0])
+    assert np.array_equal(rd("cmux.bin", 1)[0], o.cmux(x[0], x[1], rg[0]))
+    assert np.array_equal(rd("br_fwd.bin", 1)[0], o.blind_rotate_rlwe(x[0], True, rg))
+    assert np.array_equal(rd("br_bwd.bin", 1)[0], o.blind_rotate_rlwe(x[0], False, rg))
+    assert np.array_equal(rd("brf.bin", 1)[0], o.blind_rotate_full(x[0], b, a, rg))
+
+
+def test_host_buffer_forms():
+    # fheacc_expand_rlwe / fheacc_cmux (host buffers) equal the device forms
+    ctx, o, keys = subs_pair(SETS["N128_q54"], 63)
+    rng = np.random.default_rng(63)
+    x = rng.integers(0, o.Q, (2, 2, o.N), dtype=np.uint64)
+    out = torch.zeros((2, o.N, 2, o.N), dtype=torch.int64, device="cuda")
+    ctx.expand_rlwe_dev(keys, dev(x), out)
+    assert np.array_equal(ctx.expand_rlwe(keys, x), host(out))
+    rg = rng.integers(0, o.Q, (2, o.rgsw_words), dtype=np.uint64)
+    got = ctx.cmux(x[0:2], x[1::-1], rg)
+    for i in range(2):
+        assert np.array_equal(got[i], o.cmux(x[i], x[1 - i], rg[i]))
