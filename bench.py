@@ -340,11 +340,23 @@ def cpu_baseline(ctx, keys, c1, c2, gates, gpu_out, sample):
         secs = time.perf_counter() - t0
         kind = "port"
         what = "oracle/fhe_oracle.c restatement of eval_bootstrap"
+    # all host cores: the oracle's restatement, one ciphertext per thread
+    # (the reference's own path is single-threaded)
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    o = oracle.Oracle(**PARAMS)
+    mc = min(len(c1), 4 * threads)
+    t0 = time.perf_counter()
+    out_mc = o.bootstrap(c1[:mc], c2[:mc], gates[:mc], bsk, ksk, threads=threads)
+    mc_secs = time.perf_counter() - t0
+    multicore = {"value": round(mc / mc_secs, 3), "unit": "bootstraps/s", "cores": threads, "kind": "port",
+                 "sample": "%d NAND gate bootstraps, oracle/fhe_oracle.c restatement, %d threads" % (mc, threads),
+                 "bit_exact_vs_gpu": bool(np.array_equal(out_mc, gpu_out[:mc]))}
     return {
         "value": sample / secs,
         "unit": "bootstraps/s",
         "cores": 1,
         "kind": kind,
+        "all_cores": multicore,
         "sample": "%d NAND gate bootstraps of the benchmark batch, single thread, %s; key load %.1f s untimed"
                   % (sample, what, load_s),
         "seconds": secs,
