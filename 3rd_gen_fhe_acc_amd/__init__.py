@@ -26,6 +26,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libfheacc.so")
 
 GATES = {"OR": 0, "AND": 1, "NOR": 2, "NAND": 3, "XOR": 4, "XNOR": 5}
+# blind-rotation orders (include/fheacc.h FHEACC_EVAL_*)
+EVAL_SOFTWARE, EVAL_FPGA_X1, EVAL_FPGA_XN = 0, 1, 2
 
 
 class FheAccError(RuntimeError):
@@ -75,6 +77,8 @@ EXPORTS = {
     "fheacc_ctx_kernel_ms": [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     "fheacc_ctx_set_tuning": [C.c_void_p, C.c_int, C.c_int],
     "fheacc_ctx_get_tuning": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "fheacc_ctx_set_eval_mode": [C.c_void_p, C.c_int],
+    "fheacc_ctx_get_eval_mode": [C.c_void_p, C.POINTER(C.c_int)],
     "fheacc_autotune": [C.c_void_p, C.c_void_p, C.c_int64, C.c_char_p, C.c_size_t],
     "fheacc_keys_create": [C.c_void_p, C.POINTER(C.c_void_p)],
     "fheacc_keys_destroy": [C.c_void_p],
@@ -84,6 +88,10 @@ EXPORTS = {
     "fheacc_keys_download_bsk": [C.c_void_p, C.c_void_p],
     "fheacc_keys_download_ksk": [C.c_void_p, C.c_void_p],
     "fheacc_keys_download_subs": [C.c_void_p, C.c_void_p],
+    "fheacc_keys_export_fpga_bsk": [C.c_void_p, C.c_void_p],
+    "fheacc_keys_import_fpga_bsk": [C.c_void_p, C.c_void_p],
+    "fheacc_keys_export_fpga_subs": [C.c_void_p, C.c_void_p],
+    "fheacc_keys_import_fpga_subs": [C.c_void_p, C.c_void_p],
     "fheacc_keys_generate": [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p],
     "fheacc_keys_generate_subs": [C.c_void_p, C.c_uint64, C.c_void_p],
     "fheacc_keys_digest": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
@@ -219,6 +227,20 @@ class Context:
         t, p = C.c_int(), C.c_int()
         _check(lib().fheacc_ctx_get_tuning(self.h, C.byref(t), C.byref(p)))
         return t.value, p.value
+
+    @property
+    def eval_mode(self):
+        """Blind-rotation order of the gate bootstrap: EVAL_SOFTWARE (residue-0
+        steps skipped, RLWE.cpp:754-767), EVAL_FPGA_X1 (step (0,0) always
+        multiplies, myfpga.cpp:1030-1071) or EVAL_FPGA_XN (every step
+        multiplies, btkey[0] = RGSW(1) for residue 0, myfpga.cpp:1180-1225)."""
+        m = C.c_int()
+        _check(lib().fheacc_ctx_get_eval_mode(self.h, C.byref(m)))
+        return m.value
+
+    @eval_mode.setter
+    def eval_mode(self, mode):
+        _check(lib().fheacc_ctx_set_eval_mode(self.h, int(mode)))
 
     def autotune(self, keys, batch):
         """Time every blind-rotation launch shape on `batch` synthetic
@@ -430,6 +452,30 @@ class Keys:
         out = np.zeros(self.ctx.info["subs_words"], dtype=np.uint64)
         _check(lib().fheacc_keys_download_subs(self.h, _np(out)[1]))
         return out
+
+    # the accelerator's DDR key images (FPGA::dma_write_bootstrap_key /
+    # dma_write_subs_key, myfpga.cpp:596-712)
+    def export_fpga_bsk(self):
+        out = np.zeros(self.ctx.info["bsk_words"], dtype=np.uint64)
+        _check(lib().fheacc_keys_export_fpga_bsk(self.h, _np(out)[1]))
+        return out
+
+    def import_fpga_bsk(self, image):
+        image, p = _np(image)
+        if image.size != self.ctx.info["bsk_words"]:
+            raise ValueError("bootstrap key image must hold %d words" % self.ctx.info["bsk_words"])
+        _check(lib().fheacc_keys_import_fpga_bsk(self.h, p))
+
+    def export_fpga_subs(self):
+        out = np.zeros(self.ctx.info["subs_words"], dtype=np.uint64)
+        _check(lib().fheacc_keys_export_fpga_subs(self.h, _np(out)[1]))
+        return out
+
+    def import_fpga_subs(self, image):
+        image, p = _np(image)
+        if image.size != self.ctx.info["subs_words"]:
+            raise ValueError("subs key image must hold %d words" % self.ctx.info["subs_words"])
+        _check(lib().fheacc_keys_import_fpga_subs(self.h, p))
 
     def generate(self, seed, lwe_sk, rlwe_sk):
         lwe_sk, p1 = _np(lwe_sk)

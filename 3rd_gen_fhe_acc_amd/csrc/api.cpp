@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -68,7 +69,7 @@ size_t wsize(const fheacc_ctx *c) { return c->p.narrow ? 4 : 8; }
 size_t rgsw_elems(const HostParams &p) { return (size_t)p.digitG * p.N * (p.narrow ? 4 : 8); }
 size_t rgsw_bytes(const HostParams &p) { return rgsw_elems(p) * (p.narrow ? 4 : 8); }
 int n_pad(const HostParams &p) { return (p.n + 1 + 7) / 8 * 8; }
-size_t bsk_dev_bytes(const HostParams &p) { return (size_t)p.n * p.digitR * (p.Br - 1) * rgsw_bytes(p); }
+size_t bsk_dev_bytes(const HostParams &p) { return (size_t)p.n * p.digitR * p.Br * rgsw_bytes(p); }
 size_t ksk_dev_bytes(const HostParams &p) {
     return (size_t)p.N * p.digitKS * p.Bks * n_pad(p) * (p.narrow ? 4 : 8);
 }
@@ -332,63 +333,92 @@ static int alloc_key(void **p, size_t bytes) {
     return 0;
 }
 
-// map[device RGSW e] = reference RGSW index (residue 0 not stored)
+// map[device RGSW e] = reference RGSW index: device [n][digitR][Br],
+// reference btkey[Br][digitR][n] (RLWE.cpp:579-606)
 static std::vector<int64_t> bsk_map(const HostParams &p) {
-    std::vector<int64_t> m((size_t)p.n * p.digitR * (p.Br - 1));
+    std::vector<int64_t> m((size_t)p.n * p.digitR * p.Br);
     size_t e = 0;
     for (int k = 0; k < p.n; k++)
         for (uint32_t j = 0; j < p.digitR; j++)
-            for (uint64_t r = 1; r < p.Br; r++) m[e++] = ((int64_t)r * p.digitR + j) * p.n + k;
+            for (uint64_t r = 0; r < p.Br; r++) m[e++] = ((int64_t)r * p.digitR + j) * p.n + k;
     return m;
 }
 
-int fheacc_keys_upload_bsk(fheacc_keys *k, const uint64_t *bsk) {
-    if (!k || !bsk) return fail("null argument");
+// Bootstrap-key words in (host -> device layout) and out.  ddr = false: the
+// reference's btkey[Br][digitR][n] order; ddr = true: the accelerator's DDR
+// image, whose RGSW order [n][digitR][Br] is the device order.
+static int bsk_in(fheacc_keys *k, const uint64_t *words, bool ddr) {
     fheacc_ctx *c = k->ctx;
     const HostParams &p = c->p;
     HIP_TRY(hipSetDevice(c->device));
     if (int rc = alloc_key(&k->bsk, bsk_dev_bytes(p))) return rc;
     k->bsk_bytes = bsk_dev_bytes(p);
     const size_t rw = (size_t)p.digitG * 4 * p.N;
-    const size_t words = (size_t)p.Br * p.digitR * p.n * rw;
+    const int64_t count = (int64_t)p.Br * p.digitR * p.n;
     DevBuf stage, dmap;
-    HIP_TRY(hipMalloc(&stage.p, words * 8));
-    HIP_TRY(hipMemcpy(stage.p, bsk, words * 8, hipMemcpyHostToDevice));
-    std::vector<int64_t> m = bsk_map(p);
-    HIP_TRY(hipMalloc(&dmap.p, m.size() * 8));
-    HIP_TRY(hipMemcpy(dmap.p, m.data(), m.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&stage.p, count * rw * 8));
+    HIP_TRY(hipMemcpy(stage.p, words, count * rw * 8, hipMemcpyHostToDevice));
+    const int64_t *map = nullptr;
+    if (!ddr) {
+        std::vector<int64_t> m = bsk_map(p);
+        HIP_TRY(hipMalloc(&dmap.p, m.size() * 8));
+        HIP_TRY(hipMemcpy(dmap.p, m.data(), m.size() * 8, hipMemcpyHostToDevice));
+        map = (const int64_t *)dmap.p;
+    }
     hipError_t e = p.narrow
         ? launch_rgsw_to_device<uint32_t>((const uint64_t *)stage.p, rw, (uint32_t *)k->bsk, rgsw_elems(p),
-                                          p.digitG, p.N, make_sg(p.Q), (const int64_t *)dmap.p, m.size(), c->stream)
+                                          p.digitG, p.N, make_sg(p.Q), map, count, c->stream, ddr)
         : launch_rgsw_to_device<uint64_t>((const uint64_t *)stage.p, rw, (uint64_t *)k->bsk, rgsw_elems(p),
-                                          p.digitG, p.N, make_sg(p.Q), (const int64_t *)dmap.p, m.size(), c->stream);
+                                          p.digitG, p.N, make_sg(p.Q), map, count, c->stream, ddr);
     HIP_TRY(e);
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 
-int fheacc_keys_download_bsk(fheacc_keys *k, uint64_t *bsk) {
-    if (!k || !bsk || !k->bsk) return fail("no bootstrap key resident");
+static int bsk_out(fheacc_keys *k, uint64_t *words, bool ddr) {
     fheacc_ctx *c = k->ctx;
     const HostParams &p = c->p;
     HIP_TRY(hipSetDevice(c->device));
     const size_t rw = (size_t)p.digitG * 4 * p.N;
-    const size_t words = (size_t)p.Br * p.digitR * p.n * rw;
+    const int64_t count = (int64_t)p.Br * p.digitR * p.n;
     DevBuf stage, dmap;
-    HIP_TRY(hipMalloc(&stage.p, words * 8));
-    HIP_TRY(hipMemsetAsync(stage.p, 0, words * 8, c->stream));
-    std::vector<int64_t> m = bsk_map(p);
-    HIP_TRY(hipMalloc(&dmap.p, m.size() * 8));
-    HIP_TRY(hipMemcpy(dmap.p, m.data(), m.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&stage.p, count * rw * 8));
+    const int64_t *map = nullptr;
+    if (!ddr) {
+        std::vector<int64_t> m = bsk_map(p);
+        HIP_TRY(hipMalloc(&dmap.p, m.size() * 8));
+        HIP_TRY(hipMemcpy(dmap.p, m.data(), m.size() * 8, hipMemcpyHostToDevice));
+        map = (const int64_t *)dmap.p;
+    }
     hipError_t e = p.narrow
         ? launch_rgsw_to_ref<uint32_t>((const uint32_t *)k->bsk, rgsw_elems(p), (uint64_t *)stage.p, rw, p.digitG,
-                                       p.N, (const int64_t *)dmap.p, m.size(), c->stream)
+                                       p.N, map, count, c->stream, ddr)
         : launch_rgsw_to_ref<uint64_t>((const uint64_t *)k->bsk, rgsw_elems(p), (uint64_t *)stage.p, rw, p.digitG,
-                                       p.N, (const int64_t *)dmap.p, m.size(), c->stream);
+                                       p.N, map, count, c->stream, ddr);
     HIP_TRY(e);
     HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(bsk, stage.p, words * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(words, stage.p, count * rw * 8, hipMemcpyDeviceToHost));
     return 0;
+}
+
+int fheacc_keys_upload_bsk(fheacc_keys *k, const uint64_t *bsk) {
+    if (!k || !bsk) return fail("null argument");
+    return bsk_in(k, bsk, false);
+}
+
+int fheacc_keys_download_bsk(fheacc_keys *k, uint64_t *bsk) {
+    if (!k || !bsk || !k->bsk) return fail("no bootstrap key resident");
+    return bsk_out(k, bsk, false);
+}
+
+int fheacc_keys_import_fpga_bsk(fheacc_keys *k, const uint64_t *image) {
+    if (!k || !image) return fail("null argument");
+    return bsk_in(k, image, true);
+}
+
+int fheacc_keys_export_fpga_bsk(fheacc_keys *k, uint64_t *image) {
+    if (!k || !image || !k->bsk) return fail("no bootstrap key resident");
+    return bsk_out(k, image, true);
 }
 
 int fheacc_keys_upload_ksk(fheacc_keys *k, const uint64_t *ksk) {
@@ -469,6 +499,55 @@ int fheacc_keys_download_subs(fheacc_keys *k, uint64_t *subs) {
     if (!k || !subs || !k->subs) return fail("no subs key resident");
     HIP_TRY(hipSetDevice(k->ctx->device));
     return subs_download(k, subs);
+}
+
+// One RLWE (a then b, N words each) <-> the accelerator's buffer order: 4 words
+// of a, then 4 of b, N/4 times (convert_RLWE_to_buffer / convert_buffer_to_RLWE,
+// myfpga.h:201-243).
+static void rlwe_ddr_order(const uint64_t *ab, uint64_t *img, int N) {
+    for (int k = 0; k < N; k++) {
+        img[(k >> 2) * 8 + (k & 3)] = ab[k];
+        img[(k >> 2) * 8 + 4 + (k & 3)] = ab[N + k];
+    }
+}
+
+static void rlwe_from_ddr_order(const uint64_t *img, uint64_t *ab, int N) {
+    for (int k = 0; k < N; k++) {
+        ab[k] = img[(k >> 2) * 8 + (k & 3)];
+        ab[N + k] = img[(k >> 2) * 8 + 4 + (k & 3)];
+    }
+}
+
+// The subs key's DDR image (FPGA::dma_write_subs_key, myfpga.cpp:650-712):
+// subs_key[i].RLWE_kskey[l] for i < log2 N, l < digitKS_R, in that order.
+int fheacc_keys_export_fpga_subs(fheacc_keys *k, uint64_t *image) {
+    if (!k || !image || !k->subs) return fail("no subs key resident");
+    HIP_TRY(hipSetDevice(k->ctx->device));
+    const HostParams &p = k->ctx->p;
+    const size_t cnt = (size_t)p.logN * p.digitKS_R, rl = 2 * (size_t)p.N;
+    std::vector<uint64_t> ref;
+    try {
+        ref.resize(cnt * rl);
+    } catch (const std::bad_alloc &) {
+        return fail("out of host memory", FHEACC_ENOMEM);
+    }
+    if (int rc = subs_download(k, ref.data())) return rc;
+    for (size_t r = 0; r < cnt; r++) rlwe_ddr_order(ref.data() + r * rl, image + r * rl, p.N);
+    return 0;
+}
+
+int fheacc_keys_import_fpga_subs(fheacc_keys *k, const uint64_t *image) {
+    if (!k || !image) return fail("null argument");
+    const HostParams &p = k->ctx->p;
+    const size_t cnt = (size_t)p.logN * p.digitKS_R, rl = 2 * (size_t)p.N;
+    std::vector<uint64_t> ref;
+    try {
+        ref.resize(cnt * rl);
+    } catch (const std::bad_alloc &) {
+        return fail("out of host memory", FHEACC_ENOMEM);
+    }
+    for (size_t r = 0; r < cnt; r++) rlwe_from_ddr_order(image + r * rl, ref.data() + r * rl, p.N);
+    return fheacc_keys_upload_subs(k, ref.data());
 }
 
 template <typename W>
@@ -631,7 +710,7 @@ int fheacc_keys_digest(fheacc_keys *k, uint64_t *bd, uint64_t *kd) {
 
 namespace {
 constexpr uint64_t kKeyMagic = 0x314B434341454846ULL;  // "FHEACCK1"
-constexpr uint64_t kKeyVersion = 1;
+constexpr uint64_t kKeyVersion = 2;  // 2: the bootstrap key holds every residue, 0 included
 constexpr size_t kKeyChunk = 64 << 20;
 
 struct PinnedBuf {
@@ -841,6 +920,7 @@ static int gate_path_impl(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch 
     a.key_of = g.key_of;
     a.bsk_tab = (const W *const *)g.bsk_tab;
     a.ct_status = g.ct_status;
+    a.mode = c->eval_mode.load();
     const size_t lds = br_lds_bytes(p, a.pb);
     if (lds > c->lds_budget) return fail("parameter set does not fit the LDS budget");
     const int nt = B <= c->tune.small_max && c->tune.br_threads_small ? c->tune.br_threads_small : c->tune.br_threads;
@@ -1285,6 +1365,20 @@ int fheacc_ctx_get_tuning(const fheacc_ctx *c, int *threads, int *pb) {
     if (check_ctx(c) || !threads || !pb) return fail("null argument");
     *threads = c->tune.br_threads;
     *pb = c->tune.br_pb;
+    return 0;
+}
+
+int fheacc_ctx_set_eval_mode(fheacc_ctx *c, int mode) {
+    if (check_ctx(c)) return fail("null context");
+    if (mode != FHEACC_EVAL_SOFTWARE && mode != FHEACC_EVAL_FPGA_X1 && mode != FHEACC_EVAL_FPGA_XN)
+        return fail("unknown evaluation mode " + std::to_string(mode));
+    c->eval_mode.store(mode);
+    return 0;
+}
+
+int fheacc_ctx_get_eval_mode(const fheacc_ctx *c, int *mode) {
+    if (check_ctx(c) || !mode) return fail("null argument");
+    *mode = c->eval_mode.load();
     return 0;
 }
 

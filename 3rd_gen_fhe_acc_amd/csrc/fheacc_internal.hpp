@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -119,6 +120,9 @@ struct fheacc_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     int *status = nullptr;  // device status word (identical-operand flag)
+    // blind-rotation order, FHEACC_EVAL_*: read at every launch and capture,
+    // including by a serving loop's thread
+    std::atomic<int> eval_mode{FHEACC_EVAL_SOFTWARE};
     std::mutex mu;
     // profiling: (class, start, stop) event records while enabled
     bool profile = false;

@@ -5,9 +5,11 @@
 //        K0 = c_text[h][0] (multiplies the digits of a), K1 = c_text[h][1]
 //        (digits of b), RLWE.cpp:642-648.  Narrow: uint4 per (h, k).  Wide:
 //        each value followed by its Shoup companion floor(K * 2^64 / Q).
-//   bootstrap key  [n][digitR][Br-1] RGSW: the keys of one blind-rotation
-//        step (i, j) are contiguous; residue 0 is skipped by the path
-//        (RLWE.cpp:759) and not stored.
+//   bootstrap key  [n][digitR][Br] RGSW: the keys of one blind-rotation
+//        step (i, j) are contiguous.  Residue 0 (RGSW(1)) is stored as the
+//        reference's key holds it; the software evaluation skips it
+//        (RLWE.cpp:759), the FPGA evaluation order multiplies by it
+//        (myfpga.cpp:1037-1046,1190-1206).
 //   LWE ks key     [N][digitKS][Bks][n_pad] W: the Bks candidate rows of one
 //        key-switch step (coefficient i, digit j) are contiguous.
 //   subs key       [log2 N][digitKS_R][N] x {Ka, Kb} (+ companions if wide).
@@ -404,15 +406,19 @@ __global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
     }
     __syncthreads();
     ntt_forward<LOGN, NT>(acc + pad(N), 1, a.A, a.tw);
-    // blind rotation: i outer, j inner, zero residues skipped
+    // blind rotation: i outer, j inner.  Software order (RLWE.cpp:754-767)
+    // skips residue 0; the accelerator's one-ciphertext form multiplies at
+    // step (0, 0) whatever the residue, its batched forms at every step
+    // (myfpga.cpp:1030-1071, 1180-1225), with btkey[0] = RGSW(1).
     const W *bsk = a.key_of ? a.bsk_tab[a.key_of[c]] : a.bsk;
+    const int mode = a.mode;
     for (int i = 0; i < n; i++) {
         uint32_t nega = (q - av[i]) & qm;
         for (int j = 0; j < a.digitR; j++) {
             const uint32_t r = nega % a.Br;
             nega /= a.Br;
-            if (r) {
-                const W *key = bsk + ((size_t)(i * a.digitR + j) * (a.Br - 1) + (r - 1)) * a.rgsw_elems;
+            if (r || mode == FHEACC_EVAL_FPGA_XN || (mode == FHEACC_EVAL_FPGA_X1 && i == 0 && j == 0)) {
+                const W *key = bsk + ((size_t)(i * a.digitR + j) * a.Br + r) * a.rgsw_elems;
                 if constexpr (inplace) ext_prod_inplace<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
                 else ext_prod<W, LOGN, NT>(acc, dig, key, a.A, a.tw, a.d, a.bits, a.pb);
             }
@@ -792,10 +798,31 @@ __global__ __launch_bounds__(NT) void k_cmux_chain(CmuxArgs<W> a) {
 // key layout conversion and seeded generation
 // ---------------------------------------------------------------------------
 
-// reference RGSW [d][2][2][N] u64 -> device quad layout
+// Offsets, inside one RGSW of u64 words, of the four key words of gadget row h
+// and coefficient k (K0a, K0b, K1a, K1b).  Reference order [h][g][a, b][N]
+// (RLWE.h:313-329); or the accelerator's DDR order: RLWEs c_text[h][g] g-major
+// (myfpga.cpp:624-646), each as 4 words of a then 4 of b, N/4 times
+// (convert_RLWE_to_buffer, myfpga.h:201-223).
+__device__ __forceinline__ void rgsw_word_offsets(int h, int k, int d, int N, bool ddr, size_t o[4]) {
+    if (!ddr) {
+        const size_t b = (size_t)h * 4 * N + k;
+        o[0] = b;
+        o[1] = b + N;
+        o[2] = b + 2 * (size_t)N;
+        o[3] = b + 3 * (size_t)N;
+    } else {
+        const size_t in = (size_t)(k >> 2) * 8 + (k & 3);
+        o[0] = (size_t)h * 2 * N + in;
+        o[1] = o[0] + 4;
+        o[2] = (size_t)(d + h) * 2 * N + in;
+        o[3] = o[2] + 4;
+    }
+}
+
+// reference (or DDR-image) RGSW u64 words -> device quad layout
 template <typename W>
 __global__ void k_rgsw_to_device(const uint64_t *src, int64_t src_stride, W *dst, int64_t dst_stride,
-                                 int d, int N, ShoupGen sg, const int64_t *map, int64_t count) {
+                                 int d, int N, ShoupGen sg, const int64_t *map, int64_t count, bool ddr) {
     const int64_t e = blockIdx.y;
     if (e >= count) return;
     const int64_t se = map ? map[e] : e;
@@ -803,8 +830,9 @@ __global__ void k_rgsw_to_device(const uint64_t *src, int64_t src_stride, W *dst
     W *o = dst + e * dst_stride;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < d * N; idx += gridDim.x * blockDim.x) {
         const int h = idx / N, k = idx - h * N;
-        const uint64_t *r = s + (size_t)h * 4 * N;
-        uint64_t v0 = r[k], v1 = r[N + k], v2 = r[2 * N + k], v3 = r[3 * N + k];
+        size_t off[4];
+        rgsw_word_offsets(h, k, d, N, ddr, off);
+        uint64_t v0 = s[off[0]], v1 = s[off[1]], v2 = s[off[2]], v3 = s[off[3]];
         if (sizeof(W) == 4) {
             reinterpret_cast<uint4 *>(o)[idx] = make_uint4((uint32_t)v0, (uint32_t)v1, (uint32_t)v2, (uint32_t)v3);
         } else {
@@ -820,20 +848,21 @@ __global__ void k_rgsw_to_device(const uint64_t *src, int64_t src_stride, W *dst
 // device quad layout -> reference RGSW layout
 template <typename W>
 __global__ void k_rgsw_to_ref(const W *src, int64_t src_stride, uint64_t *dst, int64_t dst_stride, int d, int N,
-                              const int64_t *map, int64_t count) {
+                              const int64_t *map, int64_t count, bool ddr) {
     const int64_t e = blockIdx.y;
     if (e >= count) return;
     const W *s = src + e * src_stride;
     uint64_t *o = dst + (map ? map[e] : e) * dst_stride;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < d * N; idx += gridDim.x * blockDim.x) {
         const int h = idx / N, k = idx - h * N;
-        uint64_t *r = o + (size_t)h * 4 * N;
+        size_t off[4];
+        rgsw_word_offsets(h, k, d, N, ddr, off);
         if (sizeof(W) == 4) {
             uint4 q = reinterpret_cast<const uint4 *>(s)[idx];
-            r[k] = q.x; r[N + k] = q.y; r[2 * N + k] = q.z; r[3 * N + k] = q.w;
+            o[off[0]] = q.x; o[off[1]] = q.y; o[off[2]] = q.z; o[off[3]] = q.w;
         } else {
             const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(s) + 4 * (size_t)idx;
-            r[k] = p[0].x; r[N + k] = p[1].x; r[2 * N + k] = p[2].x; r[3 * N + k] = p[3].x;
+            o[off[0]] = p[0].x; o[off[1]] = p[1].x; o[off[2]] = p[2].x; o[off[3]] = p[3].x;
         }
     }
 }
@@ -908,9 +937,9 @@ __global__ __launch_bounds__(NT) void k_bsk_gen(KeygenArgs<W> a) {
     const int d = a.d;
     const int64_t wid = blockIdx.x;
     const int h = wid % d;
-    const int64_t dev_rgsw = wid / d;  // ((k * digitR + j) * (Br-1) + r - 1)
-    const int rr = dev_rgsw % (a.Br - 1) + 1;
-    const int64_t kj = dev_rgsw / (a.Br - 1);
+    const int64_t dev_rgsw = wid / d;  // (k * digitR + j) * Br + r
+    const int rr = dev_rgsw % a.Br;
+    const int64_t kj = dev_rgsw / a.Br;
     const int j = kj % a.digitR;
     const int k = kj / a.digitR;
     uint32_t powbr = 1;
@@ -1213,25 +1242,25 @@ hipError_t launch_cmux_chain(const CmuxArgs<W> &a, int logN, int nt, int64_t cou
 
 template <typename W>
 hipError_t launch_rgsw_to_device(const uint64_t *src, int64_t src_stride, W *dst, int64_t dst_stride, int d, int N,
-                                 ShoupGen sg, const int64_t *map, int64_t count, hipStream_t st) {
+                                 ShoupGen sg, const int64_t *map, int64_t count, hipStream_t st, bool ddr) {
     int bx = (d * N + 255) / 256;
     for (int64_t off = 0; off < count; off += 65535) {
         int64_t cnt = count - off < 65535 ? count - off : 65535;
         hipLaunchKernelGGL(k_rgsw_to_device<W>, dim3(bx, (unsigned)cnt), dim3(256), 0, st,
                            map ? src : src + off * src_stride, src_stride, dst + off * dst_stride, dst_stride, d, N,
-                           sg, map ? map + off : nullptr, cnt);
+                           sg, map ? map + off : nullptr, cnt, ddr);
     }
     return hipGetLastError();
 }
 
 template <typename W>
 hipError_t launch_rgsw_to_ref(const W *src, int64_t src_stride, uint64_t *dst, int64_t dst_stride, int d, int N,
-                              const int64_t *map, int64_t count, hipStream_t st) {
+                              const int64_t *map, int64_t count, hipStream_t st, bool ddr) {
     int bx = (d * N + 255) / 256;
     for (int64_t off = 0; off < count; off += 65535) {
         int64_t cnt = count - off < 65535 ? count - off : 65535;
         hipLaunchKernelGGL(k_rgsw_to_ref<W>, dim3(bx, (unsigned)cnt), dim3(256), 0, st, src + off * src_stride,
-                           src_stride, dst, dst_stride, d, N, map ? map + off : nullptr, cnt);
+                           src_stride, dst, dst_stride, d, N, map ? map + off : nullptr, cnt, ddr);
     }
     return hipGetLastError();
 }
@@ -1260,7 +1289,7 @@ hipError_t launch_keygen(const KeygenArgs<W> &a, const KskGenArgs<W> &ka, int lo
             auto k = k_bsk_gen<W, LOGN, 256>;
             hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
-            const int64_t wgs = (int64_t)a.n * a.digitR * (a.Br - 1) * a.d;
+            const int64_t wgs = (int64_t)a.n * a.digitR * a.Br * a.d;
             hipLaunchKernelGGL(k, dim3((unsigned)wgs), dim3(256), lds, st, a);
             hipLaunchKernelGGL(k_ksk_gen<W>, dim3((unsigned)((ka.rows + 3) / 4)), dim3(256), 0, st, ka);
         }
@@ -1292,9 +1321,9 @@ hipError_t launch_digest(const uint64_t *w, int64_t count, uint64_t *partials, i
     template hipError_t launch_expand_level<W>(const SubsArgs<W> &, int, int64_t, hipStream_t);                   \
     template hipError_t launch_cmux_chain<W>(const CmuxArgs<W> &, int, int, int64_t, size_t, hipStream_t);        \
     template hipError_t launch_rgsw_to_device<W>(const uint64_t *, int64_t, W *, int64_t, int, int, ShoupGen,     \
-                                                 const int64_t *, int64_t, hipStream_t);                          \
+                                                 const int64_t *, int64_t, hipStream_t, bool);                    \
     template hipError_t launch_rgsw_to_ref<W>(const W *, int64_t, uint64_t *, int64_t, int, int, const int64_t *, \
-                                              int64_t, hipStream_t);                                              \
+                                              int64_t, hipStream_t, bool);                                        \
     template hipError_t launch_ksk_convert<W>(const uint64_t *, W *, uint64_t *, int, int, int, int, int,         \
                                               hipStream_t);                                                       \
     template hipError_t launch_keygen<W>(const KeygenArgs<W> &, const KskGenArgs<W> &, int, const uint64_t *,     \

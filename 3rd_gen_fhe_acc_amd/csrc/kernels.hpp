@@ -38,6 +38,7 @@ struct BRArgs {
     const int32_t *key_of;
     const W *const *bsk_tab;
     int *ct_status;
+    int mode;  // FHEACC_EVAL_*: which residue-0 steps multiply by RGSW(1)
 };
 
 template <typename W>
@@ -171,12 +172,14 @@ template <typename W>
 hipError_t launch_expand_level(const SubsArgs<W> &a, int logN, int64_t B, hipStream_t st);
 template <typename W>
 hipError_t launch_cmux_chain(const CmuxArgs<W> &a, int logN, int nt, int64_t count, size_t lds, hipStream_t st);
+// RGSW words between the reference's order (ddr = false) or the accelerator's
+// DDR image order (ddr = true) and the device quad layout
 template <typename W>
 hipError_t launch_rgsw_to_device(const uint64_t *src, int64_t src_stride, W *dst, int64_t dst_stride, int d, int N,
-                                 ShoupGen sg, const int64_t *map, int64_t count, hipStream_t st);
+                                 ShoupGen sg, const int64_t *map, int64_t count, hipStream_t st, bool ddr = false);
 template <typename W>
 hipError_t launch_rgsw_to_ref(const W *src, int64_t src_stride, uint64_t *dst, int64_t dst_stride, int d, int N,
-                              const int64_t *map, int64_t count, hipStream_t st);
+                              const int64_t *map, int64_t count, hipStream_t st, bool ddr = false);
 template <typename W>
 hipError_t launch_ksk_convert(const uint64_t *src, W *dst, uint64_t *back, int Bks, int dks, int N, int n1,
                               int n_pad, hipStream_t st);

@@ -4,6 +4,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <mutex>
+
 namespace fheacc {
 
 void check(int rc) {
@@ -245,3 +247,74 @@ std::vector<LWE::LWE_ciphertext> eval_bootstrap_batch(const std::vector<LWE::LWE
 }
 
 }  // namespace fheacc
+
+namespace FPGA {
+
+namespace {
+
+std::mutex order_mu;  // one accelerator-order call at a time changes a context's order
+
+std::vector<LWE::LWE_ciphertext> in_order(const std::vector<LWE::LWE_ciphertext> &c1,
+                                          const std::vector<LWE::LWE_ciphertext> &c2, const LWE::keyswitch_key &kskey,
+                                          const std::vector<GATES> &gate, size_t want, int mode, const char *name,
+                                          int *rc) {
+    *rc = 0;
+    if (c1.size() != want || c2.size() != want || gate.size() != want) {
+        fprintf(stderr, "%s: input vectors must hold %zu ciphertexts\n", name, want);
+        *rc = -1;
+        return std::vector<LWE::LWE_ciphertext>(want);
+    }
+    if (!kskey.keys) {
+        fprintf(stderr, "%s: no key set\n", name);
+        *rc = -1;
+        return std::vector<LWE::LWE_ciphertext>(want);
+    }
+    fheacc_ctx *h = kskey.keys->context().handle();
+    std::lock_guard<std::mutex> lock(order_mu);
+    int prev = FHEACC_EVAL_SOFTWARE;
+    if (fheacc_ctx_get_eval_mode(h, &prev) != 0 || fheacc_ctx_set_eval_mode(h, mode) != 0) {
+        fprintf(stderr, "%s: %s\n", name, fheacc_last_error());
+        *rc = -1;
+        return std::vector<LWE::LWE_ciphertext>(want);
+    }
+    std::vector<LWE::LWE_ciphertext> out = fheacc::eval_bootstrap_batch(c1, c2, kskey, gate, rc);
+    fheacc_ctx_set_eval_mode(h, prev);
+    return out;
+}
+
+}  // namespace
+
+LWE::LWE_ciphertext fpga_eval_bootstrap_x1(const LWE::LWE_ciphertext &c1, const LWE::LWE_ciphertext &c2,
+                                           const LWE::keyswitch_key &kskey, const GATES gate, int *rc) {
+    return in_order({c1}, {c2}, kskey, {gate}, 1, FHEACC_EVAL_FPGA_X1, "fpga_eval_bootstrap_x1", rc)[0];
+}
+
+std::vector<LWE::LWE_ciphertext> fpga_eval_bootstrap_x4(const std::vector<LWE::LWE_ciphertext> &c1,
+                                                        const std::vector<LWE::LWE_ciphertext> &c2,
+                                                        const LWE::keyswitch_key &kskey,
+                                                        const std::vector<GATES> gate, int *rc) {
+    return in_order(c1, c2, kskey, gate, 4, FHEACC_EVAL_FPGA_XN, "fpga_eval_bootstrap_x4", rc);
+}
+
+std::vector<LWE::LWE_ciphertext> fpga_eval_bootstrap_x8(const std::vector<LWE::LWE_ciphertext> &c1,
+                                                        const std::vector<LWE::LWE_ciphertext> &c2,
+                                                        const LWE::keyswitch_key &kskey,
+                                                        const std::vector<GATES> gate, int *rc) {
+    return in_order(c1, c2, kskey, gate, 8, FHEACC_EVAL_FPGA_XN, "fpga_eval_bootstrap_x8", rc);
+}
+
+std::vector<LWE::LWE_ciphertext> fpga_eval_bootstrap_x12(const std::vector<LWE::LWE_ciphertext> &c1,
+                                                         const std::vector<LWE::LWE_ciphertext> &c2,
+                                                         const LWE::keyswitch_key &kskey,
+                                                         const std::vector<GATES> gate, int *rc) {
+    return in_order(c1, c2, kskey, gate, 12, FHEACC_EVAL_FPGA_XN, "fpga_eval_bootstrap_x12", rc);
+}
+
+std::vector<LWE::LWE_ciphertext> fpga_eval_bootstrap_x16(const std::vector<LWE::LWE_ciphertext> &c1,
+                                                         const std::vector<LWE::LWE_ciphertext> &c2,
+                                                         const LWE::keyswitch_key &kskey,
+                                                         const std::vector<GATES> gate, int *rc) {
+    return in_order(c1, c2, kskey, gate, 10, FHEACC_EVAL_FPGA_XN, "fpga_eval_bootstrap_x16", rc);
+}
+
+}  // namespace FPGA

@@ -14,6 +14,8 @@
 //   LWE::LWE_evaluate / LWE_eval_NOT            same names (host, LWE.cpp:203-231)
 //   RLWE::eval_bootstrap(c1, c2, bt, ks, gate)  same name and arguments
 //   FPGA::fpga_eval_bootstrap_x{4,8,12,16}      fheacc::eval_bootstrap_batch (any B, int* rc)
+//   FPGA::fpga_eval_bootstrap_x1..x16 (bar, fd) FPGA::fpga_eval_bootstrap_x1..x16 (no bar/fd),
+//                                               the accelerator's step order and batch sizes
 //   RLWE::RLWE_ciphertext / RGSW_ciphertext     same members (a, b, NTT_form; c_text[h][g])
 //   RLWE::RLWE_mult_RGSW(c, C)                  RLWE::RLWE_mult_RGSW(ctx, c, C)
 //   RLWE::CMUX / blind_rotate / blind_rotate_full  same names, ctx first
@@ -224,9 +226,46 @@ std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> expand_RLWE(const RLWE::RLWE
 namespace fheacc {
 
 // FPGA::fpga_eval_bootstrap_x{4,8,12,16} with any batch size; *rc = 0, -1, -ENOMEM.
+// Evaluates in the context's order (fheacc_ctx_set_eval_mode; by default the
+// software order of RLWE::eval_bootstrap).
 std::vector<LWE::LWE_ciphertext> eval_bootstrap_batch(const std::vector<LWE::LWE_ciphertext> &c1,
                                                       const std::vector<LWE::LWE_ciphertext> &c2,
                                                       const LWE::keyswitch_key &kskey, const std::vector<GATES> &gates,
                                                       int *rc);
 
 }  // namespace fheacc
+
+namespace FPGA {
+
+// The accelerator's gate calls (myfpga.h:141-401) with the card's BAR handle
+// and DMA descriptor dropped: the GPU context is the one kskey's device key
+// set belongs to.  Each evaluates in the card's step order, which differs
+// from RLWE::eval_bootstrap in noise and therefore in words:
+//   x1  the fused BOOTSTRAP_INIT at step (0, 0) multiplies by btkey[r][0][0]
+//       even for r = 0, later residue-0 steps are skipped (myfpga.cpp:1003-1127);
+//   xN  every step multiplies, by btkey[0] = RGSW(1) for residue 0
+//       (myfpga.cpp:1130-1660).
+// Batch sizes are the reference's: 4, 8, 12, and 10 for x16 (myfpga.cpp:
+// 1140,1282,1408,1541); other sizes give *rc = -1.  The order is set on the
+// shared context for the call and restored afterwards: other gate
+// bootstraps on the same context must not run concurrently with these.
+LWE::LWE_ciphertext fpga_eval_bootstrap_x1(const LWE::LWE_ciphertext &c1, const LWE::LWE_ciphertext &c2,
+                                           const LWE::keyswitch_key &kskey, const GATES gate, int *rc);
+std::vector<LWE::LWE_ciphertext> fpga_eval_bootstrap_x4(const std::vector<LWE::LWE_ciphertext> &c1,
+                                                        const std::vector<LWE::LWE_ciphertext> &c2,
+                                                        const LWE::keyswitch_key &kskey,
+                                                        const std::vector<GATES> gate, int *rc);
+std::vector<LWE::LWE_ciphertext> fpga_eval_bootstrap_x8(const std::vector<LWE::LWE_ciphertext> &c1,
+                                                        const std::vector<LWE::LWE_ciphertext> &c2,
+                                                        const LWE::keyswitch_key &kskey,
+                                                        const std::vector<GATES> gate, int *rc);
+std::vector<LWE::LWE_ciphertext> fpga_eval_bootstrap_x12(const std::vector<LWE::LWE_ciphertext> &c1,
+                                                         const std::vector<LWE::LWE_ciphertext> &c2,
+                                                         const LWE::keyswitch_key &kskey,
+                                                         const std::vector<GATES> gate, int *rc);
+std::vector<LWE::LWE_ciphertext> fpga_eval_bootstrap_x16(const std::vector<LWE::LWE_ciphertext> &c1,
+                                                         const std::vector<LWE::LWE_ciphertext> &c2,
+                                                         const LWE::keyswitch_key &kskey,
+                                                         const std::vector<GATES> gate, int *rc);
+
+}  // namespace FPGA

@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--serve-min-fill", type=int, default=0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-parameter-set batch")
     ap.add_argument("--no-leveled", action="store_true", help="skip the RLWE expansion / blind-rotate microbench")
+    ap.add_argument("--no-fpga-order", action="store_true",
+                    help="skip timing the batch in the accelerator's step orders")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (reference dispatch for HBM byte counters)")
     return ap.parse_args()
@@ -316,6 +318,38 @@ def leveled(fhe, torch, B=8, reps=3):
     }
 
 
+def fpga_orders(fhe, torch, ctx, keys, dg, d1, d2, nega, s, want, reps=3):
+    """The same batch in the accelerator's step orders (fpga_eval_bootstrap_x1
+    and _x4..x16, myfpga.cpp:1003-1660): x1 adds a product by btkey[0][0][0]
+    when step (0, 0) has residue 0, the batched forms one by RGSW(1) at every
+    residue-0 step."""
+    Br, dR = PARAMS["Br"], ctx.digitR
+    digits = np.stack([(nega // Br ** j) % Br for j in range(dR)], axis=-1)  # [B][n][digitR]
+    soft = np.count_nonzero(digits, axis=(1, 2))
+    counts = {"software": soft, "x1": soft + (digits[:, 0, 0] == 0), "xn": np.full(soft.shape, ctx.n * dR)}
+    res = {"workload": "the headline batch (4096 NAND gates, benchmark parameters), eager launches"}
+    out = torch.zeros_like(d1)
+    for name, mode in (("x1", fhe.EVAL_FPGA_X1), ("xn", fhe.EVAL_FPGA_XN)):
+        ctx.eval_mode = mode
+        ctx.bootstrap_dev(keys, dg, d1, d2, out)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            ctx.bootstrap_dev(keys, dg, d1, d2, out)
+        e1.record()
+        torch.cuda.synchronize()
+        ctx.sync()
+        ms = e0.elapsed_time(e1) / reps
+        bits = ctx.lwe_decrypt(s, out.cpu().numpy().view(np.uint64))
+        res[name] = {"ms_per_batch": round(ms, 3), "bootstraps_per_s": round(d1.shape[0] / (ms / 1e3), 1),
+                     "external_products_per_bootstrap": round(float(counts[name].mean()), 2),
+                     "decrypt_ok": bool(np.array_equal(bits, want))}
+    ctx.eval_mode = fhe.EVAL_SOFTWARE
+    res["software_external_products_per_bootstrap"] = round(float(counts["software"].mean()), 2)
+    return res
+
+
 def cpu_baseline(ctx, keys, c1, c2, gates, gpu_out, sample):
     """Time the CPU path on `sample` gates (single thread), bit-compare with the GPU."""
     import oracle
@@ -465,7 +499,11 @@ def main():
     ks_ms, ks_n = kms["lwe_keyswitch"]
     br_avg = br_ms / max(br_n, 1)
     ks_avg = ks_ms / max(ks_n, 1)
-    br_bytes = info["bsk_device_bytes"] + 2 * B * (n + 1) * 8 + B * 4 + B * N * dks + B * 8
+    # the key bytes a batch touches: every nonzero residue (residue 0 is
+    # stored but the software order never reads it)
+    Br = PARAMS["Br"]
+    touched_bsk = info["bsk_device_bytes"] // Br * (Br - 1)
+    br_bytes = touched_bsk + 2 * B * (n + 1) * 8 + B * 4 + B * N * dks + B * 8
     achieved = br_bytes / (br_avg * 1e-3) / 1e9
     nega = (PARAMS["q"] - ((c1[:, :n] + c2[:, :n]) % PARAMS["q"])) % PARAMS["q"]
     ext_products = int(np.count_nonzero(nega % 32) + np.count_nonzero((nega // 32) % 32))
@@ -537,6 +575,8 @@ def main():
                              "request_p99": rec["serving"]["request_latency_ms_p99"],
                              "note": "gate ready -> done and request submit -> result, serving loop under "
                                      "Poisson load; idle one-per-launch figures: single_per_launch"}
+    if not args.no_fpga_order:
+        rec["fpga_order"] = fpga_orders(fhe, torch, ctx, keys, dg, d1, d2, nega, s, 1 - (m1 & m2))
     if not args.no_microbench:
         rec["kernel_microbench"] = microbench(fhe, torch, B)
     if not args.no_mixed:

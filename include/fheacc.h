@@ -105,6 +105,26 @@ int fheacc_ctx_get_tuning(const fheacc_ctx *ctx, int *br_threads, int *br_polys_
  * and writes a JSON report of all candidates to report (may be NULL). */
 int fheacc_autotune(fheacc_ctx *ctx, fheacc_keys *keys, int64_t B, char *report, size_t report_len);
 
+/* Evaluation order of the blind rotation for every later gate bootstrap on
+ * the context (batches, graphs captured afterwards, the serving loop):
+ *   FHEACC_EVAL_SOFTWARE  residue-0 steps skipped: RLWE::eval_bootstrap
+ *                         (RLWE.cpp:754-767).  The default.
+ *   FHEACC_EVAL_FPGA_X1   the accelerator's one-ciphertext form: step (0,0)
+ *                         is the fused BOOTSTRAP_INIT and multiplies by
+ *                         btkey[r][0][0] even for r = 0; later residue-0
+ *                         steps are skipped (fpga_eval_bootstrap_x1,
+ *                         myfpga.cpp:1030-1071).
+ *   FHEACC_EVAL_FPGA_XN   the batched forms: every step multiplies, by
+ *                         btkey[0] = RGSW(1) when the residue is 0
+ *                         (fpga_eval_bootstrap_x4..x16, myfpga.cpp:1180-1225).
+ * The FPGA orders add noise terms, so their words differ from the software
+ * order's; each is bit-exact with its own order. */
+#define FHEACC_EVAL_SOFTWARE 0
+#define FHEACC_EVAL_FPGA_X1 1
+#define FHEACC_EVAL_FPGA_XN 2
+int fheacc_ctx_set_eval_mode(fheacc_ctx *ctx, int mode);
+int fheacc_ctx_get_eval_mode(const fheacc_ctx *ctx, int *mode);
+
 /* ---- keys (device resident, one set per client) -----------------------
  * Replaces RLWE::bootstrap_key / LWE::keyswitch_key /
  * application::substitute_key construction (RLWE.cpp:579-606,
@@ -112,15 +132,29 @@ int fheacc_autotune(fheacc_ctx *ctx, fheacc_keys *keys, int64_t B, char *report,
  * (FPGA::dma_write_bootstrap_key, dma_write_subs_key, myfpga.cpp:596-712). */
 int fheacc_keys_create(fheacc_ctx *ctx, fheacc_keys **out);
 int fheacc_keys_destroy(fheacc_keys *keys);
-/* Host arrays in the reference layouts above.  btkey[0][*][*] (residue 0) is
- * never read by the gate path and is not stored. */
+/* Host arrays in the reference layouts above.  Every residue is stored,
+ * btkey[0] (RGSW(1)) included: the FPGA evaluation orders read it. */
 int fheacc_keys_upload_bsk(fheacc_keys *keys, const uint64_t *bsk);
 int fheacc_keys_upload_ksk(fheacc_keys *keys, const uint64_t *ksk);
 int fheacc_keys_upload_subs(fheacc_keys *keys, const uint64_t *subs);
-/* Read back in the reference layouts (residue-0 RGSWs are written as zero). */
+/* Read back in the reference layouts. */
 int fheacc_keys_download_bsk(fheacc_keys *keys, uint64_t *bsk);
 int fheacc_keys_download_ksk(fheacc_keys *keys, uint64_t *ksk);
 int fheacc_keys_download_subs(fheacc_keys *keys, uint64_t *subs);
+/* The accelerator's DDR key images, word for word what
+ * FPGA::dma_write_bootstrap_key / dma_write_subs_key DMA to the card from
+ * DDR_ADDR (myfpga.cpp:596-712, myfpga.h:122-125).  Bootstrap key: for
+ * i < n, j < digitR, r < Br, g < 2, h < digitG the RLWE
+ * btkey[r][j][i].c_text[h][g] (bsk_words in all).  Subs key: for
+ * i < log2 N, l < digitKS_R the RLWE subs_key[i].RLWE_kskey[l] (subs_words).
+ * Each RLWE is in the card's buffer order, 4 words of a then 4 of b, N/4
+ * times (convert_RLWE_to_buffer, myfpga.h:201-223).  Export writes an image;
+ * import makes the key resident from one (a key read back from a card with
+ * dma_read_compare_bootstrap_key's reads, for instance). */
+int fheacc_keys_export_fpga_bsk(fheacc_keys *keys, uint64_t *image);
+int fheacc_keys_import_fpga_bsk(fheacc_keys *keys, const uint64_t *image);
+int fheacc_keys_export_fpga_subs(fheacc_keys *keys, uint64_t *image);
+int fheacc_keys_import_fpga_subs(fheacc_keys *keys, const uint64_t *image);
 /* Seeded generation on the device, straight into the device layouts.
  * lwe_sk: [n] mod q; rlwe_sk: [N] mod Q, time domain (host arrays). */
 int fheacc_keys_generate(fheacc_keys *keys, uint64_t seed, const uint64_t *lwe_sk, const uint64_t *rlwe_sk);

@@ -29,7 +29,12 @@ class Params(C.Structure):
         ("E", C.c_uint64), ("E_rlwe", C.c_uint64), ("stddev", C.c_double),
         ("gate_constant", C.c_uint64 * 6), ("rou", C.POINTER(C.c_uint64)),
         ("irou", C.POINTER(C.c_uint64)), ("cdt", C.c_uint64 * 64), ("cdt_len", C.c_int32),
+        ("eval_mode", C.c_int32),
     ]
+
+
+# blind-rotation orders (fhe_oracle.h ORC_EVAL_*)
+EVAL_SOFTWARE, EVAL_FPGA_X1, EVAL_FPGA_XN = 0, 1, 2
 
 
 def build(force=False):
@@ -136,6 +141,17 @@ class Oracle:
     @property
     def ref(self):
         return C.byref(self.p)
+
+    @property
+    def eval_mode(self):
+        """Blind-rotation order used by blind_rotate / bootstrap (EVAL_*)."""
+        return self.p.eval_mode
+
+    @eval_mode.setter
+    def eval_mode(self, mode):
+        if mode not in (EVAL_SOFTWARE, EVAL_FPGA_X1, EVAL_FPGA_XN):
+            raise ValueError("unknown evaluation mode %r" % (mode,))
+        self.p.eval_mode = mode
 
     def rou_table(self):
         return np.ctypeslib.as_array(self.p.rou, shape=(self.N,)).copy()

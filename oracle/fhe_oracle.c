@@ -581,7 +581,10 @@ uint64_t orc_lwe_decrypt(const orc_params *p, const uint64_t *sk, const uint64_t
 
 static size_t rgsw_words(const orc_params *p) { return (size_t)p->digitG * 4 * p->N; }
 
-/* RLWE.cpp:754-767: i outer, j inner, zero residues skipped */
+/* RLWE.cpp:754-767: i outer, j inner, zero residues skipped.  The FPGA
+ * orders instead multiply by btkey[0][j][i] (an RGSW of 1) at step (0,0)
+ * (x1: the fused BOOTSTRAP_INIT, myfpga.cpp:1036-1046) or at every
+ * residue-0 step (x4..x16, myfpga.cpp:1190-1206). */
 int64_t orc_blind_rotate(const orc_params *p, const uint64_t *lwe, const uint64_t *bsk, uint64_t *acc) {
     int N = p->N, n = p->n;
     uint64_t q = p->q, Br = p->Br;
@@ -592,7 +595,9 @@ int64_t orc_blind_rotate(const orc_params *p, const uint64_t *lwe, const uint64_
         uint64_t nega = (q - lwe[i]) % q;
         for (uint32_t j = 0; j < p->digitR; j++) {
             uint64_t r = nega % Br;
-            if (r > 0) {
+            int step = r > 0 || p->eval_mode == ORC_EVAL_FPGA_XN ||
+                       (p->eval_mode == ORC_EVAL_FPGA_X1 && i == 0 && j == 0);
+            if (step) {
                 const uint64_t *key = bsk + (((size_t)r * p->digitR + j) * n + i) * rw;
                 orc_rlwe_mult_rgsw(p, acc, key, tmp);
                 memcpy(acc, tmp, sizeof(uint64_t) * 2 * N);

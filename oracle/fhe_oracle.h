@@ -57,7 +57,17 @@ typedef struct orc_params {
     uint64_t *irou;      /* [N] psi^-i at bitrev(i) */
     uint64_t cdt[64];    /* Gaussian magnitude table, scaled to 2^63 */
     int32_t  cdt_len;
+    int32_t  eval_mode;  /* blind-rotation order, ORC_EVAL_* (0 after init) */
 } orc_params;
+
+/* Blind-rotation orders: the software loop skips residue-0 steps
+ * (RLWE.cpp:754-767); the accelerator's one-ciphertext form multiplies at step
+ * (0,0) whatever the residue (myfpga.cpp:1030-1071); its batched forms
+ * multiply at every step, by btkey[0] = RGSW(1) for residue 0
+ * (myfpga.cpp:1180-1225). */
+#define ORC_EVAL_SOFTWARE 0
+#define ORC_EVAL_FPGA_X1 1
+#define ORC_EVAL_FPGA_XN 2
 
 /* ---------- number theory (nbt.cpp) ---------- */
 uint64_t orc_reverse_bit(uint64_t order, uint64_t input);
@@ -123,8 +133,8 @@ void orc_lwe_modswitch(const orc_params *p, const uint64_t *in, uint64_t *out);
 uint64_t orc_lwe_decrypt(const orc_params *p, const uint64_t *sk, const uint64_t *c);
 
 /* ---------- gate bootstrap (RLWE.cpp:726-786) ---------- */
-/* Blind rotation only (acc in NTT domain after the loop).  Returns the number
- * of external products applied. */
+/* Blind rotation only (acc in NTT domain after the loop), in the order
+ * p->eval_mode names.  Returns the number of external products applied. */
 int64_t orc_blind_rotate(const orc_params *p, const uint64_t *lwe, const uint64_t *bsk, uint64_t *acc);
 /* Full gate; any capture pointer may be NULL.  Returns 0, or -1 when c1 == c2
  * (the reference exits the process there, RLWE.cpp:727-730). */

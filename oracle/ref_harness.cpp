@@ -214,6 +214,42 @@ double ref_eval_bootstrap(const uint64_t *c1, const uint64_t *c2, const int32_t 
     return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// The accelerator's gate orders (FPGA::fpga_eval_bootstrap_x1 and _x4..x16,
+// myfpga.cpp:1003-1660), driven through the reference's own CPU routines:
+// LWE_evaluate, acc_initialization, RLWE_mult_RGSW, LWE_keyswitch and
+// LWE_modswitch.  The card's BOOTSTRAP_INIT writes the test polynomial where
+// the external product's first inverse NTT would write
+// (cl_fhe_acc/design/src/iNTT_module.sv:1154-1260), which equals
+// acc_initialization followed by RLWE_mult_RGSW.  mode 1: step (0,0) always
+// multiplies, later residue-0 steps are skipped (myfpga.cpp:1030-1071);
+// mode 2: every step multiplies, by btkey[0] for residue 0
+// (myfpga.cpp:1180-1225).  cap_acc may be null.  Returns -1 for another mode.
+int ref_eval_bootstrap_fpga(const uint64_t *c1, const uint64_t *c2, const int32_t *gates, int64_t count, int mode,
+                            uint64_t *out, uint64_t *cap_acc) {
+    if (mode != 1 && mode != 2) return -1;
+    size_t row = (size_t)n + 1;
+    for (int64_t c = 0; c < count; c++) {
+        LWE::LWE_ciphertext a, b;
+        fill_lwe(a, c1 + c * row);
+        fill_lwe(b, c2 + c * row);
+        LWE::LWE_ciphertext t = LWE::LWE_evaluate(a, b, (GATES)gates[c]);
+        ciphertext_t bound1 = mod_pow2(gate_constant[gates[c]], q);
+        ciphertext_t bound2 = mod_pow2(bound1 + q2, q);
+        RLWE::RLWE_ciphertext acc = RLWE::acc_initialization(2 * N / q, bound1, bound2, t.b);
+        for (int i = 0; i < n; i++) {
+            ciphertext_t nega = (q - t.a[i]) % q;
+            for (uint64_t j = 0; j < digitR; j++) {
+                ciphertext_t r = nega % Br;
+                if (r > 0 || mode == 2 || (i == 0 && j == 0)) acc = RLWE::RLWE_mult_RGSW(acc, g_bsk->btkey[r][j][i]);
+                nega /= Br;
+            }
+        }
+        if (cap_acc) dump_rlwe(acc, cap_acc + c * 2 * N);
+        dump_lwe(LWE::LWE_modswitch(LWE::LWE_keyswitch(acc, *g_ksk)), out + c * row);
+    }
+    return 0;
+}
+
 void ref_substitute(const uint64_t *in, int f, uint64_t *out) {
     RLWE::RLWE_ciphertext a;
     fill_rlwe(a, in);

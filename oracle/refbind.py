@@ -38,6 +38,7 @@ def lib():
             "ref_load_keys": (None, [U, U]),
             "ref_lwe_keyswitch": (None, [U, U]),
             "ref_eval_bootstrap": (C.c_double, [U, U, I32, C.c_int64, U]),
+            "ref_eval_bootstrap_fpga": (C.c_int, [U, U, I32, C.c_int64, C.c_int, U, U]),
             "ref_substitute": (None, [U, C.c_int, U]),
             "ref_rlwe_keyswitch": (None, [U, U, U]),
             "ref_expand_rlwe": (None, [U, U, U]),
@@ -126,6 +127,21 @@ class Reference:
         out = np.zeros((B, self.n + 1), dtype=np.uint64)
         secs = lib().ref_eval_bootstrap(_p(c1), _p(c2), g.ctypes.data_as(C.POINTER(C.c_int32)), B, _p(out))
         return out, secs
+
+    def bootstrap_fpga(self, c1, c2, gates, mode):
+        """The accelerator's x1 (mode 1) or x4..x16 (mode 2) step order over the
+        reference's own routines; returns (out, acc)."""
+        c1 = np.ascontiguousarray(c1, dtype=np.uint64).reshape(-1, self.n + 1)
+        c2 = np.ascontiguousarray(c2, dtype=np.uint64).reshape(-1, self.n + 1)
+        B = c1.shape[0]
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(gates, dtype=np.int32), (B,)))
+        out = np.zeros((B, self.n + 1), dtype=np.uint64)
+        acc = np.zeros((B, 2, self.N), dtype=np.uint64)
+        rc = lib().ref_eval_bootstrap_fpga(_p(c1), _p(c2), g.ctypes.data_as(C.POINTER(C.c_int32)), B, mode, _p(out),
+                                           _p(acc))
+        if rc:
+            raise ValueError("mode must be 1 or 2")
+        return out, acc
 
     def substitute(self, rlwe, f):
         rlwe = np.ascontiguousarray(rlwe, dtype=np.uint64)
