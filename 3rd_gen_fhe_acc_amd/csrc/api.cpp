@@ -878,9 +878,34 @@ int fheacc_lwe_decrypt(const fheacc_ctx *c, const uint64_t *sk, const uint64_t *
 
 // ---------------------------------------------------------------- gate path
 
+// Step ranges of the gather key switch for a batch of B: about 1024
+// workgroups in all (4 per CU), at least 64 steps per range, at most 256
+// ranges.  Large batches fill the chip with one range per ciphertext.
+static int ks_gather_split(const HostParams &p, int64_t B) {
+    const int64_t cb = (p.n + 1 + 255) / 256;
+    const int64_t S = (int64_t)p.N * p.digitKS;
+    int64_t split = 1024 / std::max<int64_t>(1, cb * B);
+    split = std::min<int64_t>(split, std::min<int64_t>(256, S / 64));
+    return (int)std::max<int64_t>(split, 1);
+}
+
+// partial-sum rows the gather key switch may need for any launch of at most
+// B ciphertexts (a serving-loop segment can be smaller than its launch)
+static int64_t ks_part_rows(const HostParams &p, int64_t B) {
+    int64_t rows = 0;
+    for (int64_t b = 1; b <= std::min<int64_t>(B, 1024); b++) {
+        const int split = ks_gather_split(p, b);
+        if (split > 1) rows = std::max(rows, b * split);
+    }
+    return rows;
+}
+
+// Scratch of one gate-path launch: the key-switch digits u8[B][N][digitKS],
+// the key-switch b words u64[B], and the gather key switch's partial sums.
 size_t fheacc::gate_scratch_bytes(const HostParams &p, int64_t B) {
     const size_t ksd_bytes = (size_t)B * p.N * p.digitKS;
-    return (ksd_bytes + 255) / 256 * 256 + (size_t)B * 8;
+    const size_t ksb_end = (ksd_bytes + 255) / 256 * 256 + (size_t)B * 8;
+    return (ksb_end + 255) / 256 * 256 + (size_t)ks_part_rows(p, B) * (p.n + 1) * 8;
 }
 
 template <typename W>
@@ -891,6 +916,7 @@ static int gate_path_impl(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch 
     const size_t ksd_al = (ksd_bytes + 255) / 256 * 256;
     uint8_t *ksd = (uint8_t *)g.scratch;
     uint64_t *ksb = (uint64_t *)((char *)g.scratch + ksd_al);
+    uint64_t *part = (uint64_t *)((char *)g.scratch + (ksd_al + (size_t)B * 8 + 255) / 256 * 256);
     BRArgs<W> a{};
     a.A = make_arith<W>(p);
     a.tw = make_tw<W>(c);
@@ -954,6 +980,8 @@ static int gate_path_impl(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch 
         s.cap_ks = g.cap && g.cap->ks ? g.cap->ks + o0 * (p.n + 1) : nullptr;
         s.B = o1 - o0;
         const bool gather = s.B <= c->tune.ks_gather_max || ks_staged_lds<W>(s.Bks) > c->lds_budget;
+        s.split = gather ? ks_gather_split(p, s.B) : 1;
+        s.part = part;  // segments run in order on one stream and reuse it
         Prof pr(c, 1, g.stream, g.profile);
         HIP_TRY(launch_lwe_keyswitch<W>(s, gather, g.stream));
     }

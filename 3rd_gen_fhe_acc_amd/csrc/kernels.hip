@@ -561,17 +561,47 @@ __global__ __launch_bounds__(NT) void k_lwe_keyswitch_staged(KSArgs<W> a) {
     }
 }
 
-// Gather form for small batches: each ciphertext reads only its N*digitKS rows.
+// Gather form for small batches: each ciphertext reads only its N*digitKS
+// rows.  The steps are split into a.split ranges, a workgroup each, so that a
+// handful of ciphertexts still spreads over the CUs instead of queueing
+// thousands of dependent loads in two workgroups; k_lwe_keyswitch_finish adds
+// the partial sums.  The order of a sum mod Q does not change its value.
 template <typename W, int NT>
 __global__ __launch_bounds__(NT) void k_lwe_keyswitch_gather(KSArgs<W> a) {
-    const int col = blockIdx.x * NT + threadIdx.x;
+    const int cb = (a.n + 1 + NT - 1) / NT;  // column blocks; blockIdx.x = part * cb + column block
+    const int part = blockIdx.x / cb;
+    const int col = (blockIdx.x - part * cb) * NT + threadIdx.x;
     const int64_t c = blockIdx.y;
+    const int st0 = (int)((int64_t)a.S * part / a.split), st1 = (int)((int64_t)a.S * (part + 1) / a.split);
     const uint8_t *dg = a.ksd + c * (size_t)a.S;
     uint64_t s = 0;
     if (col <= a.n) {
-        for (int st = 0; st < a.S; st++) ks_add<W>(s, a.ksk[((size_t)st * a.Bks + dg[st]) * a.n_pad + col], a.Q);
-        a.out[(a.oidx ? a.oidx[c] : c) * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s, a.A));
+        for (int st = st0; st < st1; st++) ks_add<W>(s, a.ksk[((size_t)st * a.Bks + dg[st]) * a.n_pad + col], a.Q);
+        if (a.split == 1)
+            a.out[(a.oidx ? a.oidx[c] : c) * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s, a.A));
+        else
+            a.part[(c * a.split + part) * (a.n + 1) + col] = s;
     }
+}
+
+// Sum of the gather form's partial sums: 32-bit keys leave plain 64-bit sums
+// (below S Q in all), 64-bit keys canonical ones.
+template <typename W, int NT>
+__global__ __launch_bounds__(NT) void k_lwe_keyswitch_finish(KSArgs<W> a) {
+    const int col = blockIdx.x * NT + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (col > a.n) return;
+    uint64_t s = 0;
+    for (int part = 0; part < a.split; part++) {
+        const uint64_t v = a.part[(c * a.split + part) * (a.n + 1) + col];
+        if constexpr (sizeof(W) == 4) {
+            s += v;
+        } else {
+            const uint64_t t = s + v;
+            s = t >= a.Q ? t - a.Q : t;
+        }
+    }
+    a.out[(a.oidx ? a.oidx[c] : c) * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s, a.A));
 }
 
 // ---------------------------------------------------------------------------
@@ -1148,8 +1178,11 @@ hipError_t launch_blind_rotate(const BRArgs<W> &a, int logN, int nt, int64_t B, 
 template <typename W>
 hipError_t launch_lwe_keyswitch(const KSArgs<W> &a, bool gather, hipStream_t st) {
     if (gather) {
-        dim3 grid((a.n + 1 + 255) / 256, (unsigned)a.B);
-        hipLaunchKernelGGL((k_lwe_keyswitch_gather<W, 256>), grid, dim3(256), 0, st, a);
+        const unsigned cb = (unsigned)((a.n + 1 + 255) / 256);
+        hipLaunchKernelGGL((k_lwe_keyswitch_gather<W, 256>), dim3(cb * (unsigned)a.split, (unsigned)a.B), dim3(256), 0,
+                           st, a);
+        if (a.split > 1)
+            hipLaunchKernelGGL((k_lwe_keyswitch_finish<W, 256>), dim3(cb, (unsigned)a.B), dim3(256), 0, st, a);
         return hipGetLastError();
     }
     constexpr int KB = sizeof(W) == 4 ? 16 : 8;

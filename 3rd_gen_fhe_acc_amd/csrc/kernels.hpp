@@ -55,6 +55,10 @@ struct KSArgs {
     const int64_t *oidx;  // output row of ciphertext c (null: row c)
     int64_t B;
     int CB, KB, SB;
+    // gather form: the S steps of a ciphertext split into `split` ranges, one
+    // workgroup each, partial sums in part[B][split][n+1] (split > 1 only)
+    int split;
+    uint64_t *part;
 };
 
 template <typename W>
