@@ -43,7 +43,8 @@ class _Params(C.Structure):
 class _Info(C.Structure):
     _fields_ = [("Q", C.c_uint64), ("iN", C.c_uint64), ("ROU", C.c_uint64), ("digitG", C.c_uint32),
                 ("digitR", C.c_uint32), ("digitKS", C.c_uint32), ("digitKS_R", C.c_uint32),
-                ("narrow", C.c_uint32), ("gate_constant", C.c_uint64 * 6), ("bsk_words", C.c_uint64),
+                ("narrow", C.c_uint32), ("harvey", C.c_uint32), ("gate_constant", C.c_uint64 * 6),
+                ("bsk_words", C.c_uint64),
                 ("ksk_words", C.c_uint64), ("subs_words", C.c_uint64), ("bsk_device_bytes", C.c_uint64),
                 ("ksk_device_bytes", C.c_uint64), ("subs_device_bytes", C.c_uint64)]
 

@@ -47,6 +47,7 @@ Arith<W> make_arith(const HostParams &p) {
     A.Q = (W)p.Q;
     A.Q2 = (W)(2 * p.Q);
     A.mu = (uint64_t)(((u128)1 << 64) / p.Q);
+    if constexpr (sizeof(W) == 4) A.harvey = p.harvey;
     return A;
 }
 
@@ -272,6 +273,7 @@ int fheacc_ctx_info(const fheacc_ctx *c, fheacc_info *o) {
     o->digitKS = p.digitKS;
     o->digitKS_R = p.digitKS_R;
     o->narrow = p.narrow;
+    o->harvey = p.harvey;
     for (int g = 0; g < 6; g++) o->gate_constant[g] = p.gate_constant[g];
     o->bsk_words = (uint64_t)p.Br * p.digitR * p.n * p.digitG * 4 * p.N;
     o->ksk_words = (uint64_t)p.Bks * p.digitKS * p.N * (p.n + 1);

@@ -31,6 +31,7 @@ struct HostParams {
     std::vector<uint64_t> rou, irou;  // reference tables (params.cpp:149-175)
     std::vector<uint64_t> cdt;        // Gaussian magnitude table (2^63 scale)
     bool narrow;                      // Q < 2^30: 32-bit words on device
+    bool harvey;                      // narrow with Harvey's forward butterflies (28- to 30-bit Q)
 };
 
 int derive_params(const fheacc_params *in, HostParams *out, std::string *err);

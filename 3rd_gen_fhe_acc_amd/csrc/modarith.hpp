@@ -1,9 +1,9 @@
 // modarith.hpp -- modular arithmetic on the integer VALU of gfx950.
 //
 // Two word widths share every kernel through the Arith<W> policy:
-//   Arith<uint32_t>  "narrow", Q < 2^30 (27-bit sets): one VGPR per word,
-//                    Shoup products from v_mul_hi_u32/v_mul_lo_u32 and 64-bit
-//                    lazy sums from v_mad_u64_u32.
+//   Arith<uint32_t>  "narrow", Q < 2^30 (27- to 30-bit sets): one VGPR per
+//                    word, Shoup products from v_mul_hi_u32/v_mul_lo_u32 and
+//                    64-bit lazy sums from v_mad_u64_u32.
 //   Arith<uint64_t>  "wide", Q < 2^62 (54-bit sets): Shoup products built on
 //                    __umul64hi.
 // Every result that leaves a kernel is canonical in [0, Q), the reference's
@@ -24,6 +24,9 @@ struct Arith<uint32_t> {
     using W = uint32_t;
     W Q, Q2;
     uint64_t mu;  // floor(2^64 / Q), Barrett constant for 64-bit sums
+    // Harvey's forward butterflies (words below 4Q) instead of fully lazy
+    // ones: Q too large for (2 log2 N + 1) Q < 2^32, small enough for 4Q
+    bool harvey;
 
     // [0, 2m) -> [0, m): if x < m the subtraction wraps above x
     __device__ __forceinline__ static W red(W x, W m) {

@@ -19,8 +19,11 @@
 //
 // Forward butterflies are fully lazy: no reduction between stages.  A value
 // entering stage s is below (2s+1)Q, so after log2 N stages every word is
-// below (2 log2 N + 1) Q; the context only selects 32-bit words when that
-// bound fits in 32 bits.  The last pass makes the words canonical when asked.
+// below (2 log2 N + 1) Q.  When that bound does not fit 32-bit words but 4Q
+// does (28- to 30-bit Q), the context selects Harvey's butterflies instead
+// (Arith::harvey): the added input is first brought below 2Q, so every word
+// stays below 4Q, and the last pass leaves its words below 2Q.  The last
+// pass makes the words canonical when asked.
 #pragma once
 #include "modarith.hpp"
 
@@ -110,7 +113,8 @@ __device__ __forceinline__ ClusterTwiddles<R, W> inv_twiddles(int blk, const Twi
     return t;
 }
 
-template <int S, int R, typename W>
+// HV: Harvey's butterflies (words below 4Q); FINAL: the transform's last pass
+template <int S, int R, bool HV, bool FINAL, typename W>
 __device__ __forceinline__ void fwd_cluster(W *s, int b0, bool last, const Arith<W> &A, const Twiddles<W> &tw,
                                             const ClusterTwiddles<R, W> &t) {
     W *base = s + pad(b0);
@@ -124,12 +128,16 @@ __device__ __forceinline__ void fwd_cluster(W *s, int b0, bool last, const Arith
         for (int e = 0; e < (1 << R); e++) {
             if (e & h) continue;
             const int j = (1 << k) - 1 + (e >> (R - k));
+            if constexpr (HV) x[e] = Arith<W>::red(x[e], A.Q2);
             A.ct_butterfly(x[e], x[e + h], t.w[j], t.wp[j]);
         }
     }
     if (last) {
 #pragma unroll
         for (int e = 0; e < (1 << R); e++) x[e] = Arith<W>::red(A.mul_shoup(x[e], (W)1, tw.onep), A.Q);
+    } else if constexpr (HV && FINAL) {
+#pragma unroll
+        for (int e = 0; e < (1 << R); e++) x[e] = Arith<W>::red(x[e], A.Q2);
     }
 #pragma unroll
     for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
@@ -168,25 +176,27 @@ __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, con
 // When last, outputs are made canonical.  When NT is a multiple of the
 // clusters per poly, a thread meets the same in-poly cluster in every poly
 // it handles, and loads that cluster's twiddles once.
-template <int LOGN, int S0, int R, int NT, typename W>
+template <int LOGN, int S0, int R, int NT, bool HV, typename W>
 __device__ __forceinline__ void fwd_pass(W *s, int np, bool last, const Arith<W> &A, const Twiddles<W> &tw,
                                          int tid) {
     constexpr int N = 1 << LOGN;
     constexpr int lS = LOGN - S0 - R;  // log2 of the cluster stride
     constexpr int S = 1 << lS;
     constexpr int per_poly = N >> R;
+    constexpr bool FINAL = S0 + R == LOGN;
     if constexpr (NT % per_poly == 0) {
         const int c = tid % per_poly;
         const int blk = c >> lS, off = c & (S - 1);
         const ClusterTwiddles<R, W> t = fwd_twiddles<S0, R>(blk, tw);
         for (int p = tid / per_poly; p < np; p += NT / per_poly)
-            fwd_cluster<S, R>(s, p * N + (blk << (lS + R)) + off, last, A, tw, t);
+            fwd_cluster<S, R, HV, FINAL>(s, p * N + (blk << (lS + R)) + off, last, A, tw, t);
     } else {
         const int clusters = np * per_poly;
         for (int cg = tid; cg < clusters; cg += NT) {
             const int p = cg / per_poly, c = cg - p * per_poly;
             const int blk = c >> lS, off = c & (S - 1);
-            fwd_cluster<S, R>(s, p * N + (blk << (lS + R)) + off, last, A, tw, fwd_twiddles<S0, R>(blk, tw));
+            fwd_cluster<S, R, HV, FINAL>(s, p * N + (blk << (lS + R)) + off, last, A, tw,
+                                         fwd_twiddles<S0, R>(blk, tw));
         }
     }
 }
@@ -215,14 +225,14 @@ __device__ __forceinline__ void inv_pass(W *s, int np, const Arith<W> &A, const 
     }
 }
 
-template <int LOGN, int ST, int NT, int MAXR, typename W>
+template <int LOGN, int ST, int NT, int MAXR, bool HV, typename W>
 __device__ __forceinline__ void fwd_passes(W *s, int np, bool canonical, const Arith<W> &A, const Twiddles<W> &tw,
                                            int tid) {
     if constexpr (ST < LOGN) {
         constexpr int R = pass_stages<MAXR>(LOGN - ST);
-        fwd_pass<LOGN, ST, R, NT>(s, np, canonical && ST + R == LOGN, A, tw, tid);
+        fwd_pass<LOGN, ST, R, NT, HV>(s, np, canonical && ST + R == LOGN, A, tw, tid);
         __syncthreads();
-        fwd_passes<LOGN, ST + R, NT, MAXR>(s, np, canonical, A, tw, tid);
+        fwd_passes<LOGN, ST + R, NT, MAXR, HV>(s, np, canonical, A, tw, tid);
     }
 }
 
@@ -237,12 +247,20 @@ __device__ __forceinline__ void inv_passes(W *s, int np, const Arith<W> &A, cons
 }
 
 // Forward transform of np padded polys in LDS (canonical inputs).  Output
-// canonical in [0, Q), or, when canonical is false, below (2 logN + 1) Q and
-// congruent to the canonical word (for consumers that reduce anyway).
+// canonical in [0, Q), or, when canonical is false, below (2 logN + 1) Q
+// (below 2Q with Harvey's butterflies) and congruent to the canonical word
+// (for consumers that reduce anyway).
 template <int LOGN, int NT, int MAXR = 4, typename W>
 __device__ void ntt_forward(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, bool canonical = true,
                             int tid = -1) {
-    fwd_passes<LOGN, 0, NT, MAXR>(s, np, canonical, A, tw, tid < 0 ? (int)threadIdx.x : tid);
+    const int t = tid < 0 ? (int)threadIdx.x : tid;
+    if constexpr (sizeof(W) == 4) {
+        if (A.harvey) {
+            fwd_passes<LOGN, 0, NT, MAXR, true>(s, np, canonical, A, tw, t);
+            return;
+        }
+    }
+    fwd_passes<LOGN, 0, NT, MAXR, false>(s, np, canonical, A, tw, t);
 }
 
 // Inverse transform of np padded polys in LDS (inputs in [0, 2Q)); output

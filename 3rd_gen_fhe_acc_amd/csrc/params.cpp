@@ -9,6 +9,7 @@
 // (fhe_wo_fpga.cpp:141-146).
 #include <math.h>
 
+#include <algorithm>
 #include <string>
 
 #include "fheacc_internal.hpp"
@@ -152,12 +153,20 @@ int derive_params(const fheacc_params *in, HostParams *p, std::string *err) {
     }
     p->cdt.back() = 1ULL << 63;
     // Lazy forward butterflies leave words below (2 log2 N + 1) Q; 32-bit
-    // device words need that below 2^32 (and 2 digitG (Q-1)^2 < 2^64 for the
-    // lazy external-product sums), 64-bit words below 2^64.
+    // device words need that below 2^32, 64-bit words below 2^64.  Harvey's
+    // butterflies keep 32-bit words below 4Q and hand the MAC words below 2Q.
+    // The transformed digits reach the MAC at that bound, and a sum of
+    // max(2 digitG, digitKS_R) products with keys below Q must stay below 2^64.
     const u128 lazy = (u128)(2 * p->logN + 1) * Q;
     if (lazy >= ((u128)1 << 64)) return bad("Q too large for lazy 64-bit butterflies");
-    // (the external-product digits reach the MAC lazily, below the same bound)
-    p->narrow = lazy < ((u128)1 << 32) && (u128)(2 * p->digitG) * lazy * (Q - 1) < ((u128)1 << 64);
+    const u128 terms = (u128)std::max<uint32_t>(2 * p->digitG, p->digitKS_R);
+    const u128 two64 = (u128)1 << 64, two32 = (u128)1 << 32;
+    p->harvey = false;
+    p->narrow = lazy < two32 && terms * lazy * (Q - 1) < two64;
+    if (!p->narrow && 4 * (u128)Q < two32 && terms * 2 * (u128)Q * (Q - 1) < two64) {
+        p->narrow = true;
+        p->harvey = true;
+    }
     return 0;
 }
 

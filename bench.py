@@ -58,6 +58,10 @@ def parse():
     ap.add_argument("--no-leveled", action="store_true", help="skip the RLWE expansion / blind-rotate microbench")
     ap.add_argument("--no-fpga-order", action="store_true",
                     help="skip timing the batch in the accelerator's step orders")
+    ap.add_argument("--no-copy-ceiling", action="store_true",
+                    help="skip measuring the device-to-device copy rate (the practical HBM ceiling)")
+    ap.add_argument("--no-app256", action="store_true",
+                    help="skip the reference's 256-bit application set (29-bit Q, N=2048)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (reference dispatch for HBM byte counters)")
     return ap.parse_args()
@@ -318,6 +322,61 @@ def leveled(fhe, torch, B=8, reps=3):
     }
 
 
+def app256(fhe, torch, B=2048, reps=3):
+    """The reference's 256-bit application set (apps/bootstrap_fpga_256.cpp:
+    n=1024 q=1024 N=2048, 29-bit Q, BG=2^10, Br=32, Bks=25): 32-bit device
+    words with Harvey's butterflies."""
+    P = dict(n=1024, q=1024, N=2048, Q=0, nbit=29, BGbits=10, Br=32, Bks=25)
+    ctx = fhe.Context(**P)
+    keys = ctx.keys()
+    s, z = ctx.lwe_secret(SEED + 5, binary=True), ctx.rlwe_secret(SEED + 5)
+    keys.generate(SEED + 5, s, z)
+    tune = ctx.autotune(keys, min(B, 1024))
+    rng = np.random.default_rng(SEED + 5)
+    m1, m2 = rng.integers(0, 2, B).astype(np.uint8), rng.integers(0, 2, B).astype(np.uint8)
+    c1, c2 = ctx.lwe_encrypt(s, m1, SEED + 5, 0), ctx.lwe_encrypt(s, m2, SEED + 5, B)
+    gates = np.full(B, fhe.GATES["NAND"], dtype=np.int32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.int64) if a.dtype == np.uint64 else a).cuda()
+    d1, d2, dg = t(c1), t(c2), t(gates)
+    out = torch.zeros_like(d1)
+    ctx.bootstrap_dev(keys, dg, d1, d2, out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        ctx.bootstrap_dev(keys, dg, d1, d2, out)
+    e1.record()
+    torch.cuda.synchronize()
+    ctx.sync()
+    ms = e0.elapsed_time(e1) / reps
+    bits = ctx.lwe_decrypt(s, out.cpu().numpy().view(np.uint64))
+    info = ctx.info
+    return {"workload": "batched NAND gate bootstrap, apps/bootstrap_fpga_256.cpp parameters (n=1024 q=1024 "
+                        "N=2048 Q=%s BG=2^10 Br=32 Bks=25), batch %d, eager" % (hex(info["Q"]), B),
+            "device_words": "u32, Harvey butterflies" if info["harvey"] else ("u32" if info["narrow"] else "u64"),
+            "ms_per_batch": round(ms, 3), "bootstraps_per_s": round(B / (ms / 1e3), 1),
+            "decrypt_ok": bool(np.array_equal(bits, 1 - (m1 & m2))), "tuning": tune.get("chosen")}
+
+
+def copy_ceiling(torch, nbytes=1 << 30, reps=5):
+    """Measured device-to-device copy rate (bytes read + bytes written per
+    second): the practical HBM ceiling beside the nominal 8 TB/s."""
+    src = torch.empty(nbytes // 8, dtype=torch.int64, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def fpga_orders(fhe, torch, ctx, keys, dg, d1, d2, nega, s, want, reps=3):
     """The same batch in the accelerator's step orders (fpga_eval_bootstrap_x1
     and _x4..x16, myfpga.cpp:1003-1660): x1 adds a product by btkey[0][0][0]
@@ -492,6 +551,7 @@ def main():
     lat = np.array(lat if lat else [float("nan")])
     single_rate = 1e3 / lat.mean()
 
+    copy_gbs = None if args.no_copy_ceiling else copy_ceiling(torch)
     # roofline of the dominant kernel (blind rotation), algorithmic bytes
     info = ctx.info
     n, N, dks = PARAMS["n"], PARAMS["N"], ctx.digitKS
@@ -543,7 +603,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_blind_rotate", "avg_launch_ms": round(br_avg, 4),
-                     "algorithmic_bytes_per_launch": br_bytes},
+                     "algorithmic_bytes_per_launch": br_bytes,
+                     "copy_ceiling_GBps": round(copy_gbs, 1) if copy_gbs else None,
+                     "frac_of_copy_ceiling": round(achieved / copy_gbs, 5) if copy_gbs else None},
         "valu": {"kernel": "k_blind_rotate", "bound": "integer VALU (the kernel's roofline; no MFMA applies)",
                  "achieved": round(valu_instr * 64 / (br_avg * 1e-3) / 1e12, 3) if valu_instr else None,
                  "peak": round(valu_peak / 1e12, 3), "unit": "T lane-ops/s",
@@ -583,6 +645,8 @@ def main():
         rec["mixed"] = mixed(fhe, torch, ctx, keys)
     if not args.no_leveled:
         rec["leveled"] = leveled(fhe, torch)
+    if not args.no_app256:
+        rec["app_256"] = app256(fhe, torch)
     if not args.no_cpu_baseline and args.cpu_sample > 0:
         rec["cpu_baseline"] = cpu_baseline(ctx, keys, c1, c2, gates, out, args.cpu_sample)
     print(json.dumps(rec))

@@ -58,6 +58,7 @@ typedef struct fheacc_info {
     uint64_t Q, iN, ROU;
     uint32_t digitG, digitR, digitKS, digitKS_R;
     uint32_t narrow;        /* 1: 32-bit device words (Q < 2^30) */
+    uint32_t harvey;        /* 1: 32-bit words with Harvey's forward butterflies (28- to 30-bit Q) */
     uint64_t gate_constant[6];
     uint64_t bsk_words;     /* words of the reference-layout bootstrap key */
     uint64_t ksk_words;     /* words of the reference-layout LWE ks key */

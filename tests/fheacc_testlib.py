@@ -24,6 +24,12 @@ SETS = {
     "n12_N1024_q27": dict(n=12, q=1024, N=1024, Q=0, nbit=27, BGbits=7, Br=32, Bks=32),
     # ring degree 512, 27-bit Q, BG=2^6 d=5 (mixed-set C shape, n reduced)
     "n8_N512_q27_bg6": dict(n=8, q=1024, N=512, Q=0, nbit=27, BGbits=6, Br=32, Bks=32),
+    # 29-bit Q at N = 2048, BG=2^10 d=3, Bks=25: the reference's 256-bit app
+    # shape (apps/bootstrap_fpga_256.cpp), n reduced; 32-bit words with
+    # Harvey's butterflies
+    "n8_N2048_q29": dict(n=8, q=1024, N=2048, Q=0, nbit=29, BGbits=10, Br=32, Bks=25),
+    # 30-bit Q at N = 1024, at the 4Q < 2^32 edge of 32-bit words
+    "n8_N1024_q30": dict(n=8, q=1024, N=1024, Q=0, nbit=30, BGbits=10, Br=32, Bks=32),
 }
 
 # full-size configurations

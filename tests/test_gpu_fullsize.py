@@ -77,6 +77,8 @@ APP_SETS = {
     "app_128": dict(n=512, q=512, N=1024, Q=0, nbit=27, BGbits=9, Br=23, Bks=25),
     # apps/bootstrap_fpga_medium.cpp:43
     "app_medium": dict(n=256, q=512, N=1024, Q=0, nbit=27, BGbits=9, Br=23, Bks=25),
+    # apps/bootstrap_fpga_256.cpp:43-135: 29-bit Q at N = 2048, 32-bit words with Harvey's butterflies
+    "app_256": dict(n=1024, q=1024, N=2048, Q=0, nbit=29, BGbits=10, Br=32, Bks=25),
 }
 
 

@@ -18,6 +18,8 @@ SETS = {
     "N256_q27": dict(n=4, q=512, N=256, Q=0, nbit=27, BGbits=7, Br=16, Bks=32, Bks_R_bits=7),
     # 54-bit Q, wide device words, Bks_R = 2^9
     "N128_q54": dict(n=4, q=256, N=128, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=16, Bks=32, Bks_R_bits=9),
+    # 29-bit Q, 32-bit words with Harvey's butterflies, Bks_R = 2^8
+    "N256_q29": dict(n=4, q=512, N=256, Q=0, nbit=29, BGbits=10, Br=16, Bks=32, Bks_R_bits=8),
 }
 # the reference's expansion app (apps/rlwe_expand_fpga.cpp:43-144)
 EXPAND_APP = dict(n=512, q=512, N=2048, Q=0x3FFFFFFFFED001, nbit=54, BGbits=9, Br=23, Bks=25, Bks_R_bits=9)

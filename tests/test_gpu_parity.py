@@ -29,6 +29,7 @@ def pair(name):
 
 
 @pytest.mark.parametrize("N,nbit,Q", [(16, 27, 0), (256, 27, 0), (512, 27, 0), (1024, 27, 0), (2048, 27, 0),
+                                      (1024, 29, 0), (2048, 29, 0),  # 32-bit words, Harvey's butterflies
                                       (128, 54, 0x3FFFFFFFFED001), (1024, 54, 0x3FFFFFFFFED001),
                                       (2048, 54, 0x3FFFFFFFFED001)])
 def test_ntt_matches_oracle(N, nbit, Q):
@@ -46,6 +47,15 @@ def test_ntt_matches_oracle(N, nbit, Q):
     assert np.array_equal(host(t2), o.intt(x))
 
 
+@pytest.mark.parametrize("name,narrow,harvey", [("n12_N1024_q27", 1, 0), ("n8_N2048_q29", 1, 1),
+                                               ("n8_N1024_q30", 1, 1), ("n8_N2048_q54", 0, 0)])
+def test_word_width_selection(name, narrow, harvey):
+    # 27-bit Q: fully lazy 32-bit butterflies; 29/30-bit Q: 32-bit words with
+    # Harvey's butterflies; 54-bit Q: 64-bit words
+    info = fhe.Context(**SETS[name]).info
+    assert (info["narrow"], info["harvey"]) == (narrow, harvey)
+
+
 def test_external_product_golden(golden_dir):
     g = np.load(os.path.join(golden_dir, "rlwe_x_rgsw_n1024.npz"))
     ctx = fhe.Context(n=8, q=512, N=1024, Q=int(g["Q"]), BGbits=9)
@@ -58,7 +68,9 @@ def test_external_product_golden(golden_dir):
 @pytest.mark.parametrize("N,Q,BGbits", [(1024, 0x3FFFFFFFFED001, 9),   # microbench shape, d=6
                                         (1024, 0, 7),                   # benchmark shape, d=4
                                         (2048, 0x3FFFFFFFFED001, 18),   # set B, d=3
-                                        (512, 0, 6)])                   # set C, d=5
+                                        (512, 0, 6),                    # set C, d=5
+                                        (2048, 0x1FFF2001, 10),         # 29-bit, Harvey's butterflies
+                                        (1024, 0x3FFF7801, 10)])        # 30-bit, Harvey's butterflies
 @pytest.mark.parametrize("per_ct", [False, True])
 def test_external_product_matches_oracle(N, Q, BGbits, per_ct):
     kw = dict(n=4, q=1024, N=N, Q=Q, nbit=27, BGbits=BGbits)
@@ -133,7 +145,7 @@ def run_bootstrap_case(name, B, seed, upload=False):
 
 
 @pytest.mark.parametrize("name", ["n16_N256_q27", "n12_N512_q27", "n10_N128_q54", "n8_N2048_q54",
-                                  "n8_N512_q27_bg6"])
+                                  "n8_N512_q27_bg6", "n8_N2048_q29", "n8_N1024_q30"])
 def test_bootstrap_matches_oracle(name):
     run_bootstrap_case(name, B=24, seed=3)
 
@@ -241,7 +253,7 @@ def test_subs_keyswitch_golden(golden_dir):
     assert np.array_equal(host(out), g["out"])
 
 
-@pytest.mark.parametrize("name", ["n16_N256_q27", "n10_N128_q54"])
+@pytest.mark.parametrize("name", ["n16_N256_q27", "n10_N128_q54", "n8_N2048_q29"])
 def test_subs_keyswitch_matches_oracle(name):
     ctx, o = pair(name)
     z = ctx.rlwe_secret(9)
