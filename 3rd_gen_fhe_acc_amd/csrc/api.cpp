@@ -48,6 +48,7 @@ Arith<W> make_arith(const HostParams &p) {
     A.Q2 = (W)(2 * p.Q);
     A.mu = (uint64_t)(((u128)1 << 64) / p.Q);
     if constexpr (sizeof(W) == 4) A.harvey = p.harvey;
+    else A.Q5 = 5 * p.Q;
     return A;
 }
 

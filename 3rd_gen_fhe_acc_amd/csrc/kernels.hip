@@ -121,7 +121,7 @@ struct MacSum<uint64_t> {
     static constexpr bool kCanonicalInputs = false;
     uint64_t v = 0;
     __device__ __forceinline__ void add(const Arith<uint64_t> &A, uint64_t d, uint64_t k, uint64_t kp) {
-        v += A.mul_shoup(d, k, kp);
+        v += A.mul_shoup_lazy(d, k, kp);
     }
     __device__ __forceinline__ uint64_t get(const Arith<uint64_t> &A) const { return A.reduce_small(v); }
 };

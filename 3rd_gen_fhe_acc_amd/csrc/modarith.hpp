@@ -64,21 +64,38 @@ template <>
 struct Arith<uint64_t> {
     using W = uint64_t;
     W Q, Q2;
-    uint64_t mu;  // unused for the wide path (sums are reduced from [0, 2dQ))
+    uint64_t mu;  // unused for the wide path (sums are reduced from [0, 10dQ))
+    W Q5;         // 5Q: the bound of mul_shoup_lazy
 
     __device__ __forceinline__ static W red(W x, W m) {
         W y = x - m;
         return y < x ? y : x;
     }
+    // x * w mod Q in [0, 2Q) for any 64-bit x; wp = floor(w * 2^64 / Q)
     __device__ __forceinline__ W mul_shoup(W x, W w, W wp) const {
         W qh = __umul64hi(x, wp);
         return x * w - qh * Q;
     }
+    // x * w mod Q in [0, 5Q) for any 64-bit x.  The Shoup quotient
+    // floor(x wp / 2^64) is taken from three 32-bit partial products: the
+    // low-by-low product and the carries out of the low word are dropped,
+    // each worth less than one unit, so the estimate is at most 3 short and
+    // the remainder below 5Q.  One half-rate v_mad_u64_u32 and two
+    // v_mul_hi_u32 instead of the full 64x64 high product.
+    __device__ __forceinline__ W mul_shoup_lazy(W x, W w, W wp) const {
+        const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32);
+        const uint32_t pl = (uint32_t)wp, ph = (uint32_t)(wp >> 32);
+        const uint64_t mid = (uint64_t)__umulhi(xl, ph) + __umulhi(xh, pl);
+        const uint64_t qh = (uint64_t)xh * ph + mid;
+        return x * w - qh * Q;
+    }
+    // (a, b) -> (a + wb, a - wb + 5Q), wb = b w mod Q in [0, 5Q): a word
+    // entering stage s is below (5s + 1) Q
     __device__ __forceinline__ void ct_butterfly(W &a, W &b, W w, W wp) const {
-        const W v = mul_shoup(b, w, wp);
+        const W v = mul_shoup_lazy(b, w, wp);
         const W a0 = a;
         a = a0 + v;
-        b = a0 - v + Q2;
+        b = a0 - v + Q5;
     }
     // x < 2^62, any multiple of Q below that -> [0, Q)
     __device__ __forceinline__ W reduce_small(uint64_t x) const {

@@ -152,15 +152,18 @@ int derive_params(const fheacc_params *in, HostParams *p, std::string *err) {
         if (v == (1ULL << 63)) break;
     }
     p->cdt.back() = 1ULL << 63;
-    // Lazy forward butterflies leave words below (2 log2 N + 1) Q; 32-bit
-    // device words need that below 2^32, 64-bit words below 2^64.  Harvey's
-    // butterflies keep 32-bit words below 4Q and hand the MAC words below 2Q.
-    // The transformed digits reach the MAC at that bound, and a sum of
-    // max(2 digitG, digitKS_R) products with keys below Q must stay below 2^64.
-    const u128 lazy = (u128)(2 * p->logN + 1) * Q;
-    if (lazy >= ((u128)1 << 64)) return bad("Q too large for lazy 64-bit butterflies");
+    // Lazy forward butterflies leave 32-bit words below (2 log2 N + 1) Q,
+    // which must be below 2^32.  Harvey's butterflies keep 32-bit words below
+    // 4Q and hand the MAC words below 2Q.  The transformed digits reach the
+    // MAC at that bound, and a sum of max(2 digitG, digitKS_R) products with
+    // keys below Q must stay below 2^64.  64-bit words take products in
+    // [0, 5Q) (Arith<uint64_t>::mul_shoup_lazy): forward words stay below
+    // (5 log2 N + 1) Q < 2^64, and MAC sums below 5Q per term < 2^62.
     const u128 terms = (u128)std::max<uint32_t>(2 * p->digitG, p->digitKS_R);
     const u128 two64 = (u128)1 << 64, two32 = (u128)1 << 32;
+    if ((u128)(5 * p->logN + 1) * Q >= two64 || terms * 5 * (u128)Q >= ((u128)1 << 62))
+        return bad("Q too large for lazy 64-bit butterflies");
+    const u128 lazy = (u128)(2 * p->logN + 1) * Q;
     p->harvey = false;
     p->narrow = lazy < two32 && terms * lazy * (Q - 1) < two64;
     if (!p->narrow && 4 * (u128)Q < two32 && terms * 2 * (u128)Q * (Q - 1) < two64) {
