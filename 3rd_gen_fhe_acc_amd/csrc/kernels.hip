@@ -491,15 +491,26 @@ __device__ __forceinline__ uint64_t ks_reduce(uint64_t s, const Arith<uint64_t> 
 // ciphertext per thread.  The Bks candidate rows (KB words each) of 16 steps
 // are staged in LDS per fill; each thread reads its own 16 digits with one
 // 16-byte load and adds the rows they select.  The key is streamed once per
-// NT ciphertexts.  Rows are padded to RS words to spread LDS banks.
+// NT ciphertexts.  Rows are padded to RS words to spread LDS banks.  When a
+// fill is at most KS_PV vectors per thread (Bks <= 32) the fills are double
+// buffered: fill k+1's key vectors and digits are in flight in registers
+// while fill k is summed, and one barrier per fill suffices.
+constexpr int KS_SB = 16;  // steps per LDS fill
+constexpr int KS_PV = 8;   // staged 16-byte vectors per thread when double buffered
+
+template <typename W, int KB>
+__host__ __device__ constexpr bool ks_double_buffered(uint32_t Bks, int nt) {
+    return KS_SB * (int)Bks * (KB / (16 / (int)sizeof(W))) <= KS_PV * nt;
+}
+
 template <typename W, int NT, int KB>
 __global__ __launch_bounds__(NT) void k_lwe_keyswitch_staged(KSArgs<W> a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int SB = 16;                             // steps per LDS fill
+    constexpr int SB = KS_SB;
     constexpr int RS = KB + 16 / (int)sizeof(W);       // padded row stride (words)
     constexpr int V = 16 / (int)sizeof(W);             // words per 16-byte vector
     const uint32_t Bks = a.Bks;
-    W *kt = reinterpret_cast<W *>(smem);               // [SB][Bks][RS]
+    W *kt0 = reinterpret_cast<W *>(smem);              // [SB][Bks][RS] (twice when double buffered)
     const int col0 = blockIdx.x * KB;
     const int64_t c = (int64_t)blockIdx.y * NT + threadIdx.x;
     const bool live = c < a.B;
@@ -509,32 +520,68 @@ __global__ __launch_bounds__(NT) void k_lwe_keyswitch_staged(KSArgs<W> a) {
 #pragma unroll
     for (int j = 0; j < KB; j++) s[j] = 0;
     const int vec_per_fill = SB * Bks * (KB / V);
-    for (int s0 = 0; s0 < S; s0 += SB) {
-        const int nsb = min(SB, S - s0);
-        for (int e = threadIdx.x; e < vec_per_fill; e += NT) {
-            const int row = e / (KB / V), vv = e - row * (KB / V);  // row = st * Bks + dd
-            const int st = row / Bks;
-            const int gc = col0 + vv * V;
-            uint4 val = make_uint4(0, 0, 0, 0);
-            if (st < nsb && gc + V <= a.n_pad)
-                val = *reinterpret_cast<const uint4 *>(a.ksk + ((size_t)(s0 * Bks) + row) * a.n_pad + gc);
-            *reinterpret_cast<uint4 *>(kt + row * RS + vv * V) = val;
-        }
-        uint8_t d16[SB];
-        if (nsb == SB) {
-            uint4 dv = *reinterpret_cast<const uint4 *>(dg + s0);
-            const uint32_t w4[4] = {dv.x, dv.y, dv.z, dv.w};
-#pragma unroll
-            for (int t = 0; t < SB; t++) d16[t] = (uint8_t)(w4[t >> 2] >> (8 * (t & 3)));
+    // staged vector e of the fill that starts at step s0 (zero past the key)
+    auto fetch = [&](int s0, int e) {
+        const int row = e / (KB / V), vv = e - row * (KB / V);  // row = st * Bks + dd
+        const int st = row / Bks;
+        const int gc = col0 + vv * V;
+        u32x4 val = {0, 0, 0, 0};
+        if (s0 + st < S && gc + V <= a.n_pad)
+            val = ld_global<u32x4>(a.ksk + ((size_t)(s0 * Bks) + row) * a.n_pad + gc, 0);
+        return val;
+    };
+    auto put = [&](W *kt, int e, u32x4 val) {
+        const int row = e / (KB / V), vv = e - row * (KB / V);
+        *reinterpret_cast<u32x4 *>(kt + row * RS + vv * V) = val;
+    };
+    // this thread's 16 digits of the fill at s0, packed, zero past S
+    auto fetch_digits = [&](int s0) {
+        u32x4 dv = {0, 0, 0, 0};
+        if (s0 + SB <= S) {
+            dv = ld_global<u32x4>(dg + s0, 0);
         } else {
-#pragma unroll
-            for (int t = 0; t < SB; t++) d16[t] = t < nsb ? dg[s0 + t] : 0;
+            uint32_t w[4] = {0, 0, 0, 0};
+            for (int t = 0; s0 + t < S; t++) w[t >> 2] |= (uint32_t)dg[s0 + t] << (8 * (t & 3));
+            dv.x = w[0]; dv.y = w[1]; dv.z = w[2]; dv.w = w[3];
         }
-        __syncthreads();
+        return dv;
+    };
+    // 32-bit keys with 16 Q < 2^32: a fill's 16 rows sum in 32 bits and are
+    // added to the 64-bit sums once per fill
+    const bool fold = sizeof(W) == 4 && (uint64_t)SB * a.Q < (1ULL << 32);
+    auto accumulate = [&](const W *kt, int s0, u32x4 dv) {
+        const uint32_t w4[4] = {dv.x, dv.y, dv.z, dv.w};
+        const int nsb = min(SB, S - s0);
+        if constexpr (sizeof(W) == 4) {
+            if (fold) {
+                uint32_t f[KB];
+#pragma unroll
+                for (int j = 0; j < KB; j++) f[j] = 0;
+#pragma unroll
+                for (int st = 0; st < SB; st++) {
+                    if (st < nsb) {
+                        const int d = (int)((w4[st >> 2] >> (8 * (st & 3))) & 0xFF);
+                        const W *row = kt + (st * (int)Bks + d) * RS;
+#pragma unroll
+                        for (int vv = 0; vv < KB / V; vv++) {
+                            uint4 q = *reinterpret_cast<const uint4 *>(row + vv * V);
+                            f[vv * 4 + 0] += q.x;
+                            f[vv * 4 + 1] += q.y;
+                            f[vv * 4 + 2] += q.z;
+                            f[vv * 4 + 3] += q.w;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < KB; j++) s[j] += f[j];
+                return;
+            }
+        }
 #pragma unroll
         for (int st = 0; st < SB; st++) {
             if (st < nsb) {
-                const W *row = kt + (st * (int)Bks + d16[st]) * RS;
+                const int d = (int)((w4[st >> 2] >> (8 * (st & 3))) & 0xFF);
+                const W *row = kt + (st * (int)Bks + d) * RS;
 #pragma unroll
                 for (int vv = 0; vv < KB / V; vv++) {
                     uint4 q = *reinterpret_cast<const uint4 *>(row + vv * V);
@@ -550,7 +597,49 @@ __global__ __launch_bounds__(NT) void k_lwe_keyswitch_staged(KSArgs<W> a) {
                 }
             }
         }
-        __syncthreads();
+    };
+    if (ks_double_buffered<W, KB>(Bks, NT)) {
+        W *kt1 = kt0 + SB * (int)Bks * RS;
+#pragma unroll
+        for (int i = 0; i < KS_PV; i++) {
+            const int e = threadIdx.x + i * NT;
+            if (e < vec_per_fill) put(kt0, e, fetch(0, e));
+        }
+        u32x4 dcur = fetch_digits(0);
+        for (int s0 = 0, k = 0; s0 < S; s0 += SB, k++) {
+            W *cur = (k & 1) ? kt1 : kt0, *nxt = (k & 1) ? kt0 : kt1;
+            const bool more = s0 + SB < S;
+            u32x4 pre[KS_PV];
+            u32x4 dnext = {0, 0, 0, 0};
+            if (more) {
+#pragma unroll
+                for (int i = 0; i < KS_PV; i++) {
+                    const int e = threadIdx.x + i * NT;
+                    if (e < vec_per_fill) pre[i] = fetch(s0 + SB, e);
+                }
+                dnext = fetch_digits(s0 + SB);
+            }
+            // cur was written before this barrier (by the previous pass), and
+            // every read of nxt in the previous pass happened before it
+            __syncthreads();
+            accumulate(cur, s0, dcur);
+            if (more) {
+#pragma unroll
+                for (int i = 0; i < KS_PV; i++) {
+                    const int e = threadIdx.x + i * NT;
+                    if (e < vec_per_fill) put(nxt, e, pre[i]);
+                }
+            }
+            dcur = dnext;
+        }
+    } else {
+        for (int s0 = 0; s0 < S; s0 += SB) {
+            for (int e = threadIdx.x; e < vec_per_fill; e += NT) put(kt0, e, fetch(s0, e));
+            const u32x4 dv = fetch_digits(s0);
+            __syncthreads();
+            accumulate(kt0, s0, dv);
+            __syncthreads();
+        }
     }
     if (!live) return;
     const int64_t orow = a.oidx ? a.oidx[c] : c;
@@ -1198,7 +1287,8 @@ hipError_t launch_lwe_keyswitch(const KSArgs<W> &a, bool gather, hipStream_t st)
 template <typename W>
 size_t ks_staged_lds(uint32_t Bks) {
     constexpr int KB = sizeof(W) == 4 ? 16 : 8;
-    return (size_t)16 * Bks * (KB + 16 / sizeof(W)) * sizeof(W);
+    const size_t fill = (size_t)KS_SB * Bks * (KB + 16 / sizeof(W)) * sizeof(W);
+    return ks_double_buffered<W, KB>(Bks, 256) ? 2 * fill : fill;
 }
 template size_t ks_staged_lds<uint32_t>(uint32_t);
 template size_t ks_staged_lds<uint64_t>(uint32_t);
