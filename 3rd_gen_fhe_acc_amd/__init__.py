@@ -55,7 +55,7 @@ class _Capture(C.Structure):
 
 class _ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
-                ("min_fill", C.c_int32)]
+                ("min_fill", C.c_int32), ("fair_share", C.c_int32)]
 
 
 class _ServerStats(C.Structure):
@@ -505,9 +505,9 @@ class Server:
     """The serving loop (fheacc_server_*): gate-circuit jobs of many clients,
     admitted by HBM footprint and coalesced into shared gate-path launches."""
 
-    def __init__(self, ctx, hbm_budget=1 << 30, max_batch=2048, max_inflight=2, min_fill=0):
+    def __init__(self, ctx, hbm_budget=1 << 30, max_batch=2048, max_inflight=2, min_fill=0, fair_share=True):
         self.ctx = ctx
-        cfg = _ServerConfig(int(hbm_budget), int(max_batch), int(max_inflight), int(min_fill))
+        cfg = _ServerConfig(int(hbm_budget), int(max_batch), int(max_inflight), int(min_fill), int(bool(fair_share)))
         h = C.c_void_p()
         _check(lib().fheacc_server_create(ctx.h, C.byref(cfg), C.byref(h)))
         self.h = h

@@ -14,8 +14,10 @@
 // Inside the range the output gates come last, in output order, so the
 // outputs leave in one copy.
 //
-// Launch: a launch takes ready gates of the admitted jobs, oldest job first,
-// up to max_batch, orders them by client and runs the blind rotation once
+// Launch: a launch takes ready gates of the admitted jobs up to max_batch
+// (with fair_share, each job first gets an equal share, so that a small job
+// is not queued behind a large one's ready level; then oldest job first),
+// orders them by client and runs the blind rotation once
 // for all of them (each ciphertext indexes its client's bootstrap key) and
 // the key switch once per client segment.  Up to max_inflight launches run
 // at once, each on its own stream with its own scratch and descriptors.
@@ -349,8 +351,23 @@ bool fheacc_server::launch() {
     int si = 0;
     while (slots[si].busy) si++;
     Slot &s = slots[si];
-    // oldest job first
     s.entries.clear();
+    auto take = [&](const std::shared_ptr<Job> &jp, int64_t most) {
+        Job *j = jp.get();
+        for (int64_t t = 0; t < most && !j->ready.empty() && (int64_t)s.entries.size() < cfg.max_batch; t++) {
+            s.entries.emplace_back(jp, j->ready.front());
+            j->ready.pop_front();
+            j->inflight++;
+            nready--;
+        }
+    };
+    if (cfg.fair_share) {
+        int64_t jobs = 0;
+        for (auto &jp : active) jobs += !jp->ready.empty();
+        const int64_t share = std::max<int64_t>(1, cfg.max_batch / std::max<int64_t>(1, jobs));
+        for (auto &jp : active) take(jp, share);
+    }
+    // then oldest job first
     for (auto &jp : active) {
         Job *j = jp.get();
         while (!j->ready.empty() && (int64_t)s.entries.size() < cfg.max_batch) {
@@ -485,6 +502,7 @@ int fheacc_server_create(fheacc_ctx *ctx, const fheacc_server_config *cfg, fheac
     if (cfg->max_batch < 1 || cfg->max_batch > (1 << 20)) return fail("max_batch must be in [1, 2^20]");
     if (cfg->max_inflight < 1 || cfg->max_inflight > kMaxInflight) return fail("max_inflight must be in [1, 4]");
     if (cfg->min_fill < 0) return fail("min_fill must be >= 0");
+    if (cfg->fair_share != 0 && cfg->fair_share != 1) return fail("fair_share must be 0 or 1");
     const HostParams &p = ctx->p;
     const int64_t rw = p.n + 1;
     const int64_t rows = (int64_t)(cfg->hbm_budget / (uint64_t)(rw * 8));

@@ -52,8 +52,11 @@ def parse():
     ap.add_argument("--serve-qps", type=float, default=6.0, help="serving-loop Poisson arrival rate")
     ap.add_argument("--serve-budget-gb", type=float, default=8.0, help="serving-loop HBM budget for ciphertexts")
     ap.add_argument("--serve-max-batch", type=int, default=4096)
-    ap.add_argument("--serve-inflight", type=int, default=2)
+    ap.add_argument("--serve-inflight", type=int, default=1,
+                    help="serving launches in flight (one gives lower gate latency at this load)")
     ap.add_argument("--serve-min-fill", type=int, default=0)
+    ap.add_argument("--serve-fifo", action="store_true",
+                    help="fill serving launches oldest job first (default: fair share per job, then oldest first)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-parameter-set batch")
     ap.add_argument("--no-leveled", action="store_true", help="skip the RLWE expansion / blind-rotate microbench")
     ap.add_argument("--no-fpga-order", action="store_true",
@@ -149,7 +152,7 @@ def serving(fhe, ctx, args):
         reqs.append(dict(t=ta, client=c, k=k, m=m, inputs=inputs, ops=ops, outs=outs, want=want))
     encrypt_s = time.perf_counter() - t0
     srv = fhe.Server(ctx, hbm_budget=int(args.serve_budget_gb * 2 ** 30), max_batch=args.serve_max_batch,
-                     max_inflight=args.serve_inflight, min_fill=args.serve_min_fill)
+                     max_inflight=args.serve_inflight, min_fill=args.serve_min_fill, fair_share=not args.serve_fifo)
     ids = [srv.add_client(k) for _, k in clients]
     # warm the path once, then measure from a clean slate
     r0 = min(reqs, key=lambda r: len(r["ops"]))
@@ -191,7 +194,7 @@ def serving(fhe, ctx, args):
         "max_launch_size": int(st["max_batch_seen"]),
         "hbm_budget_bytes": int(st["pool_bytes"]), "peak_admitted_bytes": int(st["peak_bytes"]),
         "config": {"max_batch": args.serve_max_batch, "max_inflight": args.serve_inflight,
-                   "min_fill": args.serve_min_fill},
+                   "min_fill": args.serve_min_fill, "fair_share": not args.serve_fifo},
         "decrypt_ok": bool(ok), "alone_equals_mix": bool(alone_eq),
         "client_keygen_s": round(keygen_s, 3), "encrypt_s": round(encrypt_s, 3),
     }

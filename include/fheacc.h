@@ -288,6 +288,8 @@ typedef struct fheacc_server_config {
     int32_t max_batch;     /* most gate bootstraps per launch */
     int32_t max_inflight;  /* launches in flight at once, 1..4 */
     int32_t min_fill;      /* with a launch in flight, wait for this many ready gates */
+    int32_t fair_share;    /* 1: every job with ready gates first gets max_batch / jobs of a launch,
+                              the rest goes oldest job first; 0: oldest job first throughout */
 } fheacc_server_config;
 typedef struct fheacc_server_stats {
     int64_t jobs_submitted, jobs_done, jobs_failed, jobs_queued, jobs_active;

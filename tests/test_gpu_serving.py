@@ -148,3 +148,27 @@ def test_errors(serving):
         assert st["jobs_failed"] == 1
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("fair", [True, False])
+def test_fair_share_lets_a_small_job_pass_a_large_one(serving, fair):
+    # a large job (256 items of 16 bits: about 8000 gates, 4096 ready at
+    # once) and a small one (one 8-bit item) submitted right after it.  With
+    # fair share the small job takes half of each launch from the next launch
+    # on and finishes first; either way every word equals the dense path.
+    ctx, clients = serving
+    big, small = make_jobs(ctx, clients, [(0, 256, 16), (1, 1, 8)], 73)
+    srv = fhe.Server(ctx, hbm_budget=1 << 28, max_batch=1024, max_inflight=1, fair_share=fair)
+    ids = [srv.add_client(k) for _, k in clients]
+    try:
+        jb = srv.submit(ids[big["client"]], big["inputs"], big["ops"], big["outs"])
+        js = srv.submit(ids[small["client"]], small["inputs"], small["ops"], small["outs"])
+        out_s, lat_s = srv.wait(js)
+        out_b, lat_b = srv.wait(jb)
+    finally:
+        srv.close()
+    assert np.array_equal(ctx.lwe_decrypt(clients[big["client"]][0], out_b), big["want"])
+    assert np.array_equal(ctx.lwe_decrypt(clients[small["client"]][0], out_s), small["want"])
+    assert np.array_equal(out_s, dense_eval(ctx, clients[small["client"]][1], small))
+    if fair:
+        assert lat_s < lat_b
