@@ -49,6 +49,27 @@ def test_full_batch_truth_table_and_sample_parity(cfg0):
     assert np.array_equal(out[idx], eo)
 
 
+def test_nand_1000_every_word(cfg0):
+    # the benchmark parameters' parity case: NAND on 1000 seeded uniform bit
+    # pairs, every accumulator word after the blind rotation, every word after
+    # the key switch and after the mod switch, against the oracle (16 host
+    # threads, the GPU host's CPU share), and every decrypted bit
+    ctx, keys, s = cfg0
+    B = 1000
+    rng = np.random.default_rng(1000)
+    m1, m2 = rng.integers(0, 2, B), rng.integers(0, 2, B)
+    c1 = ctx.lwe_encrypt(s, m1, 1001, 0)
+    c2 = ctx.lwe_encrypt(s, m2, 1001, B)
+    gates = np.full(B, 3, dtype=np.int32)
+    out, acc, ks = ctx.bootstrap(keys, gates, c1, c2, capture=True)
+    assert np.array_equal(ctx.lwe_decrypt(s, out), 1 - (m1 & m2))
+    o = Oracle(**CFG0)
+    eo, ea, ek = o.bootstrap(c1, c2, gates, keys.download_bsk(), keys.download_ksk(), threads=16, capture=True)
+    assert np.array_equal(acc, ea)
+    assert np.array_equal(ks, ek)
+    assert np.array_equal(out, eo)
+
+
 def test_batch_equals_single(cfg0):
     # the same gate gives the same words alone and inside a batch
     ctx, keys, s = cfg0
