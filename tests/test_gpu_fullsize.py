@@ -9,7 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from fheacc_testlib import CFG0, ROOT, fhe
+from fheacc_testlib import CFG0, CFG1, ROOT, fhe
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -68,6 +68,30 @@ def test_nand_1000_every_word(cfg0):
     assert np.array_equal(acc, ea)
     assert np.array_equal(ks, ek)
     assert np.array_equal(out, eo)
+
+
+def test_microbench_shape_every_word():
+    # the 54-bit microbench shape at its own size: 4096 RLWE at N = 1024,
+    # BG = 2^9 d = 6, one shared RGSW; every word of the forward NTT of all
+    # 8192 polys, the inverse round trip, and all 4096 external products
+    # against the oracle
+    kw = CFG1
+    ctx, o = fhe.Context(**kw), Oracle(**kw)
+    rng = np.random.default_rng(4096)
+    B, N = 4096, o.N
+    x = rng.integers(0, o.Q, (B, 2, N), dtype=np.uint64)
+    t = torch.from_numpy(x.view(np.int64).copy()).cuda()
+    ctx.ntt_dev(t)
+    torch.cuda.synchronize()
+    assert np.array_equal(t.cpu().numpy().view(np.uint64), o.ntt(x))
+    ctx.ntt_dev(t, inverse=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(t.cpu().numpy().view(np.uint64), x)
+    rgsw = rng.integers(0, o.Q, o.rgsw_words, dtype=np.uint64)
+    out = torch.zeros_like(t)
+    ctx.external_product_dev(t, torch.from_numpy(rgsw.view(np.int64).copy()).cuda(), out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint64), o.mult_rgsw(x, rgsw))
 
 
 def test_batch_equals_single(cfg0):
