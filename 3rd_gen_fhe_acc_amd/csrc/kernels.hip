@@ -315,32 +315,44 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     __syncthreads();
     ntt_forward<LOGN, NT>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
     // gadget rows outer, the thread's coefficients inner: row h+1's key quads
-    // are in flight while row h is accumulated
+    // are in flight while row h is accumulated.  Rows alternate between two
+    // register sets (no copies), and each coefficient's digit words are read
+    // through a pointer that moves two padded polys per row, so the a and b
+    // digits are immediate offsets from it.
     MacSum<W> sa[CPT], sb[CPT];
     KeyQuad<W> cur[CPT], nxt[CPT];
+    const W *dcol[CPT];
 #pragma unroll
-    for (int c = 0; c < CPT; c++)
+    for (int c = 0; c < CPT; c++) {
+        dcol[c] = buf + pad(tid + c * NT);
         if (kFull || tid + c * NT < N) cur[c] = load_quad(key, tid + c * NT);
-#pragma unroll 1
-    for (int h = 0; h < d; h++) {
-        if (h + 1 < d) {
-#pragma unroll
-            for (int c = 0; c < CPT; c++)
-                if (kFull || tid + c * NT < N) nxt[c] = load_quad(key, ((h + 1) << LOGN) + tid + c * NT);
-        }
+    }
+    auto row = [&](const KeyQuad<W> *q) {
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
-            const int k = tid + c * NT;
-            if (kFull || k < N) {
-                const W da = buf[pad(((2 * h) << LOGN) + k)], db = buf[pad(((2 * h + 1) << LOGN) + k)];
-                sa[c].add(A, da, cur[c].v[0], cur[c].p[0]);
-                sb[c].add(A, da, cur[c].v[1], cur[c].p[1]);
-                sa[c].add(A, db, cur[c].v[2], cur[c].p[2]);
-                sb[c].add(A, db, cur[c].v[3], cur[c].p[3]);
+            if (kFull || tid + c * NT < N) {
+                const W da = dcol[c][0], db = dcol[c][PS];
+                sa[c].add(A, da, q[c].v[0], q[c].p[0]);
+                sb[c].add(A, da, q[c].v[1], q[c].p[1]);
+                sa[c].add(A, db, q[c].v[2], q[c].p[2]);
+                sb[c].add(A, db, q[c].v[3], q[c].p[3]);
             }
+            dcol[c] += 2 * PS;
         }
+    };
+    auto fetch = [&](KeyQuad<W> *q, int h) {
 #pragma unroll
-        for (int c = 0; c < CPT; c++) cur[c] = nxt[c];
+        for (int c = 0; c < CPT; c++)
+            if (kFull || tid + c * NT < N) q[c] = load_quad(key, (h << LOGN) + tid + c * NT);
+    };
+#pragma unroll 1
+    for (int h = 0; h < d; h += 2) {
+        if (h + 1 < d) fetch(nxt, h + 1);
+        row(cur);
+        if (h + 1 < d) {
+            if (h + 2 < d) fetch(cur, h + 2);
+            row(nxt);
+        }
     }
     // every digit of column k has been read by its owner: overwrite in place
 #pragma unroll
