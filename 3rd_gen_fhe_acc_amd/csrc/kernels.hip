@@ -191,28 +191,41 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
         if (!(P0 & 1) && !(np & 1) && N >= NT) {
             // whole (a, b) digit pairs: key quads of the next pair are loaded
             // while the current pair is accumulated (two sets in flight)
+            // (pairs alternate between two register sets; each coefficient's
+            // digits are immediate offsets from a pointer that advances two
+            // padded polys per pair, as in ext_prod_inplace)
+            constexpr int PS = N + N / 8;
             const int npairs = np >> 1, h0 = P0 >> 1;
             KeyQuad<W> cur[CPT], nxt[CPT];
+            const W *dcol[CPT];
 #pragma unroll
-            for (int c = 0; c < CPT; c++) cur[c] = load_quad(key, (h0 << LOGN) + tid + c * NT);
-#pragma unroll 1
-            for (int hp = 0; hp < npairs; hp++) {
-                if (hp + 1 < npairs) {
-#pragma unroll
-                    for (int c = 0; c < CPT; c++)
-                        nxt[c] = load_quad(key, ((h0 + hp + 1) << LOGN) + tid + c * NT);
-                }
+            for (int c = 0; c < CPT; c++) {
+                dcol[c] = dig + pad(tid + c * NT);
+                cur[c] = load_quad(key, (h0 << LOGN) + tid + c * NT);
+            }
+            auto pair = [&](const KeyQuad<W> *q) {
 #pragma unroll
                 for (int c = 0; c < CPT; c++) {
-                    const int k = tid + c * NT;
-                    const W da = dig[pad(((2 * hp) << LOGN) + k)], db = dig[pad(((2 * hp + 1) << LOGN) + k)];
-                    sa[c].add(A, da, cur[c].v[0], cur[c].p[0]);
-                    sb[c].add(A, da, cur[c].v[1], cur[c].p[1]);
-                    sa[c].add(A, db, cur[c].v[2], cur[c].p[2]);
-                    sb[c].add(A, db, cur[c].v[3], cur[c].p[3]);
+                    const W da = dcol[c][0], db = dcol[c][PS];
+                    sa[c].add(A, da, q[c].v[0], q[c].p[0]);
+                    sb[c].add(A, da, q[c].v[1], q[c].p[1]);
+                    sa[c].add(A, db, q[c].v[2], q[c].p[2]);
+                    sb[c].add(A, db, q[c].v[3], q[c].p[3]);
+                    dcol[c] += 2 * PS;
                 }
+            };
+            auto fetch = [&](KeyQuad<W> *q, int hp) {
 #pragma unroll
-                for (int c = 0; c < CPT; c++) cur[c] = nxt[c];
+                for (int c = 0; c < CPT; c++) q[c] = load_quad(key, ((h0 + hp) << LOGN) + tid + c * NT);
+            };
+#pragma unroll 1
+            for (int hp = 0; hp < npairs; hp += 2) {
+                if (hp + 1 < npairs) fetch(nxt, hp + 1);
+                pair(cur);
+                if (hp + 1 < npairs) {
+                    if (hp + 2 < npairs) fetch(cur, hp + 2);
+                    pair(nxt);
+                }
             }
             __syncthreads();
             continue;
