@@ -104,9 +104,10 @@ __device__ __forceinline__ W mul_monomial(const Arith<W> &A, const Twiddles<W> &
 
 template <typename W>
 struct MacSum;
-// Digit words may arrive lazily reduced (below (2 log2 N + 1) Q): the narrow
-// sum of 2 digitG such products stays below 2^64 (checked when the context
-// picks 32-bit words), and Shoup products accept any 64-bit input.
+// Digit words arrive lazily reduced (the forward transform's non-canonical
+// bound): the 32-bit sum of 2 digitG such products stays below 2^64, and the
+// 64-bit sum of products in [0, 5Q) below 2^62 (both checked when the context
+// derives its parameters); Shoup products accept any 64-bit input.
 template <>
 struct MacSum<uint32_t> {
     static constexpr bool kCanonicalInputs = false;
@@ -185,8 +186,7 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
             dig[pad(idx)] = (x >> ((P >> 1) * bits)) & mask;
         }
         __syncthreads();
-        // digit transforms stay lazy (below (2 logN + 1) Q): the MAC sums
-        // reduce them
+        // digit transforms stay lazy: the MAC sums reduce them
         ntt_forward<LOGN, NT>(dig, np, A, tw, MacSum<W>::kCanonicalInputs, tid);
         if (!(P0 & 1) && !(np & 1) && N >= NT) {
             // whole (a, b) digit pairs: key quads of the next pair are loaded

@@ -19,7 +19,8 @@
 //
 // Forward butterflies are fully lazy: no reduction between stages.  A value
 // entering stage s is below (2s+1)Q, so after log2 N stages every word is
-// below (2 log2 N + 1) Q.  When that bound does not fit 32-bit words but 4Q
+// below (2 log2 N + 1) Q; 64-bit words, whose products land in [0, 5Q), stay
+// below (5 log2 N + 1) Q.  When the 32-bit bound does not fit but 4Q
 // does (28- to 30-bit Q), the context selects Harvey's butterflies instead
 // (Arith::harvey): the added input is first brought below 2Q, so every word
 // stays below 4Q, and the last pass leaves its words below 2Q.  The last
@@ -248,8 +249,8 @@ __device__ __forceinline__ void inv_passes(W *s, int np, const Arith<W> &A, cons
 
 // Forward transform of np padded polys in LDS (canonical inputs).  Output
 // canonical in [0, Q), or, when canonical is false, below (2 logN + 1) Q
-// (below 2Q with Harvey's butterflies) and congruent to the canonical word
-// (for consumers that reduce anyway).
+// (below 2Q with Harvey's butterflies, below (5 logN + 1) Q on 64-bit words)
+// and congruent to the canonical word (for consumers that reduce anyway).
 template <int LOGN, int NT, int MAXR = 4, typename W>
 __device__ void ntt_forward(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, bool canonical = true,
                             int tid = -1) {
