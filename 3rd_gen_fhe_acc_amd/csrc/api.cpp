@@ -21,6 +21,11 @@ thread_local std::string g_err;
 
 int fheacc::fail(const std::string &m, int rc) {
     g_err = m;
+    // A failed runtime call (an allocation that does not fit, say) stays in
+    // the thread's last-error slot until read; the launchers check launches
+    // with hipGetLastError, so a later, good launch would report it.  The
+    // failure is reported here, so clear it.
+    (void)hipGetLastError();
     return rc;
 }
 
