@@ -66,6 +66,8 @@ Twiddles<W> make_tw(const fheacc_ctx *ctx) {
     t.iwp = (const W *)ctx->t.iwp;
     t.iN = (W)ctx->t.iN;
     t.iNp = (W)ctx->t.iNp;
+    t.iNw = (W)ctx->t.iNw;
+    t.iNwp = (W)ctx->t.iNwp;
     t.onep = sizeof(W) == 4 ? (W)shoup32h(1, (uint32_t)ctx->p.Q) : (W)shoup64h(1, ctx->p.Q);
     return t;
 }
@@ -247,6 +249,8 @@ int fheacc_ctx_create(const fheacc_params *params, int device, fheacc_ctx **out)
     }
     c->t.iN = p.iN;
     c->t.iNp = p.narrow ? shoup32h((uint32_t)p.iN, (uint32_t)p.Q) : shoup64h(p.iN, p.Q);
+    c->t.iNw = (uint64_t)((u128)p.iN * p.irou[1] % p.Q);
+    c->t.iNwp = p.narrow ? shoup32h((uint32_t)c->t.iNw, (uint32_t)p.Q) : shoup64h(c->t.iNw, p.Q);
     c->t.cdt_len = (int)p.cdt.size();
     if (hipMalloc(&c->t.cdt, p.cdt.size() * 8) != hipSuccess) return cleanup(fail("cdt alloc", FHEACC_ENOMEM));
     if (hipMemcpy(c->t.cdt, p.cdt.data(), p.cdt.size() * 8, hipMemcpyHostToDevice) != hipSuccess)

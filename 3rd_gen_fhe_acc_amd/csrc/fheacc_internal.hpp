@@ -49,6 +49,7 @@ __host__ __device__ inline uint64_t stream_id(uint64_t tag, uint64_t idx) { retu
 struct DeviceTables {
     void *fw = nullptr, *fwp = nullptr, *iw = nullptr, *iwp = nullptr;  // W-typed [N]
     uint64_t iN = 0, iNp = 0;
+    uint64_t iNw = 0, iNwp = 0;  // N^-1 irou[1] mod Q and its Shoup companion
     uint64_t *cdt = nullptr;  // [cdt_len]
     int cdt_len = 0;
 };

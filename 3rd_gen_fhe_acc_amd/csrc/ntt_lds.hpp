@@ -35,6 +35,7 @@ struct Twiddles {
     const W *fw, *fwp;   // rou, Shoup companions   [N]
     const W *iw, *iwp;   // irou, Shoup companions  [N]
     W iN, iNp;
+    W iNw, iNwp;         // N^-1 irou[1] (the inverse's last twiddle) and its companion
     W onep;              // Shoup companion of 1: floor(2^w / Q)
 };
 
@@ -152,8 +153,13 @@ __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, con
 #pragma unroll
     for (int e = 0; e < (1 << R); e++) x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
     int o = 0;
+    // the transform's last stage (one twiddle, irou[1]) carries the N^-1
+    // scaling: its sum goes straight into the product by N^-1 (a Shoup
+    // product takes any 32- or 64-bit input), its difference into one
+    // product by N^-1 irou[1]
+    constexpr int KS = LAST ? R - 1 : R;
 #pragma unroll
-    for (int k = 0; k < R; k++) {
+    for (int k = 0; k < KS; k++) {
         const int h = 1 << k;
 #pragma unroll
         for (int e = 0; e < (1 << R); e++) {
@@ -166,8 +172,13 @@ __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, con
         o += 1 << (R - k - 1);
     }
     if constexpr (LAST) {
+        constexpr int h = 1 << (R - 1);
 #pragma unroll
-        for (int e = 0; e < (1 << R); e++) x[e] = Arith<W>::red(A.mul_shoup(x[e], tw.iN, tw.iNp), A.Q);
+        for (int e = 0; e < h; e++) {
+            const W u = x[e], v = x[e + h];
+            x[e] = Arith<W>::red(A.mul_shoup(u + v, tw.iN, tw.iNp), A.Q);
+            x[e + h] = Arith<W>::red(A.mul_shoup(u - v + A.Q2, tw.iNw, tw.iNwp), A.Q);
+        }
     }
 #pragma unroll
     for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
