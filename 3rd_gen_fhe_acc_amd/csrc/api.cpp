@@ -54,6 +54,11 @@ Arith<W> make_arith(const HostParams &p) {
     A.mu = (uint64_t)(((u128)1 << 64) / p.Q);
     if constexpr (sizeof(W) == 4) A.harvey = p.harvey;
     else A.Q5 = 5 * p.Q;
+    // the largest lazy inverse pass, 2^(lzr + 1) Q <= 2^w (passes have at
+    // most 8 stages)
+    const u128 top = (u128)1 << (8 * sizeof(W));
+    A.lzr = 0;
+    while (A.lzr < 8 && ((u128)p.Q << (A.lzr + 2)) <= top) A.lzr++;
     return A;
 }
 

@@ -8,8 +8,9 @@
 //                    __umul64hi.
 // Every result that leaves a kernel is canonical in [0, Q), the reference's
 // representation (FHE_ACC_runtime/include/nbt.h:59-101); intermediate values
-// are kept lazily in [0, 2Q) or [0, 4Q) (Harvey's butterflies), which is exact
-// arithmetic mod Q and so does not change any canonical output.
+// are kept lazily below small multiples of Q (each transform states its
+// bounds), which is exact arithmetic mod Q and so does not change any
+// canonical output.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -27,6 +28,9 @@ struct Arith<uint32_t> {
     // Harvey's forward butterflies (words below 4Q) instead of fully lazy
     // ones: Q too large for (2 log2 N + 1) Q < 2^32, small enough for 4Q
     bool harvey;
+    // inverse passes of up to lzr stages leave their sums unreduced:
+    // 2^(lzr + 1) Q <= 2^32
+    int lzr;
 
     // [0, 2m) -> [0, m): if x < m the subtraction wraps above x
     __device__ __forceinline__ static W red(W x, W m) {
@@ -66,6 +70,7 @@ struct Arith<uint64_t> {
     W Q, Q2;
     uint64_t mu;  // unused for the wide path (sums are reduced from [0, 10dQ))
     W Q5;         // 5Q: the bound of mul_shoup_lazy
+    int lzr;      // as for 32-bit words: 2^(lzr + 1) Q <= 2^64
 
     __device__ __forceinline__ static W red(W x, W m) {
         W y = x - m;

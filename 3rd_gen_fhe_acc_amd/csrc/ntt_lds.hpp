@@ -145,7 +145,21 @@ __device__ __forceinline__ void fwd_cluster(W *s, int b0, bool last, const Arith
     for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
 }
 
-template <int S, int R, bool LAST, typename W>
+// Lazy inverse passes leave the butterfly sums unreduced.  Within a pass the
+// bound of every register is known at compile time: with inputs below 2Q,
+// element e after stage k is below 2^(1 + z) Q, z = lazy_run(e, k), the run
+// of zero bits of e from bit k down (a set bit means the word came out of a
+// Shoup product, below 2Q; a zero bit means a sum, twice the bound of its
+// operands, which share the bits below k and so their bound).
+__host__ __device__ constexpr int lazy_run(int e, int k) {
+    int z = 0;
+    for (int b = k; b >= 0 && !((e >> b) & 1); b--) z++;
+    return z;
+}
+
+// LZ: the lazy form (needs 2^(R+1) Q within the word); otherwise every sum is
+// reduced to [0, 2Q) at once.
+template <int S, int R, bool LAST, bool LZ, typename W>
 __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, const Twiddles<W> &tw,
                                             const ClusterTwiddles<R, W> &t) {
     W *base = s + pad(b0);
@@ -153,6 +167,44 @@ __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, con
 #pragma unroll
     for (int e = 0; e < (1 << R); e++) x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
     int o = 0;
+    if constexpr (LZ) {
+        constexpr int KS = LAST ? R - 1 : R;
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            const int h = 1 << k;
+#pragma unroll
+            for (int e = 0; e < (1 << R); e++) {
+                if (e & h) continue;
+                const int j = o + (e >> (k + 1));
+                const W c = A.Q << (1 + lazy_run(e, k - 1));  // u, v < c
+                const W u = x[e], v = x[e + h];
+                x[e] = u + v;
+                x[e + h] = A.mul_shoup(u - v + c, t.w[j], t.wp[j]);
+            }
+            o += 1 << (R - k - 1);
+        }
+        if constexpr (LAST) {
+            constexpr int h = 1 << (R - 1);
+#pragma unroll
+            for (int e = 0; e < h; e++) {
+                const W c = A.Q << (1 + lazy_run(e, R - 2));
+                const W u = x[e], v = x[e + h];
+                x[e] = Arith<W>::red(A.mul_shoup(u + v, tw.iN, tw.iNp), A.Q);
+                x[e + h] = Arith<W>::red(A.mul_shoup(u - v + c, tw.iNw, tw.iNwp), A.Q);
+            }
+        } else {
+            // back below 2Q for the next pass
+#pragma unroll
+            for (int e = 0; e < (1 << R); e++) {
+                const int z = lazy_run(e, R - 1);
+                if (z == 1) x[e] = Arith<W>::red(x[e], A.Q2);
+                else if (z > 1) x[e] = A.mul_shoup(x[e], (W)1, tw.onep);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
+        return;
+    }
     // the transform's last stage (one twiddle, irou[1]) carries the N^-1
     // scaling: its sum goes straight into the product by N^-1 (a Shoup
     // product takes any 32- or 64-bit input), its difference into one
@@ -215,7 +267,7 @@ __device__ __forceinline__ void fwd_pass(W *s, int np, bool last, const Arith<W>
 
 // Inverse pass: GS stages U0 .. U0+R-1 (half-size 2^u).  Inputs and outputs in
 // [0, 2Q).  When last, the words are scaled by N^-1 and made canonical.
-template <int LOGN, int U0, int R, int NT, typename W>
+template <int LOGN, int U0, int R, int NT, bool LZ, typename W>
 __device__ __forceinline__ void inv_pass(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid) {
     constexpr int N = 1 << LOGN;
     constexpr int S = 1 << U0;
@@ -226,13 +278,14 @@ __device__ __forceinline__ void inv_pass(W *s, int np, const Arith<W> &A, const 
         const int blk = c >> U0, off = c & (S - 1);
         const ClusterTwiddles<R, W> t = inv_twiddles<LOGN, U0, R>(blk, tw);
         for (int p = tid / per_poly; p < np; p += NT / per_poly)
-            inv_cluster<S, R, last>(s, p * N + (blk << (U0 + R)) + off, A, tw, t);
+            inv_cluster<S, R, last, LZ>(s, p * N + (blk << (U0 + R)) + off, A, tw, t);
     } else {
         const int clusters = np * per_poly;
         for (int cg = tid; cg < clusters; cg += NT) {
             const int p = cg / per_poly, c = cg - p * per_poly;
             const int blk = c >> U0, off = c & (S - 1);
-            inv_cluster<S, R, last>(s, p * N + (blk << (U0 + R)) + off, A, tw, inv_twiddles<LOGN, U0, R>(blk, tw));
+            inv_cluster<S, R, last, LZ>(s, p * N + (blk << (U0 + R)) + off, A, tw,
+                                        inv_twiddles<LOGN, U0, R>(blk, tw));
         }
     }
 }
@@ -252,7 +305,8 @@ template <int LOGN, int ST, int NT, int MAXR, typename W>
 __device__ __forceinline__ void inv_passes(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid) {
     if constexpr (ST < LOGN) {
         constexpr int R = last_pass_stages<MAXR>(LOGN - ST);
-        inv_pass<LOGN, ST, R, NT>(s, np, A, tw, tid);
+        if (R <= A.lzr) inv_pass<LOGN, ST, R, NT, true>(s, np, A, tw, tid);
+        else inv_pass<LOGN, ST, R, NT, false>(s, np, A, tw, tid);
         __syncthreads();
         inv_passes<LOGN, ST + R, NT, MAXR>(s, np, A, tw, tid);
     }
