@@ -46,7 +46,8 @@ class _Info(C.Structure):
                 ("narrow", C.c_uint32), ("harvey", C.c_uint32), ("gate_constant", C.c_uint64 * 6),
                 ("bsk_words", C.c_uint64),
                 ("ksk_words", C.c_uint64), ("subs_words", C.c_uint64), ("bsk_device_bytes", C.c_uint64),
-                ("ksk_device_bytes", C.c_uint64), ("subs_device_bytes", C.c_uint64)]
+                ("ksk_device_bytes", C.c_uint64), ("subs_device_bytes", C.c_uint64),
+                ("lazy_inverse_stages", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class _Capture(C.Structure):
@@ -181,7 +182,7 @@ class Context:
         self.device = device
         info = _Info()
         _check(lib().fheacc_ctx_info(self.h, C.byref(info)))
-        self.info = {f: getattr(info, f) for f, _ in _Info._fields_ if f != "gate_constant"}
+        self.info = {f: getattr(info, f) for f, _ in _Info._fields_ if f not in ("gate_constant", "reserved")}
         self.info["gate_constant"] = list(info.gate_constant)
         self.n, self.q, self.N, self.Q = n, q, N, info.Q
         self.digitG, self.digitR, self.digitKS, self.digitKS_R = info.digitG, info.digitR, info.digitKS, info.digitKS_R

@@ -46,6 +46,15 @@ ShoupGen make_sg(uint64_t Q) {
     return s;
 }
 
+// The largest inverse NTT pass whose sums may stay unreduced: 2^(R+1) Q <=
+// 2^bits (passes have at most 8 stages).
+static int lazy_inverse_stages(uint64_t Q, int bits) {
+    const u128 top = (u128)1 << bits;
+    int r = 0;
+    while (r < 8 && ((u128)Q << (r + 2)) <= top) r++;
+    return r;
+}
+
 template <typename W>
 Arith<W> make_arith(const HostParams &p) {
     Arith<W> A;
@@ -54,11 +63,7 @@ Arith<W> make_arith(const HostParams &p) {
     A.mu = (uint64_t)(((u128)1 << 64) / p.Q);
     if constexpr (sizeof(W) == 4) A.harvey = p.harvey;
     else A.Q5 = 5 * p.Q;
-    // the largest lazy inverse pass, 2^(lzr + 1) Q <= 2^w (passes have at
-    // most 8 stages)
-    const u128 top = (u128)1 << (8 * sizeof(W));
-    A.lzr = 0;
-    while (A.lzr < 8 && ((u128)p.Q << (A.lzr + 2)) <= top) A.lzr++;
+    A.lzr = lazy_inverse_stages(p.Q, 8 * sizeof(W));
     return A;
 }
 
@@ -289,6 +294,8 @@ int fheacc_ctx_info(const fheacc_ctx *c, fheacc_info *o) {
     o->digitKS_R = p.digitKS_R;
     o->narrow = p.narrow;
     o->harvey = p.harvey;
+    o->lazy_inverse_stages = (uint32_t)lazy_inverse_stages(p.Q, p.narrow ? 32 : 64);
+    o->reserved = 0;
     for (int g = 0; g < 6; g++) o->gate_constant[g] = p.gate_constant[g];
     o->bsk_words = (uint64_t)p.Br * p.digitR * p.n * p.digitG * 4 * p.N;
     o->ksk_words = (uint64_t)p.Bks * p.digitKS * p.N * (p.n + 1);

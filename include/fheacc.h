@@ -64,6 +64,9 @@ typedef struct fheacc_info {
     uint64_t ksk_words;     /* words of the reference-layout LWE ks key */
     uint64_t subs_words;    /* words of the reference-layout subs key */
     uint64_t bsk_device_bytes, ksk_device_bytes, subs_device_bytes;
+    uint32_t lazy_inverse_stages; /* inverse NTT passes of up to this many stages leave their
+                                     sums unreduced: 2^(stages+1) Q fits the device word */
+    uint32_t reserved;
 } fheacc_info;
 
 /* Capture buffers for intermediate words (each may be NULL).  Host or device

@@ -30,6 +30,10 @@ SETS = {
     "n8_N2048_q29": dict(n=8, q=1024, N=2048, Q=0, nbit=29, BGbits=10, Br=32, Bks=25),
     # 30-bit Q at N = 1024, at the 4Q < 2^32 edge of 32-bit words
     "n8_N1024_q30": dict(n=8, q=1024, N=1024, Q=0, nbit=30, BGbits=10, Br=32, Bks=32),
+    # 28-bit Q below 2^32 / 21 at N = 1024: fully lazy 32-bit forward
+    # butterflies, but 16 Q > 2^32 / 2, so the inverse transform's 3-stage
+    # passes leave their sums unreduced and its 4-stage pass does not
+    "n8_N1024_q28": dict(n=8, q=1024, N=1024, Q=0xBEB8801, nbit=28, BGbits=7, Br=32, Bks=32),
 }
 
 # full-size configurations

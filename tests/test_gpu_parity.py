@@ -30,6 +30,7 @@ def pair(name):
 
 @pytest.mark.parametrize("N,nbit,Q", [(16, 27, 0), (256, 27, 0), (512, 27, 0), (1024, 27, 0), (2048, 27, 0),
                                       (1024, 29, 0), (2048, 29, 0),  # 32-bit words, Harvey's butterflies
+                                      (1024, 28, 0xBEB8801),  # lazy 3-stage and strict 4-stage inverse passes
                                       (128, 54, 0x3FFFFFFFFED001), (1024, 54, 0x3FFFFFFFFED001),
                                       (2048, 54, 0x3FFFFFFFFED001)])
 def test_ntt_matches_oracle(N, nbit, Q):
@@ -47,13 +48,16 @@ def test_ntt_matches_oracle(N, nbit, Q):
     assert np.array_equal(host(t2), o.intt(x))
 
 
-@pytest.mark.parametrize("name,narrow,harvey", [("n12_N1024_q27", 1, 0), ("n8_N2048_q29", 1, 1),
-                                               ("n8_N1024_q30", 1, 1), ("n8_N2048_q54", 0, 0)])
-def test_word_width_selection(name, narrow, harvey):
-    # 27-bit Q: fully lazy 32-bit butterflies; 29/30-bit Q: 32-bit words with
-    # Harvey's butterflies; 54-bit Q: 64-bit words
+@pytest.mark.parametrize("name,narrow,harvey,lazy_inv", [("n12_N1024_q27", 1, 0, 4), ("n8_N1024_q28", 1, 0, 3),
+                                                        ("n8_N2048_q29", 1, 1, 2), ("n8_N1024_q30", 1, 1, 1),
+                                                        ("n8_N2048_q54", 0, 0, 8)])
+def test_word_width_selection(name, narrow, harvey, lazy_inv):
+    # 27- and 28-bit Q: fully lazy 32-bit butterflies; 29/30-bit Q: 32-bit
+    # words with Harvey's butterflies; 54-bit Q: 64-bit words.  Inverse NTT
+    # passes of up to lazy_inv stages leave their sums unreduced (the
+    # 1024-point inverse runs passes of 3, 3 and 4 stages)
     info = fhe.Context(**SETS[name]).info
-    assert (info["narrow"], info["harvey"]) == (narrow, harvey)
+    assert (info["narrow"], info["harvey"], info["lazy_inverse_stages"]) == (narrow, harvey, lazy_inv)
 
 
 def test_external_product_golden(golden_dir):
@@ -70,7 +74,8 @@ def test_external_product_golden(golden_dir):
                                         (2048, 0x3FFFFFFFFED001, 18),   # set B, d=3
                                         (512, 0, 6),                    # set C, d=5
                                         (2048, 0x1FFF2001, 10),         # 29-bit, Harvey's butterflies
-                                        (1024, 0x3FFF7801, 10)])        # 30-bit, Harvey's butterflies
+                                        (1024, 0x3FFF7801, 10),         # 30-bit, Harvey's butterflies
+                                        (1024, 0xBEB8801, 7)])          # 28-bit, lazy and strict inverse passes
 @pytest.mark.parametrize("per_ct", [False, True])
 def test_external_product_matches_oracle(N, Q, BGbits, per_ct):
     kw = dict(n=4, q=1024, N=N, Q=Q, nbit=27, BGbits=BGbits)
@@ -145,7 +150,7 @@ def run_bootstrap_case(name, B, seed, upload=False):
 
 
 @pytest.mark.parametrize("name", ["n16_N256_q27", "n12_N512_q27", "n10_N128_q54", "n8_N2048_q54",
-                                  "n8_N512_q27_bg6", "n8_N2048_q29", "n8_N1024_q30"])
+                                  "n8_N512_q27_bg6", "n8_N2048_q29", "n8_N1024_q30", "n8_N1024_q28"])
 def test_bootstrap_matches_oracle(name):
     run_bootstrap_case(name, B=24, seed=3)
 
