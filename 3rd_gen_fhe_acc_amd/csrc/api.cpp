@@ -1404,7 +1404,7 @@ int fheacc_ctx_set_tuning(fheacc_ctx *c, int threads, int pb) {
     if (check_ctx(c)) return -1;
     if (threads != 128 && threads != 256 && threads != 512 && threads != 1024)
         return fail("threads must be 128, 256, 512 or 1024");
-    if (threads == 1024 && (c->p.N < 1024 || !c->p.narrow)) return fail("1024 threads need N >= 1024 and Q < 2^30");
+    if (threads == 1024 && c->p.N < 1024) return fail("1024 threads need N >= 1024");
     if (pb < 0 || pb > 2 * (int)c->p.digitG) return fail("polys per batch out of range");
     std::lock_guard<std::mutex> lk(c->mu);
     c->tune.br_threads = threads;
@@ -1498,7 +1498,7 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
     int best_t = saved.br_threads, best_pb = saved.br_pb;
     bool first = true;
     for (int threads : {128, 256, 512, 1024}) {
-        if (threads > p.N || (threads == 1024 && (p.N < 1024 || !p.narrow))) continue;
+        if (threads > p.N || (threads == 1024 && p.N < 1024)) continue;
         for (int pb = 2; pb <= 2 * (int)p.digitG; pb += 2) {
             const size_t lds = br_lds_bytes(p, pb);
             if (lds > c->lds_budget) continue;
@@ -1522,7 +1522,7 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
     int best_ts = best_t;
     std::string rep_s;
     for (int threads : {128, 256, 512, 1024}) {
-        if (threads > p.N || Bs <= 0 || (threads == 1024 && (p.N < 1024 || !p.narrow))) continue;
+        if (threads > p.N || Bs <= 0 || (threads == 1024 && p.N < 1024)) continue;
         double br = 0;
         if (int rc = time_shape(threads, best_pb, Bs, &br)) return restore(rc);
         rep_s += std::string(rep_s.empty() ? "" : ", ") + "{\"threads\": " + std::to_string(threads) +

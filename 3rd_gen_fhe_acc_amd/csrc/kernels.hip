@@ -287,7 +287,10 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     // with N a multiple of NT every k = tid + c NT is a coefficient; the
     // opaque tid hides that from the compiler, so say it here
     constexpr bool kFull = N % NT == 0;
-    ntt_inverse<LOGN, NT>(buf, 2, A, tw, tid);
+    // 64-bit words at 1024 threads: 3-stage passes keep a cluster and its
+    // twiddles within 128 VGPRs
+    constexpr int MAXR = (sizeof(W) == 8 && NT >= 1024) ? 3 : 4;
+    ntt_inverse<LOGN, NT, MAXR>(buf, 2, A, tw, tid);
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
     W xa[CPT], xb[CPT];
 #pragma unroll
@@ -326,7 +329,7 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
         }
     }
     __syncthreads();
-    ntt_forward<LOGN, NT>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
+    ntt_forward<LOGN, NT, MAXR>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
     // gadget rows outer, the thread's coefficients inner: row h+1's key quads
     // are in flight while row h is accumulated.  Rows alternate between two
     // register sets (no copies), and each coefficient's digit words are read
@@ -358,13 +361,23 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
         for (int c = 0; c < CPT; c++)
             if (kFull || tid + c * NT < N) q[c] = load_quad(key, (h << LOGN) + tid + c * NT);
     };
+    if constexpr (sizeof(W) == 8 && NT >= 1024) {
+        // 64-bit words at 1024 threads: one register set of key quads (two
+        // would not fit in 128 VGPRs); the 16 waves per CU hide the loads
 #pragma unroll 1
-    for (int h = 0; h < d; h += 2) {
-        if (h + 1 < d) fetch(nxt, h + 1);
-        row(cur);
-        if (h + 1 < d) {
-            if (h + 2 < d) fetch(cur, h + 2);
-            row(nxt);
+        for (int h = 0; h < d; h++) {
+            if (h) fetch(cur, h);
+            row(cur);
+        }
+    } else {
+#pragma unroll 1
+        for (int h = 0; h < d; h += 2) {
+            if (h + 1 < d) fetch(nxt, h + 1);
+            row(cur);
+            if (h + 1 < d) {
+                if (h + 2 < d) fetch(cur, h + 2);
+                row(nxt);
+            }
         }
     }
     // every digit of column k has been read by its owner: overwrite in place
@@ -1281,8 +1294,7 @@ hipError_t launch_blind_rotate(const BRArgs<W> &a, int logN, int nt, int64_t B, 
     FHEACC_LOGN_SWITCH(logN, {
         if (nt == 128 && LOGN >= 7) return launch_br_nt<W, LOGN, 128>(a, B, lds, st);
         if (nt == 512 && LOGN >= 9) return launch_br_nt<W, LOGN, 512>(a, B, lds, st);
-        // 1024 threads: 32-bit words only (64-bit ones spill at 128 registers)
-        if constexpr (sizeof(W) == 4 && LOGN >= 10)
+        if constexpr (LOGN >= 10)
             if (nt == 1024) return launch_br_nt<W, LOGN, 1024>(a, B, lds, st);
         return launch_br_nt<W, LOGN, 256>(a, B, lds, st);
     })

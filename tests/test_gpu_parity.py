@@ -155,7 +155,8 @@ def test_bootstrap_matches_oracle(name):
     run_bootstrap_case(name, B=24, seed=3)
 
 
-@pytest.mark.parametrize("name,shapes", [("n12_N512_q27", (128, 256, 512)), ("n12_N1024_q27", (256, 1024))])
+@pytest.mark.parametrize("name,shapes", [("n12_N512_q27", (128, 256, 512)), ("n12_N1024_q27", (256, 1024)),
+                                         ("n8_N2048_q54", (256, 512, 1024))])
 def test_every_launch_shape_bit_exact(name, shapes):
     # the autotuner may pick any shape: all must give the same words
     ctx, o = pair(name)
@@ -172,7 +173,7 @@ def test_every_launch_shape_bit_exact(name, shapes):
     rg = rng.integers(0, o.Q, o.rgsw_words, dtype=np.uint64)
     cm = np.stack([o.cmux(x0[i], x1[i], rg) for i in range(3)])
     for threads in shapes:
-        for pb in (1, 2, 3, 4, 8):
+        for pb in sorted({1, 2, 3, 4, 2 * ctx.digitG}):  # pb = 2d: the in-place product
             ctx.set_tuning(threads, pb)
             assert np.array_equal(ctx.bootstrap(keys, gates, c1, c2), exp), (threads, pb)
             out = torch.zeros((3, 2, o.N), dtype=torch.int64, device="cuda")
