@@ -753,7 +753,10 @@ __global__ __launch_bounds__(NT) void k_ntt(NTTArgs<W> a) {
     for (int i = threadIdx.x; i < np * N; i += NT) g[i] = s[pad(i)];
 }
 
-template <typename W, int LOGN, int NT>
+// INPLACE (pb >= 2d): the product runs in the 2d digit polys, the RLWE in
+// the first two of them, as in the blind rotation; otherwise acc (2 polys)
+// plus a pb-poly digit buffer.
+template <typename W, int LOGN, int NT, bool INPLACE>
 __global__ __launch_bounds__(NT) void k_external_product(EPArgs<W> a) {
     constexpr int N = 1 << LOGN;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -763,7 +766,8 @@ __global__ __launch_bounds__(NT) void k_external_product(EPArgs<W> a) {
     const uint64_t *src = a.acc + c * 2 * N;
     for (int i = threadIdx.x; i < 2 * N; i += NT) acc[pad(i)] = (W)src[i];
     __syncthreads();
-    ext_prod<W, LOGN, NT>(acc, dig, a.key + c * a.key_stride, a.A, a.tw, a.d, a.bits, a.pb);
+    if constexpr (INPLACE) ext_prod_inplace<W, LOGN, NT>(acc, a.key + c * a.key_stride, a.A, a.tw, a.d, a.bits);
+    else ext_prod<W, LOGN, NT>(acc, dig, a.key + c * a.key_stride, a.A, a.tw, a.d, a.bits, a.pb);
     uint64_t *dst = a.out + c * 2 * N;
     for (int i = threadIdx.x; i < 2 * N; i += NT) dst[i] = acc[pad(i)];
 }
@@ -1340,16 +1344,29 @@ hipError_t launch_ntt(const NTTArgs<W> &a, int logN, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <typename W, int LOGN, int NT, bool INPLACE>
+static hipError_t launch_ep_nt(const EPArgs<W> &a, int64_t count, size_t lds, hipStream_t st) {
+    auto k = k_external_product<W, LOGN, NT, INPLACE>;
+    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(NT), lds, st, a);
+    return hipGetLastError();
+}
+
+// One workgroup per product.  All 2d digit polys fit the LDS at every
+// supported shape but one workgroup per CU at large N, so the in-place
+// product runs 1024 threads from N = 1024 up (16 waves per CU instead of 4).
 template <typename W>
 hipError_t launch_external_product(const EPArgs<W> &a, int logN, int64_t count, hipStream_t st) {
-    const size_t lds = (size_t)(padded(2 << logN) + padded(a.pb << logN)) * sizeof(W);
+    const bool inplace = a.pb >= 2 * a.d;
+    const size_t lds = inplace ? (size_t)padded(std::max(2, 2 * a.d) << logN) * sizeof(W)
+                               : (size_t)(padded(2 << logN) + padded(a.pb << logN)) * sizeof(W);
     FHEACC_LOGN_SWITCH(logN, {
-        auto k = k_external_product<W, LOGN, 256>;
-        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(256), lds, st, a);
+        if (!inplace) return launch_ep_nt<W, LOGN, 256, false>(a, count, lds, st);
+        if constexpr (LOGN >= 10) return launch_ep_nt<W, LOGN, 1024, true>(a, count, lds, st);
+        else return launch_ep_nt<W, LOGN, 256, true>(a, count, lds, st);
     })
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 template <typename W>
