@@ -786,6 +786,8 @@ template <typename W, int LOGN, int NT, bool EXPAND>
 __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
     constexpr int N = 1 << LOGN;
     constexpr int CPT = (N + NT - 1) / NT;
+    // at 1024 threads 3-stage NTT passes keep within 128 registers
+    constexpr int MAXR = NT >= 1024 ? 3 : 4;
     extern __shared__ __align__(16) unsigned char smem[];
     W *x = reinterpret_cast<W *>(smem);  // input, time domain (2 padded polys)
     W *y = x + padded(2 * N);            // substituted (2 padded polys)
@@ -796,7 +798,7 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
     const uint64_t *src = a.in + row * 2 * N;
     for (int i = threadIdx.x; i < 2 * N; i += NT) x[pad(i)] = (W)src[i];
     __syncthreads();
-    ntt_inverse<LOGN, NT>(x, 2, a.A, a.tw);
+    ntt_inverse<LOGN, NT, MAXR>(x, 2, a.A, a.tw);
     const int power = (N >> a.sub_factor) + 1;
     for (int idx = threadIdx.x; idx < 2 * N; idx += NT) {
         const int half = idx >> LOGN, i = idx & (N - 1);
@@ -809,7 +811,7 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
     // output b starts from NTT(subs(b)); the digits come from subs(a)
     for (int i = threadIdx.x; i < N; i += NT) x[pad(N + i)] = y[pad(N + i)];
     __syncthreads();
-    ntt_forward<LOGN, NT>(x + pad(N), 1, a.A, a.tw);
+    ntt_forward<LOGN, NT, MAXR>(x + pad(N), 1, a.A, a.tw);
     MacSum<W> sa[CPT], sb[CPT];
     const int d = a.d, bits = a.bits, dch = a.dch;
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
@@ -821,7 +823,7 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
             for (int hh = 0; hh < nd; hh++) dig[pad((hh << LOGN) + k)] = (v >> ((h0 + hh) * bits)) & mask;
         }
         __syncthreads();
-        ntt_forward<LOGN, NT>(dig, nd, a.A, a.tw, MacSum<W>::kCanonicalInputs);
+        ntt_forward<LOGN, NT, MAXR>(dig, nd, a.A, a.tw, MacSum<W>::kCanonicalInputs);
 #pragma unroll
         for (int cc = 0; cc < CPT; cc++) {
             const int k = threadIdx.x + cc * NT;
@@ -1369,16 +1371,24 @@ hipError_t launch_external_product(const EPArgs<W> &a, int logN, int64_t count, 
     return hipSuccess;
 }
 
+// One workgroup per RLWE.  From N = 1024 up the LDS (4 polys plus the digit
+// batch) leaves one workgroup per CU, so it runs 1024 threads (16 waves per
+// CU instead of 4).
+template <typename W, int LOGN, bool EXPAND>
+static hipError_t launch_subs_nt(const SubsArgs<W> &a, int64_t count, size_t lds, hipStream_t st) {
+    constexpr int NT = LOGN >= 10 ? 1024 : 256;
+    auto k = k_subs_keyswitch<W, LOGN, NT, EXPAND>;
+    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(NT), lds, st, a);
+    return hipGetLastError();
+}
+
 template <typename W>
 hipError_t launch_subs_keyswitch(const SubsArgs<W> &a, int logN, int64_t count, hipStream_t st) {
     const size_t lds = (size_t)(2 * padded(2 << logN) + padded(a.dch << logN)) * sizeof(W);
-    FHEACC_LOGN_SWITCH(logN, {
-        auto k = k_subs_keyswitch<W, LOGN, 256, false>;
-        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(256), lds, st, a);
-    })
-    return hipGetLastError();
+    FHEACC_LOGN_SWITCH(logN, { return launch_subs_nt<W, LOGN, false>(a, count, lds, st); })
+    return hipSuccess;
 }
 
 template <typename W>
@@ -1386,13 +1396,8 @@ hipError_t launch_expand_level(const SubsArgs<W> &a, int logN, int64_t B, hipStr
     const size_t lds = (size_t)(2 * padded(2 << logN) + padded(a.dch << logN)) * sizeof(W);
     const int64_t count = B << a.sub_factor;
     if (count > 0x7fffffff) return hipErrorInvalidValue;
-    FHEACC_LOGN_SWITCH(logN, {
-        auto k = k_subs_keyswitch<W, LOGN, 256, true>;
-        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(256), lds, st, a);
-    })
-    return hipGetLastError();
+    FHEACC_LOGN_SWITCH(logN, { return launch_subs_nt<W, LOGN, true>(a, count, lds, st); })
+    return hipSuccess;
 }
 
 template <typename W, int LOGN, int NT>
