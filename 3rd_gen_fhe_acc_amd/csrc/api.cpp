@@ -1242,7 +1242,8 @@ static int cmux_impl(fheacc_ctx *c, const uint64_t *c0, const uint64_t *c1, cons
     const size_t lds = br_lds_bytes(p, a.pb);
     if (lds > c->lds_budget) return fail("parameter set does not fit the LDS budget");
     Prof pr(c, 2);
-    HIP_TRY(launch_cmux_chain<W>(a, p.logN, c->tune.br_threads, count, lds, c->stream));
+    const int nt = c->tune.shape_set || p.N < 1024 ? c->tune.br_threads : 1024;
+    HIP_TRY(launch_cmux_chain<W>(a, p.logN, nt, count, lds, c->stream));
     return 0;
 }
 
@@ -1410,6 +1411,7 @@ int fheacc_ctx_set_tuning(fheacc_ctx *c, int threads, int pb) {
     c->tune.br_threads = threads;
     c->tune.br_pb = pb;
     c->tune.br_threads_small = 0;
+    c->tune.shape_set = true;
     return 0;
 }
 

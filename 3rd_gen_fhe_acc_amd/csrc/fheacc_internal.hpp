@@ -54,7 +54,7 @@ struct DeviceTables {
     int cdt_len = 0;
 };
 
-// Tuning knobs picked per parameter set by the autotuner (tune.cpp).
+// Tuning knobs picked per parameter set by the autotuner (fheacc_autotune).
 struct Tuning {
     int br_threads = 256;     // threads per blind-rotation workgroup
     int br_pb = 0;            // digit polys per NTT batch (0: as many as fit)
@@ -67,6 +67,10 @@ struct Tuning {
     // which shortens each gate (0: br_threads).  Set by the autotuner.
     int br_threads_small = 0;
     int64_t small_max = 0;
+    // fheacc_ctx_set_tuning was called: the CMUX chains take br_threads too.
+    // Otherwise they run 1024 threads from N = 1024 up (one workgroup fills
+    // a CU's LDS there, and 1024 threads give it 16 waves).
+    bool shape_set = false;
 };
 
 // Error text of the calling thread (fheacc_last_error) and the C ABI's

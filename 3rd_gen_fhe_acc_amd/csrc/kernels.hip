@@ -174,7 +174,11 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
     // every pass out of that loop and runs out of registers.
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
-    ntt_inverse<LOGN, NT>(acc, 2, A, tw, tid);
+    // 64-bit words at 1024 threads: 3-stage passes and one set of key quads
+    // keep within 128 registers
+    constexpr bool kLean = sizeof(W) == 8 && NT >= 1024;
+    constexpr int MAXR = kLean ? 3 : 4;
+    ntt_inverse<LOGN, NT, MAXR>(acc, 2, A, tw, tid);
     MacSum<W> sa[CPT], sb[CPT];
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
     const int total = 2 * d;
@@ -187,7 +191,7 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
         }
         __syncthreads();
         // digit transforms stay lazy: the MAC sums reduce them
-        ntt_forward<LOGN, NT>(dig, np, A, tw, MacSum<W>::kCanonicalInputs, tid);
+        ntt_forward<LOGN, NT, MAXR>(dig, np, A, tw, MacSum<W>::kCanonicalInputs, tid);
         if (!(P0 & 1) && !(np & 1) && N >= NT) {
             // whole (a, b) digit pairs: key quads of the next pair are loaded
             // while the current pair is accumulated (two sets in flight)
@@ -218,13 +222,21 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
 #pragma unroll
                 for (int c = 0; c < CPT; c++) q[c] = load_quad(key, ((h0 + hp) << LOGN) + tid + c * NT);
             };
+            if constexpr (kLean) {
 #pragma unroll 1
-            for (int hp = 0; hp < npairs; hp += 2) {
-                if (hp + 1 < npairs) fetch(nxt, hp + 1);
-                pair(cur);
-                if (hp + 1 < npairs) {
-                    if (hp + 2 < npairs) fetch(cur, hp + 2);
-                    pair(nxt);
+                for (int hp = 0; hp < npairs; hp++) {
+                    if (hp) fetch(cur, hp);
+                    pair(cur);
+                }
+            } else {
+#pragma unroll 1
+                for (int hp = 0; hp < npairs; hp += 2) {
+                    if (hp + 1 < npairs) fetch(nxt, hp + 1);
+                    pair(cur);
+                    if (hp + 1 < npairs) {
+                        if (hp + 2 < npairs) fetch(cur, hp + 2);
+                        pair(nxt);
+                    }
                 }
             }
             __syncthreads();
@@ -1415,7 +1427,7 @@ hipError_t launch_cmux_chain(const CmuxArgs<W> &a, int logN, int nt, int64_t cou
     FHEACC_LOGN_SWITCH(logN, {
         if (nt == 128 && LOGN >= 7) return launch_cmux_nt<W, LOGN, 128>(a, count, lds, st);
         if (nt == 512 && LOGN >= 9) return launch_cmux_nt<W, LOGN, 512>(a, count, lds, st);
-        if constexpr (sizeof(W) == 4 && LOGN >= 10)
+        if constexpr (LOGN >= 10)
             if (nt == 1024) return launch_cmux_nt<W, LOGN, 1024>(a, count, lds, st);
         return launch_cmux_nt<W, LOGN, 256>(a, count, lds, st);
     })
