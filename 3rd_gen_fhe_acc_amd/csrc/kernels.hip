@@ -299,9 +299,10 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     // with N a multiple of NT every k = tid + c NT is a coefficient; the
     // opaque tid hides that from the compiler, so say it here
     constexpr bool kFull = N % NT == 0;
-    // 64-bit words at 1024 threads: 3-stage passes keep a cluster and its
-    // twiddles within 128 VGPRs
-    constexpr int MAXR = (sizeof(W) == 8 && NT >= 1024) ? 3 : 4;
+    // 64-bit words at 1024 threads: 3-stage passes (a cluster and its
+    // twiddles) and one set of key quads keep within 128 VGPRs
+    constexpr bool kLean = sizeof(W) == 8 && NT >= 1024;
+    constexpr int MAXR = kLean ? 3 : 4;
     ntt_inverse<LOGN, NT, MAXR>(buf, 2, A, tw, tid);
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
     W xa[CPT], xb[CPT];
@@ -344,9 +345,9 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     ntt_forward<LOGN, NT, MAXR>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
     // gadget rows outer, the thread's coefficients inner: row h+1's key quads
     // are in flight while row h is accumulated.  Rows alternate between two
-    // register sets (no copies), and each coefficient's digit words are read
-    // through a pointer that moves two padded polys per row, so the a and b
-    // digits are immediate offsets from it.
+    // register sets (no copies; one set when kLean), and each coefficient's
+    // digit words are read through a pointer that moves two padded polys per
+    // row, so the a and b digits are immediate offsets from it.
     MacSum<W> sa[CPT], sb[CPT];
     KeyQuad<W> cur[CPT], nxt[CPT];
     const W *dcol[CPT];
@@ -373,9 +374,9 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
         for (int c = 0; c < CPT; c++)
             if (kFull || tid + c * NT < N) q[c] = load_quad(key, (h << LOGN) + tid + c * NT);
     };
-    if constexpr (sizeof(W) == 8 && NT >= 1024) {
-        // 64-bit words at 1024 threads: one register set of key quads (two
-        // would not fit in 128 VGPRs); the 16 waves per CU hide the loads
+    if constexpr (kLean) {
+        // one register set of key quads (two would not fit in 128 VGPRs);
+        // the 16 waves per CU hide the loads
 #pragma unroll 1
         for (int h = 0; h < d; h++) {
             if (h) fetch(cur, h);
