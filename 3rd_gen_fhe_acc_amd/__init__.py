@@ -390,8 +390,8 @@ class BootstrapGraph:
 
 
 class MixedGraph(BootstrapGraph):
-    """Several parameter sets' batches captured into one HIP graph, one
-    concurrent branch per set; launch() replays them as one unit.
+    """Several parameter sets' batches captured into one HIP graph, set
+    after set; launch() replays them as one unit.
     sets: [(ctx, keys, gates, c1, c2, out)] with device tensors."""
 
     def __init__(self, sets):

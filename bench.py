@@ -212,7 +212,8 @@ def mixed(fhe, torch, main_ctx, main_keys, total=8192, reps=3):
     """Uneven mixed batch in one launch: 60 % set A (the benchmark set), 30 %
     set B (N=2048, 54-bit Q, d=3, n=630), 10 % set C (N=512, 27-bit Q, d=5),
     8192 seeded random-bit ciphertexts uniformly interleaved, captured as one
-    HIP graph with a branch per set, each at its autotuned launch shape."""
+    HIP graph (the sets' kernels one after another), each set at its autotuned
+    launch shape."""
     rng = np.random.default_rng(SEED + 5)
     names = list(MIXED)
     counts = [int(round(total * MIXED_SHARE[k])) for k in names]
@@ -273,7 +274,7 @@ def mixed(fhe, torch, main_ctx, main_keys, total=8192, reps=3):
     g.close()
     return {
         "workload": "8192 ciphertexts uniformly interleaved: 60% N=1024 27-bit d=4 n=500, 30% N=2048 54-bit d=3 "
-                    "n=630, 10% N=512 27-bit d=5 n=500; one graph launch, a branch per set",
+                    "n=630, 10% N=512 27-bit d=5 n=500; one graph launch, sets back to back",
         "ms_per_launch": round(ms, 3), "bootstraps_per_s": round(total / (ms * 1e-3), 1),
         "sets": per_set, "sum_alone_ms": round(sum(v["alone_ms"] for v in per_set.values()), 3),
         "decrypt_ok": bool(ok), "equals_per_set_launch": bool(dense_equal), "setup_s": round(setup_s, 2),

@@ -210,10 +210,10 @@ int fheacc_graph_capture_bootstrap(fheacc_ctx *ctx, fheacc_keys *keys, const int
                                    const uint64_t *c2, uint64_t *out, int64_t B, fheacc_graph **graph);
 /* Mixed parameter sets in one launch: set i is a dense batch of B[i] pairs
  * under its own context and keys (each context one parameter set, all on one
- * device).  One graph holds a branch per set, forked from one root, so the
- * sets run concurrently and the graph launches (on ctxs[0]'s stream) as one
- * unit.  Each branch runs at its context's tuning.  Each set's words are
- * those of fheacc_bootstrap_batch_dev on its own. */
+ * device).  One graph holds every set's kernels, set after set, and launches
+ * (on ctxs[0]'s stream) as one unit.  Each set runs at its context's tuning,
+ * with its own kernel instantiation.  Each set's words are those of
+ * fheacc_bootstrap_batch_dev on its own. */
 int fheacc_graph_capture_mixed(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *keys,
                                const int32_t *const *gates, const uint64_t *const *c1, const uint64_t *const *c2,
                                uint64_t *const *out, const int64_t *B, fheacc_graph **graph);

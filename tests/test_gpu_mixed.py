@@ -1,5 +1,5 @@
 """Mixed parameter sets in one launch (fheacc_graph_capture_mixed): one HIP
-graph with a branch per set, replayed as one unit. Each set's words must be
+graph holding every set's kernels, replayed as one unit. Each set's words must be
 bit-exact:
 - at reduced sizes against the oracle, and against the same set run on its
   own;
