@@ -14,7 +14,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfheacc.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "api.cpp", "params.cpp", "server.cpp"]
+SOURCES = ["kernels_br_u32.hip", "kernels_br_u64.hip", "kernels_cmux_u32.hip", "kernels_cmux_u64.hip",
+           "kernels_misc.hip", "api.cpp", "params.cpp", "server.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 forms stay off (none of this is floating point anyway)
@@ -23,7 +24,7 @@ FLAGS = [
 ]
 
 
-def build(verbose=False, jobs=3):
+def build(verbose=False, jobs=8):
     os.makedirs(LIBDIR, exist_ok=True)
     objs = []
     procs = []

@@ -126,6 +126,13 @@ int pick_pb(const HostParams &p, int total, size_t fixed_bytes, size_t budget, i
 
 int check_ctx(const fheacc_ctx *c) { return c ? 0 : fail("null context"); }
 
+// Keys index the device with their own context's strides: a key set used
+// under another context would read past its allocations.
+int check_keys(const fheacc_ctx *c, const fheacc_keys *k) {
+    if (!k) return fail("null key set");
+    return k->ctx == c ? 0 : fail("keys belong to another context");
+}
+
 // LDS of the blind-rotation kernel with pb digit polys per batch: pb >= 2d
 // runs the product in place in 2d polys, otherwise acc (2 polys) plus a
 // pb-poly digit buffer; then the n evaluated a words.
@@ -271,14 +278,23 @@ int fheacc_ctx_create(const fheacc_params *params, int device, fheacc_ctx **out)
     return 0;
 }
 
-int fheacc_ctx_destroy(fheacc_ctx *c) {
-    if (!c) return 0;
+static void ctx_free(fheacc_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *p : {c->t.fw, c->t.fwp, c->t.iw, c->t.iwp, (void *)c->t.cdt, c->scratch, (void *)c->status})
         if (p) (void)hipFree(p);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+void fheacc::ctx_retain(fheacc_ctx *c) { c->refs.fetch_add(1); }
+
+void fheacc::ctx_release(fheacc_ctx *c) {
+    if (c && c->refs.fetch_sub(1) == 1) ctx_free(c);
+}
+
+int fheacc_ctx_destroy(fheacc_ctx *c) {
+    ctx_release(c);
     return 0;
 }
 
@@ -325,6 +341,7 @@ int fheacc_sync(fheacc_ctx *c) {
     HIP_TRY(hipMemcpy(&st, c->status, sizeof(int), hipMemcpyDeviceToHost));
     if (st) {
         HIP_TRY(hipMemset(c->status, 0, sizeof(int)));
+        if (st & FHEACC_STATUS_GATE) return fail("gate out of range: gates are enum GATES values 0..5 (params.h:107)");
         return fail("identical operands passed to the gate bootstrap (RLWE.cpp:727-730)");
     }
     return 0;
@@ -337,17 +354,26 @@ int fheacc_keys_create(fheacc_ctx *c, fheacc_keys **out) {
     fheacc_keys *k = new (std::nothrow) fheacc_keys();
     if (!k) return fail("out of host memory", FHEACC_ENOMEM);
     k->ctx = c;
+    ctx_retain(c);
     *out = k;
     return 0;
 }
 
-int fheacc_keys_destroy(fheacc_keys *k) {
-    if (!k) return 0;
-    (void)hipSetDevice(k->ctx->device);
-    (void)hipStreamSynchronize(k->ctx->stream);
+void fheacc::keys_retain(fheacc_keys *k) { k->refs.fetch_add(1); }
+
+void fheacc::keys_release(fheacc_keys *k) {
+    if (!k || k->refs.fetch_sub(1) != 1) return;
+    fheacc_ctx *c = k->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     for (void *p : {k->bsk, k->ksk, k->subs})
         if (p) (void)hipFree(p);
     delete k;
+    ctx_release(c);
+}
+
+int fheacc_keys_destroy(fheacc_keys *k) {
+    keys_release(k);
     return 0;
 }
 
@@ -1035,6 +1061,7 @@ static int bootstrap_dense(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, 
 int fheacc_bootstrap_batch_dev(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
                                const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap) {
     if (check_ctx(c) || !k || !gates || !c1 || !c2 || !out) return fail("null argument");
+    if (check_keys(c, k)) return -1;
     if (!k->bsk || !k->ksk) return fail("bootstrap and key-switch keys must be resident");
     if (B <= 0) return B == 0 ? 0 : fail("negative batch");
     if (B > 0x7fffffff) return fail("batch too large");
@@ -1046,6 +1073,7 @@ int fheacc_bootstrap_batch_dev(fheacc_ctx *c, fheacc_keys *k, const int32_t *gat
 int fheacc_bootstrap_batch(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
                            const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap) {
     if (check_ctx(c) || !k || !gates || !c1 || !c2 || !out) return fail("null argument");
+    if (check_keys(c, k)) return -1;
     if (B <= 0) return B == 0 ? 0 : fail("negative batch");
     const HostParams &p = c->p;
     const size_t lw = (size_t)p.n + 1;
@@ -1153,6 +1181,7 @@ static int subs_impl(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, int f, u
 int fheacc_subs_keyswitch_dev(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, int f, uint64_t *out,
                               int64_t count) {
     if (check_ctx(c) || !k || !in || !out) return fail("null argument");
+    if (check_keys(c, k)) return -1;
     if (!k->subs) return fail("no subs key resident");
     if (f < 0 || f >= c->p.logN) return fail("sub_factor out of range");
     if (count <= 0) return 0;
@@ -1194,6 +1223,7 @@ static int expand_impl(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, uint64
 
 int fheacc_expand_rlwe_dev(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, uint64_t *out, int64_t count) {
     if (check_ctx(c) || !k || !in || !out) return fail("null argument");
+    if (check_keys(c, k)) return -1;
     if (!k->subs) return fail("no subs key resident");
     if (count <= 0) return count == 0 ? 0 : fail("negative count");
     if ((count << c->p.logN) > 0x7fffffff) return fail("batch too large");
@@ -1325,6 +1355,7 @@ struct HostCall {
 
 int fheacc_expand_rlwe(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, uint64_t *out, int64_t count) {
     if (check_ctx(c) || !k || !in || !out) return fail("null argument");
+    if (check_keys(c, k)) return -1;
     if (count <= 0) return count == 0 ? 0 : fail("negative count");
     const size_t rb = 2 * (size_t)c->p.N * 8;
     HIP_TRY(hipSetDevice(c->device));
@@ -1452,6 +1483,7 @@ __global__ void k_fill_lwe(uint64_t *c1, uint64_t *c2, int32_t *g, int64_t words
 
 int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size_t report_len) {
     if (check_ctx(c) || !k) return fail("null argument");
+    if (check_keys(c, k)) return -1;
     if (!k->bsk || !k->ksk) return fail("bootstrap and key-switch keys must be resident");
     if (B <= 0 || B > (1 << 20)) return fail("tuning batch out of range");
     HIP_TRY(hipSetDevice(c->device));
@@ -1557,13 +1589,16 @@ struct fheacc_graph {
     fheacc_ctx *ctx = nullptr;  // launches on this context's stream
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    // one scratch buffer per branch, owned by the graph: later eager calls
-    // may grow (reallocate) their context's scratch
+    // one scratch buffer per set, owned by the graph: later eager calls may
+    // grow (reallocate) their context's scratch
     std::vector<void *> scratch;
+    // the contexts and key sets the captured kernels read (held until destroy)
+    std::vector<fheacc_ctx *> ctxs;
+    std::vector<fheacc_keys *> keys;
 };
 
-// Captures one gate-path branch per set into one graph: a fork from a root
-// stream, one stream per set, a join back to the root.
+// Captures the gate path of every set into one graph, set after set on one
+// capture stream, each set with a scratch buffer of its own.
 static int capture_sets(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *keys, const int32_t *const *gates,
                         const uint64_t *const *c1, const uint64_t *const *c2, uint64_t *const *out,
                         const int64_t *B, fheacc_graph **g) {
@@ -1581,6 +1616,12 @@ static int capture_sets(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *
     HIP_TRY(hipSetDevice(ctxs[0]->device));
     std::unique_ptr<fheacc_graph, int (*)(fheacc_graph *)> gr(new fheacc_graph(), fheacc_graph_destroy);
     gr->ctx = ctxs[0];
+    for (int i = 0; i < nsets; i++) {
+        ctx_retain(ctxs[i]);
+        gr->ctxs.push_back(ctxs[i]);
+        keys_retain(keys[i]);
+        gr->keys.push_back(keys[i]);
+    }
     for (int i = 0; i < nsets; i++) {
         void *p = nullptr;
         HIP_TRY(hipMalloc(&p, gate_scratch_bytes(ctxs[i]->p, B[i])));
@@ -1648,6 +1689,8 @@ int fheacc_graph_destroy(fheacc_graph *g) {
     }
     if (g->graph) (void)hipGraphDestroy(g->graph);
     for (void *p : g->scratch) (void)hipFree(p);
+    for (fheacc_keys *k : g->keys) keys_release(k);
+    for (fheacc_ctx *c : g->ctxs) ctx_release(c);
     delete g;
     return 0;
 }

@@ -36,6 +36,10 @@ struct HostParams {
 
 int derive_params(const fheacc_params *in, HostParams *out, std::string *err);
 
+// Per-ciphertext status flags of the gate path (the context's status word
+// ORs them over a batch; the serving loop keeps one per ciphertext).
+enum : int { FHEACC_STATUS_SAME = 1, FHEACC_STATUS_GATE = 2 };
+
 // Host mirror of the device sampler (identical word streams).
 uint64_t rng_word(uint64_t seed, uint64_t stream, uint64_t ctr);
 int64_t gauss_sample(const HostParams &p, uint64_t word);
@@ -112,6 +116,14 @@ size_t gate_scratch_bytes(const HostParams &p, int64_t B);
 // Launches only: no locking, no allocation, no synchronisation.
 int launch_gate_path(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch &g);
 
+// Lifetimes: a context is freed when its creator and every key set, graph and
+// server made from it have let go of it; a key set when its owner and every
+// graph and server that uses it have.
+void ctx_retain(fheacc_ctx *c);
+void ctx_release(fheacc_ctx *c);
+void keys_retain(fheacc_keys *k);
+void keys_release(fheacc_keys *k);
+
 }  // namespace fheacc
 
 struct fheacc_ctx {
@@ -137,13 +149,15 @@ struct fheacc_ctx {
         hipEvent_t a, b;
     };
     std::vector<EvRec> events;
+    std::atomic<int> refs{1};
 };
 
 struct fheacc_keys {
     fheacc_ctx *ctx = nullptr;
-    void *bsk = nullptr;   // device layout, see kernels.hip
+    void *bsk = nullptr;   // device layout, see kernels_impl.hpp
     void *ksk = nullptr;
     void *subs = nullptr;
     size_t bsk_bytes = 0, ksk_bytes = 0, subs_bytes = 0;
     uint64_t bsk_digest = 0, ksk_digest = 0;
+    std::atomic<int> refs{1};
 };

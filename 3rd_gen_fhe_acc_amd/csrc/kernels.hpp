@@ -1,4 +1,4 @@
-// kernels.hpp -- kernel argument blocks and host launchers (kernels.hip).
+// kernels.hpp -- kernel argument blocks and host launchers (kernels_impl.hpp).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,6 +39,7 @@ struct BRArgs {
     const W *const *bsk_tab;
     int *ct_status;
     int mode;  // FHEACC_EVAL_*: which residue-0 steps multiply by RGSW(1)
+    int64_t c_base;  // ciphertext of workgroup 0 (a batch split over several launches)
 };
 
 template <typename W>
@@ -80,6 +81,7 @@ struct EPArgs {
     const W *key;
     int64_t key_stride;
     int d, bits, pb;
+    int64_t c_base;  // ciphertext of workgroup 0
 };
 
 template <typename W>
@@ -92,6 +94,7 @@ struct SubsArgs {
     int64_t key_stride;
     int sub_factor;
     int d, bits, dch;
+    int64_t c_base;  // workgroup index of workgroup 0
 };
 
 // CMUX chains (RLWE.cpp:788-838): per ciphertext acc = c0; for each step
@@ -110,6 +113,7 @@ struct CmuxArgs {
     int steps;
     int pre_rot;              // public rotation applied first (exponent in [0, 2N)), 0 = none
     int d, bits, pb;
+    int64_t c_base;           // ciphertext of workgroup 0
 };
 
 struct ShoupGen {

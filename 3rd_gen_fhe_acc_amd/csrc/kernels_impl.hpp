@@ -1,4 +1,7 @@
-// kernels.hip -- HIP kernels of the gate-bootstrapping path for gfx950.
+// kernels_impl.hpp -- HIP kernels of the gate-bootstrapping path for gfx950.
+//
+// Included by the kernels_*.hip translation units; each instantiates a share
+// of the launchers so that the library's device code compiles in parallel.
 //
 // Device layouts (W = uint32_t for Q < 2^30, else uint64_t):
 //   RGSW (external-product key)  [digitG][N] x {K0a, K0b, K1a, K1b}
@@ -17,8 +20,11 @@
 // One ciphertext per workgroup for every polynomial kernel; polynomials live
 // in LDS and are transformed by ntt_lds.hpp.  Nothing here is floating
 // point: no MFMA.
+#pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 #include "fheacc_internal.hpp"
 #include "kernels.hpp"
@@ -423,25 +429,31 @@ __global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
     uint16_t *av = reinterpret_cast<uint16_t *>(
         inplace ? acc + padded(max(2, 2 * a.d) * N) : dig + padded(a.pb * N));
     __shared__ uint32_t s_b;
-    const int64_t c = blockIdx.x;
-    const int n = a.n;
+    const int64_t c = a.c_base + blockIdx.x;
     const uint32_t q = a.q, qm = a.q - 1;
-    const int g = a.gates[c];
+    // gates outside enum GATES [0, 5] are an error of the call: flagged like
+    // identical operands, and the ciphertext skips its blind rotation
+    int g = a.gates[c];
+    const bool bad_gate = (unsigned)g > 5u;
+    if (bad_gate) g = 0;
+    const int n = bad_gate ? 0 : a.n;
     const bool xr = (g == 4 || g == 5);
     const uint64_t *x1 = a.c1 + (a.idx1 ? a.idx1[c] : c) * (n + 1);
     const uint64_t *x2 = a.c2 + (a.idx2 ? a.idx2[c] : c) * (n + 1);
     int same = 1;
-    for (int k = threadIdx.x; k <= n; k += NT) {
+    for (int k = threadIdx.x; k <= a.n; k += NT) {
         uint64_t u = x1[k], v = x2[k];
         same &= (u == v);
         uint32_t r = (uint32_t)(xr ? 2 * (u - v) : u + v) & qm;
-        if (k < n) av[k] = (uint16_t)r;
+        if (k < a.n) av[k] = (uint16_t)r;
         else s_b = r;
     }
     const int all_same = __syncthreads_and(same);
     if (threadIdx.x == 0) {
-        if (a.ct_status) a.ct_status[c] = all_same;
-        else if (all_same) atomicOr(a.status, 1);
+        // bit 0: identical operands (RLWE.cpp:727-730); bit 1: gate out of range
+        const int flags = (all_same ? FHEACC_STATUS_SAME : 0) | (bad_gate ? FHEACC_STATUS_GATE : 0);
+        if (a.ct_status) a.ct_status[c] = flags;
+        else if (flags) atomicOr(a.status, flags);
     }
     // accumulator init: only b carries the test polynomial
     const uint64_t Q8 = a.A.Q / 8 + 1, nQ8 = a.A.Q - Q8;
@@ -775,7 +787,7 @@ __global__ __launch_bounds__(NT) void k_external_product(EPArgs<W> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     W *acc = reinterpret_cast<W *>(smem);
     W *dig = acc + padded(2 * N);
-    const int64_t c = blockIdx.x;
+    const int64_t c = a.c_base + blockIdx.x;
     const uint64_t *src = a.acc + c * 2 * N;
     for (int i = threadIdx.x; i < 2 * N; i += NT) acc[pad(i)] = (W)src[i];
     __syncthreads();
@@ -805,7 +817,7 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
     W *x = reinterpret_cast<W *>(smem);  // input, time domain (2 padded polys)
     W *y = x + padded(2 * N);            // substituted (2 padded polys)
     W *dig = y + padded(2 * N);          // dch padded digit polys
-    const int64_t c = blockIdx.x;
+    const int64_t c = a.c_base + blockIdx.x;
     int64_t row = c;
     if constexpr (EXPAND) row = ((c >> a.sub_factor) << LOGN) + (c & ((1 << a.sub_factor) - 1));
     const uint64_t *src = a.in + row * 2 * N;
@@ -912,7 +924,7 @@ __global__ __launch_bounds__(NT) void k_cmux_chain(CmuxArgs<W> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     W *acc = reinterpret_cast<W *>(smem);
     W *dig = acc + padded(2 * N);
-    const int64_t c = blockIdx.x;
+    const int64_t c = a.c_base + blockIdx.x;
     const uint64_t *src = a.c0 + c * 2 * N;
     const uint64_t Q = a.A.Q;
     W sav[2][CPT];
@@ -1267,21 +1279,6 @@ __global__ void k_subs_from_ref(const uint64_t *src, W *dst, int N, int64_t tota
     }
 }
 
-// FNV-1a style digest over 64-bit words (order dependent), one partial per block
-__global__ void k_digest(const uint64_t *w, int64_t count, uint64_t *partials) {
-    __shared__ uint64_t sh[256];
-    uint64_t h = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
-        h += mix64(w[i] ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(i + 1)));
-    sh[threadIdx.x] = h;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
-}
-
 // ---------------------------------------------------------------------------
 // host launchers (template dispatch on word width, log2 N, workgroup size)
 // ---------------------------------------------------------------------------
@@ -1299,13 +1296,27 @@ __global__ void k_digest(const uint64_t *w, int64_t count, uint64_t *partials) {
         default: return hipErrorInvalidValue;            \
     }
 
+// Workgroups of one launch: a grid dimension must stay below 2^32 work-items,
+// so larger batches go out as several launches (the kernels add a base to
+// their workgroup index).
+template <int NT>
+constexpr int64_t max_wgs_per_launch() {
+    return ((int64_t)1 << 31) / NT;
+}
+
 template <typename W, int LOGN, int NT>
 static hipError_t launch_br_nt(const BRArgs<W> &a, int64_t B, size_t lds, hipStream_t st) {
     auto k = a.pb >= 2 * a.d ? k_blind_rotate<W, LOGN, NT, true> : k_blind_rotate<W, LOGN, NT, false>;
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3((unsigned)B), dim3(NT), lds, st, a);
-    return hipGetLastError();
+    for (int64_t off = 0; off < B; off += max_wgs_per_launch<NT>()) {
+        BRArgs<W> part = a;
+        part.c_base = a.c_base + off;
+        hipLaunchKernelGGL(k, dim3((unsigned)std::min(B - off, max_wgs_per_launch<NT>())), dim3(NT), lds, st, part);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <typename W>
@@ -1346,8 +1357,6 @@ size_t ks_staged_lds(uint32_t Bks) {
     const size_t fill = (size_t)KS_SB * Bks * (KB + 16 / sizeof(W)) * sizeof(W);
     return ks_double_buffered<W, KB>(Bks, 256) ? 2 * fill : fill;
 }
-template size_t ks_staged_lds<uint32_t>(uint32_t);
-template size_t ks_staged_lds<uint64_t>(uint32_t);
 
 template <typename W>
 hipError_t launch_ntt(const NTTArgs<W> &a, int logN, hipStream_t st) {
@@ -1364,8 +1373,15 @@ static hipError_t launch_ep_nt(const EPArgs<W> &a, int64_t count, size_t lds, hi
     auto k = k_external_product<W, LOGN, NT, INPLACE>;
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(NT), lds, st, a);
-    return hipGetLastError();
+    for (int64_t off = 0; off < count; off += max_wgs_per_launch<NT>()) {
+        EPArgs<W> part = a;
+        part.c_base = off;
+        hipLaunchKernelGGL(k, dim3((unsigned)std::min(count - off, max_wgs_per_launch<NT>())), dim3(NT), lds, st,
+                           part);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // One workgroup per product.  All 2d digit polys fit the LDS at every
@@ -1393,8 +1409,15 @@ static hipError_t launch_subs_nt(const SubsArgs<W> &a, int64_t count, size_t lds
     auto k = k_subs_keyswitch<W, LOGN, NT, EXPAND>;
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(NT), lds, st, a);
-    return hipGetLastError();
+    for (int64_t off = 0; off < count; off += max_wgs_per_launch<NT>()) {
+        SubsArgs<W> part = a;
+        part.c_base = off;
+        hipLaunchKernelGGL(k, dim3((unsigned)std::min(count - off, max_wgs_per_launch<NT>())), dim3(NT), lds, st,
+                           part);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <typename W>
@@ -1408,7 +1431,6 @@ template <typename W>
 hipError_t launch_expand_level(const SubsArgs<W> &a, int logN, int64_t B, hipStream_t st) {
     const size_t lds = (size_t)(2 * padded(2 << logN) + padded(a.dch << logN)) * sizeof(W);
     const int64_t count = B << a.sub_factor;
-    if (count > 0x7fffffff) return hipErrorInvalidValue;
     FHEACC_LOGN_SWITCH(logN, { return launch_subs_nt<W, LOGN, true>(a, count, lds, st); })
     return hipSuccess;
 }
@@ -1418,13 +1440,19 @@ static hipError_t launch_cmux_nt(const CmuxArgs<W> &a, int64_t count, size_t lds
     auto k = a.pb >= 2 * a.d ? k_cmux_chain<W, LOGN, NT, true> : k_cmux_chain<W, LOGN, NT, false>;
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(NT), lds, st, a);
-    return hipGetLastError();
+    for (int64_t off = 0; off < count; off += max_wgs_per_launch<NT>()) {
+        CmuxArgs<W> part = a;
+        part.c_base = off;
+        hipLaunchKernelGGL(k, dim3((unsigned)std::min(count - off, max_wgs_per_launch<NT>())), dim3(NT), lds, st,
+                           part);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <typename W>
 hipError_t launch_cmux_chain(const CmuxArgs<W> &a, int logN, int nt, int64_t count, size_t lds, hipStream_t st) {
-    if (count > 0x7fffffff) return hipErrorInvalidValue;
     FHEACC_LOGN_SWITCH(logN, {
         if (nt == 128 && LOGN >= 7) return launch_cmux_nt<W, LOGN, 128>(a, count, lds, st);
         if (nt == 512 && LOGN >= 9) return launch_cmux_nt<W, LOGN, 512>(a, count, lds, st);
@@ -1502,30 +1530,5 @@ hipError_t launch_subs_convert(const void *src, void *dst, int N, int64_t total,
     return hipGetLastError();
 }
 
-hipError_t launch_digest(const uint64_t *w, int64_t count, uint64_t *partials, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL(k_digest, dim3(blocks), dim3(256), 0, st, w, count, partials);
-    return hipGetLastError();
-}
-
-#define FHEACC_INSTANTIATE(W)                                                                                     \
-    template hipError_t launch_blind_rotate<W>(const BRArgs<W> &, int, int, int64_t, size_t, hipStream_t);       \
-    template hipError_t launch_lwe_keyswitch<W>(const KSArgs<W> &, bool, hipStream_t);                            \
-    template hipError_t launch_ntt<W>(const NTTArgs<W> &, int, hipStream_t);                                      \
-    template hipError_t launch_external_product<W>(const EPArgs<W> &, int, int64_t, hipStream_t);                 \
-    template hipError_t launch_subs_keyswitch<W>(const SubsArgs<W> &, int, int64_t, hipStream_t);                 \
-    template hipError_t launch_expand_level<W>(const SubsArgs<W> &, int, int64_t, hipStream_t);                   \
-    template hipError_t launch_cmux_chain<W>(const CmuxArgs<W> &, int, int, int64_t, size_t, hipStream_t);        \
-    template hipError_t launch_rgsw_to_device<W>(const uint64_t *, int64_t, W *, int64_t, int, int, ShoupGen,     \
-                                                 const int64_t *, int64_t, hipStream_t, bool);                    \
-    template hipError_t launch_rgsw_to_ref<W>(const W *, int64_t, uint64_t *, int64_t, int, int, const int64_t *, \
-                                              int64_t, hipStream_t, bool);                                        \
-    template hipError_t launch_ksk_convert<W>(const uint64_t *, W *, uint64_t *, int, int, int, int, int,         \
-                                              hipStream_t);                                                       \
-    template hipError_t launch_keygen<W>(const KeygenArgs<W> &, const KskGenArgs<W> &, int, const uint64_t *,     \
-                                         hipStream_t, bool);                                                      \
-    template hipError_t launch_subs_convert<W>(const void *, void *, int, int64_t, bool, ShoupGen, hipStream_t);
-
-FHEACC_INSTANTIATE(uint32_t)
-FHEACC_INSTANTIATE(uint64_t)
 
 }  // namespace fheacc

@@ -493,6 +493,8 @@ static void server_release(fheacc_server *s) {
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->pool) (void)hipFree(s->pool);
     if (s->key_tab_dev) (void)hipFree(s->key_tab_dev);
+    for (fheacc_keys *k : s->clients) keys_release(k);
+    if (s->ctx) ctx_release(s->ctx);
     delete s;
 }
 
@@ -509,6 +511,7 @@ int fheacc_server_create(fheacc_ctx *ctx, const fheacc_server_config *cfg, fheac
     if (rows < 3) return fail("hbm_budget holds fewer than 3 ciphertexts");
     HIP_TRY(hipSetDevice(ctx->device));
     fheacc_server *s = new fheacc_server();
+    ctx_retain(ctx);
     s->ctx = ctx;
     s->cfg = *cfg;
     s->row_words = rw;
@@ -564,12 +567,14 @@ int fheacc_server_add_client(fheacc_server *s, fheacc_keys *keys, int32_t *clien
     if (a.n != b.n || a.q != b.q || a.N != b.N || a.Q != b.Q || a.BGbits != b.BGbits || a.Br != b.Br ||
         a.Bks != b.Bks)
         return fail("keys belong to a different parameter set");
+    if (keys->ctx->device != s->ctx->device) return fail("keys are resident on another device");
     if (!keys->bsk || !keys->ksk) return fail("bootstrap and key-switch keys must be resident");
     std::lock_guard<std::mutex> lk(s->mu);
     if ((int)s->clients.size() >= kMaxClients) return fail("too many clients (64)");
     const int32_t id = (int32_t)s->clients.size();
     HIP_TRY(hipSetDevice(s->ctx->device));
     HIP_TRY(hipMemcpy(s->key_tab_dev + id, &keys->bsk, sizeof(void *), hipMemcpyHostToDevice));
+    keys_retain(keys);
     s->clients.push_back(keys);
     *client = id;
     return 0;

@@ -82,6 +82,9 @@ typedef struct fheacc_capture {
  * accelerator configuration FPGA::OCL_config_wr_rd + BAR1_ROU_table_*_wr
  * (myfpga.h:141-180, myfpga.cpp:82-594). */
 int fheacc_ctx_create(const fheacc_params *params, int device, fheacc_ctx **out);
+/* Releases the caller's handle.  Key sets, graphs and servers made from the
+ * context hold references of their own: its device memory is freed when the
+ * last of them is destroyed, so they may be destroyed in any order. */
 int fheacc_ctx_destroy(fheacc_ctx *ctx);
 int fheacc_ctx_info(const fheacc_ctx *ctx, fheacc_info *out);
 /* Launch on a caller-owned hipStream_t from now on (NULL: the null stream).
@@ -137,6 +140,9 @@ int fheacc_ctx_get_eval_mode(const fheacc_ctx *ctx, int *mode);
  * LWE.cpp:55-106, app.cpp:4-24) and the key DMA to the card
  * (FPGA::dma_write_bootstrap_key, dma_write_subs_key, myfpga.cpp:596-712). */
 int fheacc_keys_create(fheacc_ctx *ctx, fheacc_keys **out);
+/* Releases the caller's handle; graphs and servers that use the key set keep
+ * it resident until they are destroyed.  Every call that takes a context and
+ * a key set fails with -1 when the keys were created under another context. */
 int fheacc_keys_destroy(fheacc_keys *keys);
 /* Host arrays in the reference layouts above.  Every residue is stored,
  * btkey[0] (RGSW(1)) included: the FPGA evaluation orders read it. */
@@ -192,9 +198,14 @@ int fheacc_lwe_decrypt(const fheacc_ctx *ctx, const uint64_t *sk, const uint64_t
 /* ---- the gate path ------------------------------------------------------
  * RLWE::eval_bootstrap (RLWE.cpp:726-786) for B independent pairs, and the
  * accelerator's batched form FPGA::fpga_eval_bootstrap_x{1,4,8,12,16}
- * (myfpga.h:141-401, myfpga.cpp:1003-1660) with any B.  gates[i] per pair.
- * Identical operands (the reference exits the process, RLWE.cpp:727-730)
- * fail the call with -1.  Output: [B][n+1] LWE mod q. */
+ * (myfpga.h:141-401, myfpga.cpp:1003-1660) with any B up to 2^31 - 1 (larger
+ * grids go out as several launches).  gates[i] per pair, an enum GATES value
+ * in [0, 5].  Identical operands (the reference exits the process,
+ * RLWE.cpp:727-730) and gates outside [0, 5] fail the call with -1: the host
+ * form checks them before launching; the device form and graph replays flag
+ * them on the device (those ciphertexts skip their blind rotation, and
+ * fheacc_sync, which the host form calls, reports the error).  Output:
+ * [B][n+1] LWE mod q. */
 int fheacc_bootstrap_batch(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t *gates, const uint64_t *c1,
                            const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap);
 int fheacc_bootstrap_batch_dev(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t *gates, const uint64_t *c1,
