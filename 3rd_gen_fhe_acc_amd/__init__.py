@@ -107,6 +107,8 @@ EXPORTS = {
                                C.c_int64, C.POINTER(_Capture)],
     "fheacc_bootstrap_batch_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_int64, C.POINTER(_Capture)],
+    "fheacc_bootstrap_batch_dev_noalloc": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_int64, C.POINTER(_Capture)],
     "fheacc_graph_capture_bootstrap": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_int64, C.POINTER(C.c_void_p)],
     "fheacc_graph_capture_mixed": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -307,6 +309,15 @@ class Context:
                        cap_ks.data_ptr() if cap_ks is not None else None)
         _check(lib().fheacc_bootstrap_batch_dev(self.h, keys.h, _dptr(gates), _dptr(c1), _dptr(c2), _dptr(out),
                                                 B, C.byref(cap)))
+
+    def bootstrap_dev_noalloc(self, keys, gates, c1, c2, out, cap_acc=None, cap_ks=None):
+        """bootstrap_dev with its scratch in the library's static device array:
+        no device allocation (fheacc_bootstrap_batch_dev_noalloc)."""
+        B = c1.shape[0]
+        cap = _Capture(cap_acc.data_ptr() if cap_acc is not None else None,
+                       cap_ks.data_ptr() if cap_ks is not None else None)
+        _check(lib().fheacc_bootstrap_batch_dev_noalloc(self.h, keys.h, _dptr(gates), _dptr(c1), _dptr(c2),
+                                                        _dptr(out), B, C.byref(cap)))
 
     def capture_bootstrap(self, keys, gates, c1, c2, out):
         """Capture the gate path over fixed device buffers into a HIP graph."""

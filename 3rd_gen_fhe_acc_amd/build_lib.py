@@ -15,7 +15,7 @@ LIB = os.path.join(LIBDIR, "libfheacc.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["kernels_br_u32.hip", "kernels_br_u64.hip", "kernels_cmux_u32.hip", "kernels_cmux_u64.hip",
-           "kernels_misc.hip", "api.cpp", "params.cpp", "server.cpp"]
+           "kernels_misc.hip", "api.cpp", "params.cpp", "server.cpp", "noalloc.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 forms stay off (none of this is floating point anyway)

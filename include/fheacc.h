@@ -211,6 +211,24 @@ int fheacc_bootstrap_batch(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t *ga
 int fheacc_bootstrap_batch_dev(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t *gates, const uint64_t *c1,
                                const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap);
 
+/* The gate path of fheacc_bootstrap_batch_dev with no device allocation.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory: code under HIP graph capture, serving loops and harnesses
+ * that forbid hipMalloc.  Its scratch (the key-switch digits, B * N * digitKS
+ * bytes, the b words and the gather key switch's partial sums) lives in a
+ * static device array of FHEACC_NOALLOC_SCRATCH_BYTES on each GPU, held
+ * until the process ends; a batch that needs more fails with -1 (at the
+ * benchmark's N = 1024, n = 500, digitKS = 6 that is above about 5,100
+ * ciphertexts).  Every call on one GPU shares that GPU's array and leaves its
+ * words behind: calls that overlap on one GPU (two streams, threads or
+ * contexts) race; calls on one stream are ordered.  Per-kernel profiling is
+ * skipped.  Words, checks and status reporting are those of
+ * fheacc_bootstrap_batch_dev. */
+#define FHEACC_NOALLOC_SCRATCH_BYTES (32u << 20)
+int fheacc_bootstrap_batch_dev_noalloc(fheacc_ctx *ctx, fheacc_keys *keys, const int32_t *gates,
+                                       const uint64_t *c1, const uint64_t *c2, uint64_t *out, int64_t B,
+                                       const fheacc_capture *cap);
+
 /* The gate path for fixed device buffers captured once into a HIP graph and
  * replayed: each launch re-reads gates/c1/c2 and rewrites out (the FPGA form
  * issues one MMIO instruction per external product instead,
