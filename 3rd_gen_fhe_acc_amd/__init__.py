@@ -124,6 +124,14 @@ EXPORTS = {
     "fheacc_blind_rotate_full_dev": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                      C.c_int64],
     "fheacc_expand_rlwe": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64],
+    "fheacc_subs_keyswitch": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64],
+    "fheacc_homo_expand_dev": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64],
+    "fheacc_homo_expand": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64],
+    "fheacc_cmux_tree_dev": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64],
+    "fheacc_cmux_tree": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64],
+    "fheacc_rlwe_encrypt": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.c_void_p],
+    "fheacc_rgsw_encrypt": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.c_void_p],
+    "fheacc_rlwe_decrypt": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64, C.c_void_p],
     "fheacc_external_product": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64],
     "fheacc_cmux": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64],
     "fheacc_blind_rotate_rlwe": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int64],
@@ -367,6 +375,64 @@ class Context:
         out = np.zeros(c0.shape, dtype=np.uint64)
         stride = 0 if rgsw.size == self.digitG * 4 * self.N else self.digitG * 4 * self.N
         _check(lib().fheacc_cmux(self.h, p0, p1, pg, stride, _np(out)[1], c0.shape[0]))
+        return out
+
+    # RLWE / RGSW encryption and RLWE decryption (host arrays, device work)
+    def rlwe_encrypt(self, sk, msgs, seed, first_index=0):
+        """RLWE_encrypt (RLWE.cpp:324-344) of each row of msgs (time domain)."""
+        msgs, pm = _np(np.asarray(msgs).reshape(-1, self.N))
+        out = np.zeros((msgs.shape[0], 2, self.N), dtype=np.uint64)
+        _check(lib().fheacc_rlwe_encrypt(self.h, _np(sk)[1], pm, msgs.shape[0], seed, first_index, _np(out)[1]))
+        return out
+
+    def rgsw_encrypt(self, sk, msgs, seed, first_index=0):
+        """RGSW encryption (RLWE.cpp:561-577) of each row of msgs (time domain);
+        reference-layout RGSWs [count][digitG * 4 * N]."""
+        msgs, pm = _np(np.asarray(msgs).reshape(-1, self.N))
+        out = np.zeros((msgs.shape[0], self.digitG * 4 * self.N), dtype=np.uint64)
+        _check(lib().fheacc_rgsw_encrypt(self.h, _np(sk)[1], pm, msgs.shape[0], seed, first_index, _np(out)[1]))
+        return out
+
+    def rlwe_decrypt(self, sk, c, t):
+        """RLWE_decrypt(s, c, t) (RLWE.cpp:362-376): [count][N] in [0, t]."""
+        c, pc = _np(np.asarray(c).reshape(-1, 2, self.N))
+        out = np.zeros((c.shape[0], self.N), dtype=np.uint64)
+        _check(lib().fheacc_rlwe_decrypt(self.h, _np(sk)[1], pc, c.shape[0], t, _np(out)[1]))
+        return out
+
+    def subs_keyswitch(self, keys, rlwe, sub_factor):
+        """Host arrays: RLWE_substitute + RLWE_keyswitch of [count][2][N]."""
+        rlwe, pr = _np(np.asarray(rlwe).reshape(-1, 2, self.N))
+        out = np.zeros(rlwe.shape, dtype=np.uint64)
+        _check(lib().fheacc_subs_keyswitch(self.h, keys.h, pr, sub_factor, _np(out)[1], rlwe.shape[0]))
+        return out
+
+    def homo_expand_dev(self, keys, packed, enc_sec, out):
+        """homo_expand (app.cpp:78-97): packed [count][digitG][2][N] ->
+        out [count][N][digitG * 4 * N] (device tensors)."""
+        count = packed.numel() // (self.digitG * 2 * self.N)
+        _check(lib().fheacc_homo_expand_dev(self.h, keys.h, _dptr(packed), _dptr(enc_sec), _dptr(out), count))
+
+    def homo_expand(self, keys, packed, enc_sec):
+        packed, pp = _np(np.asarray(packed).reshape(-1, self.digitG, 2, self.N))
+        enc_sec, pe = _np(enc_sec)
+        out = np.zeros((packed.shape[0], self.N, self.digitG * 4 * self.N), dtype=np.uint64)
+        _check(lib().fheacc_homo_expand(self.h, keys.h, pp, pe, _np(out)[1], packed.shape[0]))
+        return out
+
+    def cmux_tree_dev(self, leaves, rgsws, out, levels):
+        """CMUX tree (test.cpp:1576-1582): leaves [count][2^levels][2][N],
+        rgsws [levels] reference-layout RGSWs -> out [count][2][N]."""
+        count = leaves.numel() // ((2 << levels) * self.N)
+        _check(lib().fheacc_cmux_tree_dev(self.h, _dptr(leaves), levels, _dptr(rgsws) if levels else None, 0,
+                                          _dptr(out), count))
+
+    def cmux_tree(self, leaves, rgsws):
+        rgsws, pg = _np(np.asarray(rgsws).reshape(-1, self.digitG * 4 * self.N))
+        levels = rgsws.shape[0]
+        leaves, pl = _np(np.asarray(leaves).reshape(-1, 1 << levels, 2, self.N))
+        out = np.zeros((leaves.shape[0], 2, self.N), dtype=np.uint64)
+        _check(lib().fheacc_cmux_tree(self.h, pl, levels, pg, 0, _np(out)[1], leaves.shape[0]))
         return out
 
     def subs_keyswitch_dev(self, keys, rlwe, sub_factor, out):

@@ -69,7 +69,7 @@ def build_host():
     subprocess.check_call(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", so + ".tmp",
                            os.path.join(host, "fhe_acc.cpp"), "-lfheacc"] + link)
     os.replace(so + ".tmp", so)
-    for name in ("gate_demo", "rlwe_demo"):
+    for name in ("gate_demo", "rlwe_demo", "tree_demo"):
         demo = os.path.join(LIBDIR, name)
         subprocess.check_call(["g++", "-O2", "-std=c++17", "-Wall", "-o", demo + ".tmp",
                                os.path.join(host, name + ".cpp"), "-lfheacc_host", "-lfheacc"] + link)

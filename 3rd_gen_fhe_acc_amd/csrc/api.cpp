@@ -1123,9 +1123,11 @@ int fheacc_ntt_dev(fheacc_ctx *c, uint64_t *polys, int64_t count, int inverse) {
     return 0;
 }
 
+// RLWE_mult_RGSW over count inputs acc + i in_stride -> out + i out_stride
+// (words; 2N for dense batches).
 template <typename W>
 static int ep_impl(fheacc_ctx *c, const uint64_t *acc, const uint64_t *rgsw, int64_t stride, uint64_t *out,
-                   int64_t count) {
+                   int64_t count, int64_t in_stride = 0, int64_t out_stride = 0) {
     const HostParams &p = c->p;
     const int64_t nk = stride ? count : 1;
     const size_t ke = rgsw_elems(p);
@@ -1138,6 +1140,8 @@ static int ep_impl(fheacc_ctx *c, const uint64_t *acc, const uint64_t *rgsw, int
     a.tw = make_tw<W>(c);
     a.acc = acc;
     a.out = out;
+    a.in_stride = in_stride ? in_stride : 2 * (int64_t)p.N;
+    a.out_stride = out_stride ? out_stride : 2 * (int64_t)p.N;
     a.key = (const W *)c->scratch;
     a.key_stride = stride ? (int64_t)ke : 0;
     a.d = p.digitG;
@@ -1194,14 +1198,21 @@ int fheacc_subs_keyswitch_dev(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in,
 
 // ------------------------------------------- RLWE expansion and CMUX chains
 
+// expand_RLWE of B inputs (in[b], 2N words apart); slot k of expansion b is
+// written at out + b set_stride + k slot_stride (words; 0: dense [B][N][2][N]).
 template <typename W>
-static int expand_impl(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, uint64_t *out, int64_t B) {
+static int expand_impl(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, uint64_t *out, int64_t B,
+                       int64_t set_stride = 0, int64_t slot_stride = 0) {
     const HostParams &p = c->p;
     const size_t rw = 2 * (size_t)p.N;
+    if (!slot_stride) slot_stride = (int64_t)rw;
+    if (!set_stride) set_stride = (int64_t)p.N * slot_stride;
     // slot 0 of every expansion starts as its input
-    HIP_TRY(hipMemcpy2DAsync(out, (size_t)p.N * rw * 8, in, rw * 8, rw * 8, (size_t)B, hipMemcpyDeviceToDevice,
+    HIP_TRY(hipMemcpy2DAsync(out, (size_t)set_stride * 8, in, rw * 8, rw * 8, (size_t)B, hipMemcpyDeviceToDevice,
                              c->stream));
     SubsArgs<W> a{};
+    a.set_stride = set_stride;
+    a.slot_stride = slot_stride;
     a.A = make_arith<W>(p);
     a.tw = make_tw<W>(c);
     a.in = out;
@@ -1258,6 +1269,7 @@ static int cmux_impl(fheacc_ctx *c, const uint64_t *c0, const uint64_t *c1, cons
     a.tw = make_tw<W>(c);
     a.c0 = c0;
     a.c1 = c1;
+    a.in_stride = 2 * (int64_t)p.N;
     a.out = out;
     a.key = (const W *)c->scratch;
     a.key_stride = per_ct ? 0 : (int64_t)ke;
@@ -1320,6 +1332,177 @@ int fheacc_blind_rotate_full_dev(fheacc_ctx *c, const uint64_t *in, int b, const
                          count);
 }
 
+// ------------------------------------------ homo_expand and the CMUX tree
+
+// application::homo_expand (app.cpp:78-97) for count queries: the digitG
+// expansions of each query are written straight into column 1 of the output
+// RGSWs' rows (RGSW i, row j, column 1 = slot i of expansion j), then every
+// row's column 0 is its column 1 times RGSW(-s), one batched external product.
+template <typename W>
+static int homo_expand_impl(fheacc_ctx *c, fheacc_keys *k, const uint64_t *packed, const uint64_t *enc_sec,
+                            uint64_t *out, int64_t count) {
+    const HostParams &p = c->p;
+    const int64_t N = p.N, G = (int64_t)p.digitG * 4 * N;  // words per RGSW
+    for (int64_t q = 0; q < count; q++)
+        if (int rc = expand_impl<W>(c, k, packed + q * (int64_t)p.digitG * 2 * N, out + q * N * G + 2 * N,
+                                    p.digitG, 4 * N, G))
+            return rc;
+    return ep_impl<W>(c, out + 2 * N, enc_sec, 0, out, count * N * p.digitG, 4 * N, 4 * N);
+}
+
+int fheacc_homo_expand_dev(fheacc_ctx *c, fheacc_keys *k, const uint64_t *packed, const uint64_t *enc_sec,
+                           uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !k || !packed || !enc_sec || !out) return fail("null argument");
+    if (check_keys(c, k)) return -1;
+    if (!k->subs) return fail("no subs key resident");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    if ((count << c->p.logN) * (int64_t)c->p.digitG > 0x7fffffff) return fail("batch too large");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return c->p.narrow ? homo_expand_impl<uint32_t>(c, k, packed, enc_sec, out, count)
+                       : homo_expand_impl<uint64_t>(c, k, packed, enc_sec, out, count);
+}
+
+// The tree selection's CMUX tree (test.cpp:1576-1582) over count trees of
+// 2^levels leaves: level l halves every tree with RGSW l (v[j/2] =
+// CMUX(v[j], v[j+1], C[l])), ping-ponging between two scratch buffers; the
+// last level writes out.
+template <typename W>
+static int cmux_tree_impl(fheacc_ctx *c, const uint64_t *leaves, int levels, const uint64_t *rgsw,
+                          int64_t rgsw_stride, uint64_t *out, int64_t count) {
+    const HostParams &p = c->p;
+    const int64_t rw = 2 * (int64_t)p.N;
+    if (levels == 0) {
+        HIP_TRY(hipMemcpyAsync(out, leaves, (size_t)count * rw * 8, hipMemcpyDeviceToDevice, c->stream));
+        return 0;
+    }
+    const size_t ke = rgsw_elems(p);
+    const size_t kbytes = ((size_t)levels * ke * sizeof(W) + 255) / 256 * 256;
+    const size_t pbytes = ((size_t)(count << (levels - 1)) * rw * 8 + 255) / 256 * 256;
+    if (int rc = ensure_scratch(c, kbytes + 2 * pbytes)) return rc;
+    const int64_t ref_words = (int64_t)p.digitG * 4 * p.N;
+    HIP_TRY(launch_rgsw_to_device<W>(rgsw, rgsw_stride ? rgsw_stride : ref_words, (W *)c->scratch, ke, p.digitG,
+                                     p.N, make_sg(p.Q), nullptr, levels, c->stream));
+    uint64_t *buf[2] = {(uint64_t *)((char *)c->scratch + kbytes), (uint64_t *)((char *)c->scratch + kbytes + pbytes)};
+    CmuxArgs<W> a{};
+    a.A = make_arith<W>(p);
+    a.tw = make_tw<W>(c);
+    a.in_stride = 2 * rw;  // the leaf pairs (2p, 2p + 1) of every tree
+    a.steps = 1;
+    a.d = p.digitG;
+    a.bits = p.BGbits;
+    a.pb = c->tune.br_pb > 0 ? std::min(c->tune.br_pb, 2 * (int)p.digitG) : 2 * (int)p.digitG;
+    while (a.pb > 1 && br_lds_bytes(p, a.pb) > c->lds_budget) a.pb--;
+    const size_t lds = br_lds_bytes(p, a.pb);
+    if (lds > c->lds_budget) return fail("parameter set does not fit the LDS budget");
+    const int nt = c->tune.shape_set || p.N < 1024 ? c->tune.br_threads : 1024;
+    const uint64_t *src = leaves;
+    for (int l = 0; l < levels; l++) {
+        uint64_t *dst = l == levels - 1 ? out : buf[l & 1];
+        a.c0 = src;
+        a.c1 = src + rw;
+        a.out = dst;
+        a.key = (const W *)c->scratch + l * ke;
+        Prof pr(c, 2);
+        HIP_TRY(launch_cmux_chain<W>(a, p.logN, nt, count << (levels - 1 - l), lds, c->stream));
+        src = dst;
+    }
+    return 0;
+}
+
+int fheacc_cmux_tree_dev(fheacc_ctx *c, const uint64_t *leaves, int levels, const uint64_t *rgsw,
+                         int64_t rgsw_stride, uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !leaves || !out || (levels > 0 && !rgsw)) return fail("null argument");
+    if (levels < 0 || levels > 24) return fail("levels must be in [0, 24]");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    if ((count << levels) > 0x7fffffff) return fail("batch too large");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return c->p.narrow ? cmux_tree_impl<uint32_t>(c, leaves, levels, rgsw, rgsw_stride, out, count)
+                       : cmux_tree_impl<uint64_t>(c, leaves, levels, rgsw, rgsw_stride, out, count);
+}
+
+// ------------------------------------------ RLWE / RGSW encryption, decryption
+
+// which: 0 RLWE_encrypt, 1 RGSW encryption, 2 RLWE_decrypt (host buffers; the
+// work runs on the device with the library's seeded sampler)
+template <typename W>
+static int encrypt_impl(fheacc_ctx *c, int which, const uint64_t *sk, const uint64_t *in, int64_t count,
+                        uint64_t seed, uint64_t first, uint64_t t, uint64_t *out) {
+    const HostParams &p = c->p;
+    const int64_t N = p.N;
+    const int64_t in_words = which == 2 ? 2 * N : N;
+    const int64_t out_words = which == 0 ? 2 * N : which == 1 ? (int64_t)p.digitG * 4 * N : N;
+    DevBuf dz, dzn, dznp, din, dout, dpow, dpowp;
+    HIP_TRY(hipMalloc(&dz.p, N * 8));
+    HIP_TRY(hipMemcpy(dz.p, sk, N * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&dzn.p, N * sizeof(W)));
+    HIP_TRY(hipMalloc(&dznp.p, N * sizeof(W)));
+    HIP_TRY(hipMalloc(&din.p, (size_t)count * in_words * 8));
+    HIP_TRY(hipMemcpy(din.p, in, (size_t)count * in_words * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&dout.p, (size_t)count * out_words * 8));
+    std::vector<uint64_t> pw(p.digitG), pwp(p.digitG);
+    for (uint32_t h = 0; h < p.digitG; h++) {
+        pw[h] = h ? mulmodh(pw[h - 1], (1ULL << p.BGbits) % p.Q, p.Q) : 1 % p.Q;
+        pwp[h] = p.narrow ? shoup32h((uint32_t)pw[h], (uint32_t)p.Q) : shoup64h(pw[h], p.Q);
+    }
+    HIP_TRY(hipMalloc(&dpow.p, p.digitG * 8));
+    HIP_TRY(hipMalloc(&dpowp.p, p.digitG * 8));
+    HIP_TRY(hipMemcpy(dpow.p, pw.data(), p.digitG * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dpowp.p, pwp.data(), p.digitG * 8, hipMemcpyHostToDevice));
+    EncArgs<W> e{};
+    KeygenArgs<W> &kg = e.k;
+    kg.A = make_arith<W>(p);
+    kg.tw = make_tw<W>(c);
+    kg.seed = seed;
+    kg.cdt = c->t.cdt;
+    kg.cdt_len = c->t.cdt_len;
+    kg.E_rlwe = p.E_rlwe;
+    kg.zn = (W *)dzn.p;
+    kg.znp = (W *)dznp.p;
+    kg.sg = make_sg(p.Q);
+    kg.d = p.digitG;
+    kg.bg_pow = (const uint64_t *)dpow.p;
+    kg.bg_powp = (const uint64_t *)dpowp.p;
+    e.msg = (const uint64_t *)din.p;
+    e.ct = (const uint64_t *)din.p;
+    e.out = (uint64_t *)dout.p;
+    e.first = first;
+    e.scale = t ? p.Q / t : 1;
+    HIP_TRY(launch_secret_ntt<W>(kg, p.logN, (const uint64_t *)dz.p, c->stream));
+    HIP_TRY(launch_encrypt<W>(e, which, p.logN, count, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(out, dout.p, (size_t)count * out_words * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int encrypt_call(fheacc_ctx *c, int which, const uint64_t *sk, const uint64_t *in, int64_t count,
+                        uint64_t seed, uint64_t first, uint64_t t, uint64_t *out) {
+    if (check_ctx(c) || !sk || !in || !out) return fail("null argument");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    if (count * (int64_t)c->p.digitG > 0x7fffffff) return fail("batch too large");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    return c->p.narrow ? encrypt_impl<uint32_t>(c, which, sk, in, count, seed, first, t, out)
+                       : encrypt_impl<uint64_t>(c, which, sk, in, count, seed, first, t, out);
+}
+
+int fheacc_rlwe_encrypt(fheacc_ctx *c, const uint64_t *sk, const uint64_t *msg, int64_t count, uint64_t seed,
+                        uint64_t first_index, uint64_t *out) {
+    return encrypt_call(c, 0, sk, msg, count, seed, first_index, 0, out);
+}
+
+int fheacc_rgsw_encrypt(fheacc_ctx *c, const uint64_t *sk, const uint64_t *msg, int64_t count, uint64_t seed,
+                        uint64_t first_index, uint64_t *out) {
+    return encrypt_call(c, 1, sk, msg, count, seed, first_index, 0, out);
+}
+
+int fheacc_rlwe_decrypt(fheacc_ctx *c, const uint64_t *sk, const uint64_t *ct, int64_t count, uint64_t t,
+                        uint64_t *out) {
+    if (c && (t < 2 || t > c->p.Q)) return fail("plaintext modulus t must be in [2, Q]");
+    return encrypt_call(c, 2, sk, ct, count, 0, 0, t, out);
+}
+
 // ------------------------------------- host-buffer forms of the RLWE calls
 
 namespace {
@@ -1365,6 +1548,55 @@ int fheacc_expand_rlwe(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, uint64
     if (int rc = h.out(count * rb * c->p.N, &dout)) return rc;
     if (int rc = fheacc_expand_rlwe_dev(c, k, di, dout, count)) return rc;
     return h.back(out, dout, count * rb * c->p.N);
+}
+
+int fheacc_subs_keyswitch(fheacc_ctx *c, fheacc_keys *k, const uint64_t *in, int sub_factor, uint64_t *out,
+                          int64_t count) {
+    if (check_ctx(c) || !k || !in || !out) return fail("null argument");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    const size_t rb = 2 * (size_t)c->p.N * 8;
+    HIP_TRY(hipSetDevice(c->device));
+    HostCall h(c);
+    uint64_t *di, *dout;
+    if (int rc = h.in(in, count * rb, &di)) return rc;
+    if (int rc = h.out(count * rb, &dout)) return rc;
+    if (int rc = fheacc_subs_keyswitch_dev(c, k, di, sub_factor, dout, count)) return rc;
+    return h.back(out, dout, count * rb);
+}
+
+int fheacc_homo_expand(fheacc_ctx *c, fheacc_keys *k, const uint64_t *packed, const uint64_t *enc_sec,
+                       uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !k || !packed || !enc_sec || !out) return fail("null argument");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    const HostParams &p = c->p;
+    const size_t gb = (size_t)p.digitG * 4 * p.N * 8;
+    HIP_TRY(hipSetDevice(c->device));
+    HostCall h(c);
+    uint64_t *dp, *de, *dout;
+    if (int rc = h.in(packed, count * p.digitG * 2 * (size_t)p.N * 8, &dp)) return rc;
+    if (int rc = h.in(enc_sec, gb, &de)) return rc;
+    if (int rc = h.out(count * (size_t)p.N * gb, &dout)) return rc;
+    if (int rc = fheacc_homo_expand_dev(c, k, dp, de, dout, count)) return rc;
+    return h.back(out, dout, count * (size_t)p.N * gb);
+}
+
+int fheacc_cmux_tree(fheacc_ctx *c, const uint64_t *leaves, int levels, const uint64_t *rgsw, int64_t rgsw_stride,
+                     uint64_t *out, int64_t count) {
+    if (check_ctx(c) || !leaves || !out || (levels > 0 && !rgsw)) return fail("null argument");
+    if (levels < 0 || levels > 24) return fail("levels must be in [0, 24]");
+    if (count <= 0) return count == 0 ? 0 : fail("negative count");
+    const HostParams &p = c->p;
+    const size_t rb = 2 * (size_t)p.N * 8;
+    const int64_t stride = rgsw_stride ? rgsw_stride : (int64_t)p.digitG * 4 * p.N;
+    HIP_TRY(hipSetDevice(c->device));
+    HostCall h(c);
+    uint64_t *dl, *dg = nullptr, *dout;
+    if (int rc = h.in(leaves, (size_t)(count << levels) * rb, &dl)) return rc;
+    if (levels > 0)
+        if (int rc = h.in(rgsw, ((size_t)(levels - 1) * stride + (size_t)p.digitG * 4 * p.N) * 8, &dg)) return rc;
+    if (int rc = h.out(count * rb, &dout)) return rc;
+    if (int rc = fheacc_cmux_tree_dev(c, dl, levels, dg, stride, dout, count)) return rc;
+    return h.back(out, dout, count * rb);
 }
 
 int fheacc_external_product(fheacc_ctx *c, const uint64_t *acc, const uint64_t *rgsw, int64_t stride, uint64_t *out,

@@ -46,7 +46,7 @@ int64_t gauss_sample(const HostParams &p, uint64_t word);
 
 enum : uint64_t {
     TAG_LWE_SK = 1, TAG_RLWE_SK = 2, TAG_BSK = 3, TAG_KSK = 4, TAG_SUBS = 5,
-    TAG_LWE_ENC = 6, TAG_RLWE_ENC = 7
+    TAG_LWE_ENC = 6, TAG_RLWE_ENC = 7, TAG_RGSW_ENC = 8
 };
 __host__ __device__ inline uint64_t stream_id(uint64_t tag, uint64_t idx) { return (tag << 56) | idx; }
 

@@ -82,6 +82,9 @@ struct EPArgs {
     int64_t key_stride;
     int d, bits, pb;
     int64_t c_base;  // ciphertext of workgroup 0
+    // words from one input (output) RLWE to the next: 2N for dense batches;
+    // homo_expand reads and writes rows of RGSWs in place
+    int64_t in_stride, out_stride;
 };
 
 template <typename W>
@@ -95,6 +98,9 @@ struct SubsArgs {
     int sub_factor;
     int d, bits, dch;
     int64_t c_base;  // workgroup index of workgroup 0
+    // expansion slots: slot k of expansion b is the RLWE at in + b set_stride
+    // + k slot_stride (dense [B][N][2][N]: N 2N and 2N; homo_expand: RGSW rows)
+    int64_t set_stride, slot_stride;
 };
 
 // CMUX chains (RLWE.cpp:788-838): per ciphertext acc = c0; for each step
@@ -104,7 +110,8 @@ template <typename W>
 struct CmuxArgs {
     Arith<W> A;
     Twiddles<W> tw;
-    const uint64_t *c0, *c1;  // [B][2][N]; c1 null: rotation steps
+    const uint64_t *c0, *c1;  // ciphertext c at c0 + c in_stride; c1 null: rotation steps
+    int64_t in_stride;        // 2N for dense inputs; the CMUX tree reads leaf pairs
     uint64_t *out;            // [B][2][N]
     const W *key;             // device-layout RGSW of (ciphertext c, step s) at
     int64_t key_stride;       //   key + c key_ct_stride + s key_stride
@@ -146,6 +153,19 @@ struct KeygenArgs {
     const uint64_t *z;                 // RLWE secret, time domain
     const uint64_t *bg_pow, *bg_powp;  // BG^h (subs mode: Bks_R^l) mod Q
     ShoupGen sg;
+};
+
+// RLWE / RGSW encryption of message polynomials and RLWE decryption
+// (RLWE.cpp:324-376, 561-577); keygen fields as above (zn, znp: the secret's
+// transform and its Shoup companions; bg_pow: BG^h mod Q).
+template <typename W>
+struct EncArgs {
+    KeygenArgs<W> k;
+    const uint64_t *msg;   // [count][N] time domain mod Q
+    const uint64_t *ct;    // decrypt input [count][2][N]
+    uint64_t *out;         // RLWE [count][2][N], RGSW [count][d][2][2][N], or messages [count][N]
+    uint64_t first;        // stream index of the first ciphertext
+    uint64_t scale;        // decrypt: Q / t
 };
 
 template <typename W>
@@ -194,6 +214,12 @@ hipError_t launch_ksk_convert(const uint64_t *src, W *dst, uint64_t *back, int B
 template <typename W>
 hipError_t launch_keygen(const KeygenArgs<W> &a, const KskGenArgs<W> &ka, int logN, const uint64_t *z, hipStream_t st,
                          bool subs);
+// which: 0 RLWE encrypt, 1 RGSW encrypt, 2 RLWE decrypt (k.zn must hold the
+// secret's transform: launch_secret_ntt first)
+template <typename W>
+hipError_t launch_secret_ntt(const KeygenArgs<W> &a, int logN, const uint64_t *z, hipStream_t st);
+template <typename W>
+hipError_t launch_encrypt(const EncArgs<W> &a, int which, int logN, int64_t count, hipStream_t st);
 template <typename W>
 hipError_t launch_subs_convert(const void *src, void *dst, int N, int64_t total, bool to_ref, ShoupGen sg,
                                hipStream_t st);

@@ -788,12 +788,12 @@ __global__ __launch_bounds__(NT) void k_external_product(EPArgs<W> a) {
     W *acc = reinterpret_cast<W *>(smem);
     W *dig = acc + padded(2 * N);
     const int64_t c = a.c_base + blockIdx.x;
-    const uint64_t *src = a.acc + c * 2 * N;
+    const uint64_t *src = a.acc + c * a.in_stride;
     for (int i = threadIdx.x; i < 2 * N; i += NT) acc[pad(i)] = (W)src[i];
     __syncthreads();
     if constexpr (INPLACE) ext_prod_inplace<W, LOGN, NT>(acc, a.key + c * a.key_stride, a.A, a.tw, a.d, a.bits);
     else ext_prod<W, LOGN, NT>(acc, dig, a.key + c * a.key_stride, a.A, a.tw, a.d, a.bits, a.pb);
-    uint64_t *dst = a.out + c * 2 * N;
+    uint64_t *dst = a.out + c * a.out_stride;
     for (int i = threadIdx.x; i < 2 * N; i += NT) dst[i] = acc[pad(i)];
 }
 
@@ -818,9 +818,10 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
     W *y = x + padded(2 * N);            // substituted (2 padded polys)
     W *dig = y + padded(2 * N);          // dch padded digit polys
     const int64_t c = a.c_base + blockIdx.x;
-    int64_t row = c;
-    if constexpr (EXPAND) row = ((c >> a.sub_factor) << LOGN) + (c & ((1 << a.sub_factor) - 1));
-    const uint64_t *src = a.in + row * 2 * N;
+    // EXPAND: workgroup c takes slot c mod 2^i of expansion c / 2^i
+    const int64_t off = EXPAND ? (c >> a.sub_factor) * a.set_stride + (c & ((1 << a.sub_factor) - 1)) * a.slot_stride
+                               : c * 2 * N;
+    const uint64_t *src = a.in + off;
     for (int i = threadIdx.x; i < 2 * N; i += NT) x[pad(i)] = (W)src[i];
     __syncthreads();
     ntt_inverse<LOGN, NT, MAXR>(x, 2, a.A, a.tw);
@@ -887,7 +888,7 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
     } else {
         // slot j is re-read from memory (this workgroup alone writes it, and
         // only after every thread has read its own words)
-        uint64_t *lo = a.out + row * 2 * N, *hi = lo + ((int64_t)2 * N << a.sub_factor);
+        uint64_t *lo = a.out + off, *hi = lo + (a.slot_stride << a.sub_factor);
         const int e = 2 * N - (1 << a.sub_factor);
 #pragma unroll
         for (int cc = 0; cc < CPT; cc++) {
@@ -925,7 +926,7 @@ __global__ __launch_bounds__(NT) void k_cmux_chain(CmuxArgs<W> a) {
     W *acc = reinterpret_cast<W *>(smem);
     W *dig = acc + padded(2 * N);
     const int64_t c = a.c_base + blockIdx.x;
-    const uint64_t *src = a.c0 + c * 2 * N;
+    const uint64_t *src = a.c0 + c * a.in_stride;
     const uint64_t Q = a.A.Q;
     W sav[2][CPT];
 #pragma unroll
@@ -948,7 +949,7 @@ __global__ __launch_bounds__(NT) void k_cmux_chain(CmuxArgs<W> a) {
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const W v = sav[h][cc];
-                    const W o = a.c1 ? (W)a.c1[c * 2 * N + h * N + k] : mul_monomial<W, LOGN>(a.A, a.tw, v, k, e);
+                    const W o = a.c1 ? (W)a.c1[c * a.in_stride + h * N + k] : mul_monomial<W, LOGN>(a.A, a.tw, v, k, e);
                     acc[pad(h * N + k)] = o >= v ? o - v : o + (W)Q - v;
                 }
             }
@@ -1244,6 +1245,90 @@ __global__ __launch_bounds__(NT) void k_subs_gen(KeygenArgs<W> a) {
     }
 }
 
+// RLWE_encrypt (RLWE.cpp:324-344) of message polynomials given in the time
+// domain: the encryption of zero of stream (TAG_RLWE_ENC, first + c), then
+// NTT(m) added to b.  One workgroup per ciphertext.
+template <typename W, int LOGN, int NT>
+__global__ __launch_bounds__(NT) void k_rlwe_encrypt(EncArgs<W> a) {
+    constexpr int N = 1 << LOGN;
+    extern __shared__ __align__(16) unsigned char smem[];
+    W *buf = reinterpret_cast<W *>(smem);  // the encryption of zero (a, b)
+    W *m = buf + padded(2 * N);            // the message
+    const int64_t c = blockIdx.x;
+    const uint64_t Q = a.k.A.Q;
+    for (int k = threadIdx.x; k < N; k += NT) m[pad(k)] = (W)(a.msg[c * N + k] % Q);
+    __syncthreads();
+    ntt_forward<LOGN, NT>(m, 1, a.k.A, a.k.tw);
+    rlwe_enc_zero<W, LOGN, NT>(buf, a.k.zn, a.k.znp, a.k.seed, stream_id(TAG_RLWE_ENC, a.first + c), a.k);
+    uint64_t *o = a.out + c * 2 * N;
+    for (int k = threadIdx.x; k < N; k += NT) {
+        const uint64_t t = (uint64_t)buf[pad(N + k)] + m[pad(k)];
+        o[k] = buf[pad(k)];
+        o[N + k] = t >= Q ? t - Q : t;
+    }
+}
+
+// RGSW encryption of message polynomials (RLWE.cpp:561-577), written in the
+// reference RGSW layout: workgroup (e, h) writes row h of RGSW e, column g =
+// the encryption of zero of stream (TAG_RGSW_ENC, (first + e) 2d) + 2h + g
+// with NTT(m) BG^h added to a (g = 0) or b (g = 1).
+template <typename W, int LOGN, int NT>
+__global__ __launch_bounds__(NT) void k_rgsw_encrypt(EncArgs<W> a) {
+    constexpr int N = 1 << LOGN;
+    extern __shared__ __align__(16) unsigned char smem[];
+    W *buf = reinterpret_cast<W *>(smem);
+    W *m = buf + padded(2 * N);
+    const int d = a.k.d;
+    const int64_t e = blockIdx.x / d;
+    const int h = blockIdx.x % d;
+    const uint64_t Q = a.k.A.Q;
+    for (int k = threadIdx.x; k < N; k += NT) m[pad(k)] = (W)(a.msg[e * N + k] % Q);
+    __syncthreads();
+    ntt_forward<LOGN, NT>(m, 1, a.k.A, a.k.tw);
+    for (int k = threadIdx.x; k < N; k += NT)
+        m[pad(k)] = mulmod_generic<W>(a.k.A, m[pad(k)], (W)a.k.bg_pow[h], (W)a.k.bg_powp[h]);
+    const uint64_t sbase = stream_id(TAG_RGSW_ENC, (a.first + e) * 2 * d) + 2 * (uint64_t)h;
+    uint64_t *o = a.out + ((size_t)e * d + h) * 4 * N;
+    for (int g = 0; g < 2; g++) {
+        __syncthreads();  // buf is rewritten; m is complete
+        rlwe_enc_zero<W, LOGN, NT>(buf, a.k.zn, a.k.znp, a.k.seed, sbase + g, a.k);
+        for (int k = threadIdx.x; k < N; k += NT) {
+            uint64_t va = buf[pad(k)], vb = buf[pad(N + k)];
+            uint64_t &t = g == 0 ? va : vb;
+            t += m[pad(k)];
+            t = t >= Q ? t - Q : t;
+            o[(size_t)g * 2 * N + k] = va;
+            o[(size_t)g * 2 * N + N + k] = vb;
+        }
+    }
+}
+
+// RLWE_decrypt(s, c, t) (RLWE.cpp:362-376): b - a s in the NTT domain, the
+// inverse transform, then denoise(t) (RLWE.cpp:138-155): (x + scale/2) mod Q
+// divided by scale = Q / t.
+template <typename W, int LOGN, int NT>
+__global__ __launch_bounds__(NT) void k_rlwe_decrypt(EncArgs<W> a) {
+    constexpr int N = 1 << LOGN;
+    extern __shared__ __align__(16) unsigned char smem[];
+    W *m = reinterpret_cast<W *>(smem);
+    const int64_t c = blockIdx.x;
+    const uint64_t Q = a.k.A.Q;
+    const uint64_t *x = a.ct + c * 2 * N;
+    for (int k = threadIdx.x; k < N; k += NT) {
+        const W as = mulmod_generic<W>(a.k.A, (W)x[k], a.k.zn[k], a.k.znp[k]);
+        const W b = (W)x[N + k];
+        m[pad(k)] = b >= as ? b - as : (W)(b + (Q - as));
+    }
+    __syncthreads();
+    ntt_inverse<LOGN, NT>(m, 1, a.k.A, a.k.tw);
+    const uint64_t half = a.scale / 2;
+    for (int k = threadIdx.x; k < N; k += NT) {
+        uint64_t v = (uint64_t)m[pad(k)] + half;
+        v = v >= Q ? v - Q : v;
+        a.out[c * N + k] = v / a.scale;
+    }
+}
+
 // subs key device -> reference [logN][dks_R][2][N]
 template <typename W>
 __global__ void k_subs_to_ref(const W *src, uint64_t *dst, int N, int64_t total) {
@@ -1515,6 +1600,37 @@ hipError_t launch_keygen(const KeygenArgs<W> &a, const KskGenArgs<W> &ka, int lo
             const int64_t wgs = (int64_t)a.n * a.digitR * a.Br * a.d;
             hipLaunchKernelGGL(k, dim3((unsigned)wgs), dim3(256), lds, st, a);
             hipLaunchKernelGGL(k_ksk_gen<W>, dim3((unsigned)((ka.rows + 3) / 4)), dim3(256), 0, st, ka);
+        }
+    })
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_secret_ntt(const KeygenArgs<W> &a, int logN, const uint64_t *z, hipStream_t st) {
+    FHEACC_LOGN_SWITCH(logN, {
+        constexpr int N = 1 << LOGN;
+        hipLaunchKernelGGL((k_secret_ntt<W, LOGN, 256>), dim3(1), dim3(256), padded(N) * sizeof(W), st, z, (W *)a.zn,
+                           (W *)a.znp, a.A, a.tw, a.sg);
+    })
+    return hipGetLastError();
+}
+
+template <typename W>
+hipError_t launch_encrypt(const EncArgs<W> &a, int which, int logN, int64_t count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    if (count * (which == 1 ? a.k.d : 1) > 0x7fffffff) return hipErrorInvalidValue;
+    FHEACC_LOGN_SWITCH(logN, {
+        constexpr int N = 1 << LOGN;
+        constexpr int NT = 256;
+        if (which == 0) {
+            hipLaunchKernelGGL((k_rlwe_encrypt<W, LOGN, NT>), dim3((unsigned)count), dim3(NT),
+                               (padded(2 * N) + padded(N)) * sizeof(W), st, a);
+        } else if (which == 1) {
+            hipLaunchKernelGGL((k_rgsw_encrypt<W, LOGN, NT>), dim3((unsigned)(count * a.k.d)), dim3(NT),
+                               (padded(2 * N) + padded(N)) * sizeof(W), st, a);
+        } else {
+            hipLaunchKernelGGL((k_rlwe_decrypt<W, LOGN, NT>), dim3((unsigned)count), dim3(NT), padded(N) * sizeof(W),
+                               st, a);
         }
     })
     return hipGetLastError();

@@ -43,7 +43,9 @@ template size_t ks_staged_lds<uint64_t>(uint32_t);
                                               hipStream_t);                                                       \
     template hipError_t launch_keygen<W>(const KeygenArgs<W> &, const KskGenArgs<W> &, int, const uint64_t *,     \
                                          hipStream_t, bool);                                                      \
-    template hipError_t launch_subs_convert<W>(const void *, void *, int, int64_t, bool, ShoupGen, hipStream_t);
+    template hipError_t launch_subs_convert<W>(const void *, void *, int, int64_t, bool, ShoupGen, hipStream_t); \
+    template hipError_t launch_secret_ntt<W>(const KeygenArgs<W> &, int, const uint64_t *, hipStream_t);          \
+    template hipError_t launch_encrypt<W>(const EncArgs<W> &, int, int, int64_t, hipStream_t);
 
 FHEACC_INSTANTIATE(uint32_t)
 FHEACC_INSTANTIATE(uint64_t)

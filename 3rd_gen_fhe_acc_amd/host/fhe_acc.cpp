@@ -142,6 +142,41 @@ static void flat_rgsw(const fheacc::Context &ctx, const RGSW_ciphertext &C, std:
     }
 }
 
+static std::vector<ciphertext_t> plain_words(const fheacc::Context &ctx, const RLWE_plaintext &p) {
+    if ((int)p.m.size() != ctx.N()) throw fheacc::Error(-1, "RLWE plaintext must hold N words");
+    return p.m;
+}
+
+RLWE_ciphertext RLWE_encrypt(const fheacc::Context &ctx, const RLWE_secretkey &s, const RLWE_plaintext &p,
+                             uint64_t seed, uint64_t index) {
+    std::vector<ciphertext_t> m = plain_words(ctx, p), out(2 * (size_t)ctx.N());
+    fheacc::check(fheacc_rlwe_encrypt(ctx.handle(), s.a.data(), m.data(), 1, seed, index, out.data()));
+    return unflat(ctx.N(), out.data());
+}
+
+RLWE_plaintext RLWE_decrypt(const fheacc::Context &ctx, const RLWE_secretkey &s, const RLWE_ciphertext &c,
+                            ciphertext_t t) {
+    std::vector<ciphertext_t> x = flat(ctx, c), out(ctx.N());
+    fheacc::check(fheacc_rlwe_decrypt(ctx.handle(), s.a.data(), x.data(), 1, t, out.data()));
+    return RLWE_plaintext(out);
+}
+
+static RGSW_ciphertext unflat_rgsw(const fheacc::Context &ctx, const ciphertext_t *w) {
+    const uint32_t d = ctx.info().digitG;
+    const int N = ctx.N();
+    std::vector<std::vector<RLWE_ciphertext>> rows(d);
+    for (uint32_t h = 0; h < d; h++)
+        for (int g = 0; g < 2; g++) rows[h].push_back(unflat(N, w + ((size_t)h * 2 + g) * 2 * N));
+    return RGSW_ciphertext(std::move(rows));
+}
+
+RGSW_ciphertext RGSW_encrypt(const fheacc::Context &ctx, const RLWE_secretkey &z, const RGSW_plaintext &m,
+                             uint64_t seed, uint64_t index) {
+    std::vector<ciphertext_t> w = plain_words(ctx, m.m), out((size_t)ctx.info().digitG * 4 * ctx.N());
+    fheacc::check(fheacc_rgsw_encrypt(ctx.handle(), z.a.data(), w.data(), 1, seed, index, out.data()));
+    return unflat_rgsw(ctx, out.data());
+}
+
 RLWE_ciphertext RLWE_mult_RGSW(const fheacc::Context &ctx, const RLWE_ciphertext &c, const RGSW_ciphertext &C) {
     std::vector<ciphertext_t> in = flat(ctx, c), g, out(2 * (size_t)ctx.N());
     flat_rgsw(ctx, C, g);
@@ -199,6 +234,28 @@ std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> expand_RLWE(const RLWE::RLWE
     fheacc::check(fheacc_expand_rlwe(ctx.handle(), subs_key.keys->handle(), in.data(), out.data(), 1));
     auto res = std::make_unique<std::vector<RLWE::RLWE_ciphertext>>(N);
     for (int k = 0; k < N; k++) (*res)[k] = RLWE::unflat(N, out.data() + (size_t)k * 2 * N);
+    return res;
+}
+
+std::shared_ptr<std::vector<RLWE::RGSW_ciphertext>> homo_expand(
+    const std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> &packed_bits, const RLWE::RGSW_ciphertext &enc_sec,
+    const substitute_key &subs_key) {
+    if (!subs_key.keys) throw fheacc::Error(-1, "substitute key not generated");
+    const fheacc::Context &ctx = subs_key.keys->context();
+    const int N = ctx.N();
+    const uint32_t d = ctx.info().digitG;
+    if (!packed_bits || packed_bits->size() != d) throw fheacc::Error(-1, "homo_expand takes digitG packed RLWEs");
+    std::vector<ciphertext_t> in, g;
+    for (const RLWE::RLWE_ciphertext &c : *packed_bits) {
+        std::vector<ciphertext_t> w = RLWE::flat(ctx, c);
+        in.insert(in.end(), w.begin(), w.end());
+    }
+    RLWE::flat_rgsw(ctx, enc_sec, g);
+    const size_t gw = (size_t)d * 4 * N;
+    std::vector<ciphertext_t> out((size_t)N * gw);
+    fheacc::check(fheacc_homo_expand(ctx.handle(), subs_key.keys->handle(), in.data(), g.data(), out.data(), 1));
+    auto res = std::make_shared<std::vector<RLWE::RGSW_ciphertext>>(N);
+    for (int i = 0; i < N; i++) (*res)[i] = RLWE::unflat_rgsw(ctx, out.data() + (size_t)i * gw);
     return res;
 }
 
@@ -315,6 +372,42 @@ std::vector<LWE::LWE_ciphertext> fpga_eval_bootstrap_x16(const std::vector<LWE::
                                                          const LWE::keyswitch_key &kskey,
                                                          const std::vector<GATES> gate, int *rc) {
     return in_order(c1, c2, kskey, gate, 10, FHEACC_EVAL_FPGA_XN, "fpga_eval_bootstrap_x16", rc);
+}
+
+std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> fpga_expand_RLWE(const RLWE::RLWE_ciphertext &c1,
+                                                                     const application::substitute_key &subs_key,
+                                                                     int *rc) {
+    *rc = 0;
+    try {
+        return application::expand_RLWE(c1, subs_key);
+    } catch (const fheacc::Error &e) {
+        fprintf(stderr, "fpga_expand_RLWE: %s\n", e.what());
+        *rc = e.rc ? e.rc : -1;
+        return nullptr;
+    }
+}
+
+std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> fpga_expand_RLWE_cont_rd_wr(
+    const RLWE::RLWE_ciphertext &c1, const application::substitute_key &subs_key, int *rc) {
+    return fpga_expand_RLWE(c1, subs_key, rc);
+}
+
+std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> fpga_expand_RLWE_batch_rd_wr(
+    const RLWE::RLWE_ciphertext &c1, const application::substitute_key &subs_key, int *rc) {
+    return fpga_expand_RLWE(c1, subs_key, rc);
+}
+
+std::shared_ptr<std::vector<RLWE::RGSW_ciphertext>> fpga_homo_expand(
+    const std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> &packed_bits, const RLWE::RGSW_ciphertext &enc_sec,
+    const application::substitute_key &subs_key, int *rc) {
+    *rc = 0;
+    try {
+        return application::homo_expand(packed_bits, enc_sec, subs_key);
+    } catch (const fheacc::Error &e) {
+        fprintf(stderr, "fpga_homo_expand: %s\n", e.what());
+        *rc = e.rc ? e.rc : -1;
+        return nullptr;
+    }
 }
 
 }  // namespace FPGA

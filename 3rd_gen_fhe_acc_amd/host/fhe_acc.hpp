@@ -21,6 +21,12 @@
 //   RLWE::CMUX / blind_rotate / blind_rotate_full  same names, ctx first
 //   application::substitute_key(sk, uni, norm)  application::substitute_key(ctx, sk, seed)
 //   application::expand_RLWE(c, subs_key)       same name and arguments
+//   application::homo_expand(bits, enc_sec, sk) same name and arguments
+//   RLWE::RLWE_encrypt(s, p, uni, norm)         RLWE::RLWE_encrypt(ctx, s, p, seed, index)
+//   RLWE::RLWE_decrypt(s, c, t)                 RLWE::RLWE_decrypt(ctx, s, c, t)
+//   RGSW_ciphertext(m, z, uni, norm)            RLWE::RGSW_encrypt(ctx, z, m, seed, index)
+//   FPGA::fpga_expand_RLWE{,_cont_rd_wr,_batch_rd_wr}, fpga_homo_expand
+//     (bar, rc, fds; keys DMA'd to the card)    same names (subs key / RGSW(-s) passed, rc)
 //
 // Keys live on the device; the classes hold shared handles.  eval_bootstrap
 // on identical operands prints the reference's message and exits the process
@@ -192,6 +198,34 @@ class RGSW_ciphertext {
     std::vector<std::vector<RLWE_ciphertext>> c_text;
 };
 
+// RLWE.h RLWE_plaintext: the message polynomial, here kept in the time domain
+// (the reference's RLWE_plaintext(m, false) constructor transforms it and
+// RLWE_decrypt's denoise transforms it back, RLWE.cpp:71-84,138-155)
+class RLWE_plaintext {
+   public:
+    RLWE_plaintext() = default;
+    explicit RLWE_plaintext(std::vector<ciphertext_t> m_time) : m(std::move(m_time)) {}
+    std::vector<ciphertext_t> m;  // time domain, mod Q
+};
+
+// RLWE.h RGSW_plaintext
+class RGSW_plaintext {
+   public:
+    RGSW_plaintext() = default;
+    explicit RGSW_plaintext(RLWE_plaintext m_in) : m(std::move(m_in)) {}
+    RLWE_plaintext m;
+};
+
+// RLWE.cpp:324-344; seeded (stream `index` of the library's sampler)
+RLWE_ciphertext RLWE_encrypt(const fheacc::Context &ctx, const RLWE_secretkey &s, const RLWE_plaintext &p,
+                             uint64_t seed, uint64_t index);
+// RLWE.cpp:362-376: RLWE_decrypt(s, c, t), the message rounded to [0, t]
+RLWE_plaintext RLWE_decrypt(const fheacc::Context &ctx, const RLWE_secretkey &s, const RLWE_ciphertext &c,
+                            ciphertext_t t);
+// RLWE.cpp:561-577: RGSW_ciphertext(m, z, uni, norm), seeded
+RGSW_ciphertext RGSW_encrypt(const fheacc::Context &ctx, const RLWE_secretkey &z, const RGSW_plaintext &m,
+                             uint64_t seed, uint64_t index);
+
 // RLWE.cpp:642-704
 RLWE_ciphertext RLWE_mult_RGSW(const fheacc::Context &ctx, const RLWE_ciphertext &c, const RGSW_ciphertext &C);
 // RLWE.cpp:788-793
@@ -220,6 +254,11 @@ class substitute_key {
 // app.cpp:26-97
 std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> expand_RLWE(const RLWE::RLWE_ciphertext &c1,
                                                                 const substitute_key &subs_key);
+// app.cpp:78-97: digitG packed RLWEs -> N RGSWs, RGSW i row h =
+// (u_h[i] (x) enc_sec, u_h[i]) with u_h = expand_RLWE(packed_bits[h])
+std::shared_ptr<std::vector<RLWE::RGSW_ciphertext>> homo_expand(
+    const std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> &packed_bits, const RLWE::RGSW_ciphertext &enc_sec,
+    const substitute_key &subs_key);
 
 }  // namespace application
 
@@ -267,5 +306,24 @@ std::vector<LWE::LWE_ciphertext> fpga_eval_bootstrap_x16(const std::vector<LWE::
                                                          const std::vector<LWE::LWE_ciphertext> &c2,
                                                          const LWE::keyswitch_key &kskey,
                                                          const std::vector<GATES> gate, int *rc);
+
+// The accelerator's expansion calls (myfpga.h:342-359).  The card reads the
+// subs key and RGSW(-s) it was sent beforehand (dma_write_subs_key,
+// dma_write_RGSW_enc_sk); here they are the resident substitute_key and an
+// RGSW passed in.  The card runs RLWESUBS and the host the additions and
+// rotations (myfpga.cpp:1662-2276), the CPU form's words: the three
+// expansion variants differ only in how they stream RLWEs over PCIe, so all
+// three are the one device expansion.  *rc = 0 or -1 (the failure printed);
+// a failed call returns a null pointer, as the card's calls do.
+std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> fpga_expand_RLWE(const RLWE::RLWE_ciphertext &c1,
+                                                                     const application::substitute_key &subs_key,
+                                                                     int *rc);
+std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> fpga_expand_RLWE_cont_rd_wr(
+    const RLWE::RLWE_ciphertext &c1, const application::substitute_key &subs_key, int *rc);
+std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> fpga_expand_RLWE_batch_rd_wr(
+    const RLWE::RLWE_ciphertext &c1, const application::substitute_key &subs_key, int *rc);
+std::shared_ptr<std::vector<RLWE::RGSW_ciphertext>> fpga_homo_expand(
+    const std::unique_ptr<std::vector<RLWE::RLWE_ciphertext>> &packed_bits, const RLWE::RGSW_ciphertext &enc_sec,
+    const application::substitute_key &subs_key, int *rc);
 
 }  // namespace FPGA

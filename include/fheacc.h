@@ -194,6 +194,24 @@ int fheacc_lwe_encrypt(const fheacc_ctx *ctx, const uint64_t *sk, const uint8_t 
                        uint64_t seed, uint64_t first_index, uint64_t *out);
 int fheacc_lwe_decrypt(const fheacc_ctx *ctx, const uint64_t *sk, const uint64_t *c, int64_t count,
                        uint8_t *bits);
+/* RLWE::RLWE_encrypt of message polynomials (RLWE.cpp:324-344, with
+ * RLWE_plaintext(m, false) transforming m, RLWE.cpp:71-84): sk [N] time
+ * domain; msg [count][N] time domain mod Q; out [count][2][N] NTT domain.
+ * Ciphertext i draws from the seeded sampler's stream first_index + i.  Runs
+ * on the device; host buffers. */
+int fheacc_rlwe_encrypt(fheacc_ctx *ctx, const uint64_t *sk, const uint64_t *msg, int64_t count, uint64_t seed,
+                        uint64_t first_index, uint64_t *out);
+/* RGSW encryption of message polynomials, RGSW_ciphertext(m, z, ...)
+ * (RLWE.cpp:561-577): out [count] RGSWs in the reference layout
+ * [digitG][2][2][N], row h adding NTT(m) BG^h to a (column 0) or b
+ * (column 1).  An RGSW of -s feeds fheacc_homo_expand. */
+int fheacc_rgsw_encrypt(fheacc_ctx *ctx, const uint64_t *sk, const uint64_t *msg, int64_t count, uint64_t seed,
+                        uint64_t first_index, uint64_t *out);
+/* RLWE::RLWE_decrypt(s, c, t) (RLWE.cpp:362-376): b - a s, to the time
+ * domain, rounded to multiples of Q / t (denoise, RLWE.cpp:138-155);
+ * out [count][N] in [0, t]. */
+int fheacc_rlwe_decrypt(fheacc_ctx *ctx, const uint64_t *sk, const uint64_t *ct, int64_t count, uint64_t t,
+                        uint64_t *out);
 
 /* ---- the gate path ------------------------------------------------------
  * RLWE::eval_bootstrap (RLWE.cpp:726-786) for B independent pairs, and the
@@ -267,6 +285,22 @@ int fheacc_subs_keyswitch_dev(fheacc_ctx *ctx, fheacc_keys *keys, const uint64_t
  * [count][N][2][N], slot k of input c at out + (c N + k) 2N.  Needs the subs
  * key.  in and out must not overlap. */
 int fheacc_expand_rlwe_dev(fheacc_ctx *ctx, fheacc_keys *keys, const uint64_t *in, uint64_t *out, int64_t count);
+/* application::homo_expand (app.cpp:78-97) and the accelerator's
+ * fpga_homo_expand (myfpga.h:359, myfpga.cpp:2278-2452) for count queries:
+ * packed [count][digitG][2][N] (packed[h] encrypts sum_i b_i iN BG^h X^i),
+ * enc_sec one RGSW of -s in the reference layout; out [count][N] RGSWs in
+ * the reference layout, RGSW i row h = (u_h[i] (x) enc_sec, u_h[i]) with
+ * u_h = expand_RLWE(packed[h]).  Needs the subs key. */
+int fheacc_homo_expand_dev(fheacc_ctx *ctx, fheacc_keys *keys, const uint64_t *packed, const uint64_t *enc_sec,
+                           uint64_t *out, int64_t count);
+/* The CMUX tree of the reference's tree selection (test.cpp:1576-1582 and the
+ * accelerator's RLWE_MULT_RGSW form, test.cpp:4185-4330): count trees of
+ * leaves [count][2^levels][2][N]; level l (< levels) halves every tree,
+ * v[j/2] = CMUX(v[j], v[j+1], C[l]), with C[l] the reference-layout RGSW at
+ * rgsw + l rgsw_stride (0: RGSWs back to back); out [count][2][N].  The
+ * reference then blind-rotates the result (fheacc_blind_rotate_rlwe_dev). */
+int fheacc_cmux_tree_dev(fheacc_ctx *ctx, const uint64_t *leaves, int levels, const uint64_t *rgsw,
+                         int64_t rgsw_stride, uint64_t *out, int64_t count);
 /* RLWE::CMUX (RLWE.cpp:788-793): out[i] = (c1[i] - c0[i]) (x) rgsw[i] + c0[i];
  * rgsw in the reference RGSW layout, rgsw_stride words apart (0: one shared
  * RGSW). */
@@ -287,6 +321,12 @@ int fheacc_blind_rotate_full_dev(fheacc_ctx *ctx, const uint64_t *in, int b, con
  * them): copies in, the device call, copies out; they return when out holds
  * the result. */
 int fheacc_expand_rlwe(fheacc_ctx *ctx, fheacc_keys *keys, const uint64_t *in, uint64_t *out, int64_t count);
+int fheacc_subs_keyswitch(fheacc_ctx *ctx, fheacc_keys *keys, const uint64_t *in, int sub_factor, uint64_t *out,
+                          int64_t count);
+int fheacc_homo_expand(fheacc_ctx *ctx, fheacc_keys *keys, const uint64_t *packed, const uint64_t *enc_sec,
+                       uint64_t *out, int64_t count);
+int fheacc_cmux_tree(fheacc_ctx *ctx, const uint64_t *leaves, int levels, const uint64_t *rgsw, int64_t rgsw_stride,
+                     uint64_t *out, int64_t count);
 int fheacc_external_product(fheacc_ctx *ctx, const uint64_t *acc, const uint64_t *rgsw, int64_t rgsw_stride,
                             uint64_t *out, int64_t count);
 int fheacc_cmux(fheacc_ctx *ctx, const uint64_t *c0, const uint64_t *c1, const uint64_t *rgsw, int64_t rgsw_stride,

@@ -102,6 +102,11 @@ def lib():
             "orc_rlwe_ksk_gen": (None, [P, U, U, C.c_uint64, C.c_uint64, U]),
             "orc_subs_key_gen": (None, [P, U, C.c_uint64, U]),
             "orc_lwe_encrypt": (None, [P, U, C.c_uint64, C.c_uint64, C.c_uint64, U]),
+            "orc_rlwe_encrypt": (None, [P, U, U, C.c_uint64, C.c_uint64, U]),
+            "orc_rlwe_decrypt": (None, [P, U, U, C.c_uint64, U]),
+            "orc_rgsw_encrypt": (None, [P, U, U, C.c_uint64, C.c_uint64, U]),
+            "orc_homo_expand": (None, [P, U, U, U, U]),
+            "orc_cmux_tree": (None, [P, U, C.c_int, U, C.c_size_t, U]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -372,6 +377,57 @@ class Oracle:
     def gauss(self, word):
         return lib().orc_gauss(self.ref, word)
 
+    # RLWE / RGSW encryption of message polynomials (time domain, mod Q)
+    def rlwe_encrypt(self, sk_time, msgs, seed, first_index=0):
+        """RLWE_encrypt (RLWE.cpp:324-344) of each row of msgs; ciphertext i
+        draws from stream (TAG_RLWE_ENC, first_index + i)."""
+        msgs = np.ascontiguousarray(msgs, dtype=np.uint64).reshape(-1, self.N)
+        sk_ntt = self.ntt(np.asarray(sk_time, dtype=np.uint64))
+        out = np.zeros((msgs.shape[0], 2, self.N), dtype=np.uint64)
+        for i in range(msgs.shape[0]):
+            lib().orc_rlwe_encrypt(self.ref, _u64(sk_ntt)[1], _u64(msgs[i])[1], seed,
+                                   stream(TAG_RLWE_ENC, first_index + i), _u64(out[i])[1])
+        return out
+
+    def rlwe_decrypt(self, sk_time, c, t):
+        """RLWE_decrypt(s, c, t) (RLWE.cpp:362-376): [count][N] time domain."""
+        c = np.ascontiguousarray(c, dtype=np.uint64).reshape(-1, 2, self.N)
+        sk_ntt = self.ntt(np.asarray(sk_time, dtype=np.uint64))
+        out = np.zeros((c.shape[0], self.N), dtype=np.uint64)
+        for i in range(c.shape[0]):
+            lib().orc_rlwe_decrypt(self.ref, _u64(sk_ntt)[1], _u64(c[i])[1], t, _u64(out[i])[1])
+        return out
+
+    def rgsw_encrypt(self, sk_time, msgs, seed, first_index=0):
+        """RGSW encryption (RLWE.cpp:561-577) of each row of msgs; RGSW i draws
+        from streams (TAG_RGSW_ENC, (first_index + i) 2 digitG + 2h + g)."""
+        msgs = np.ascontiguousarray(msgs, dtype=np.uint64).reshape(-1, self.N)
+        sk_ntt = self.ntt(np.asarray(sk_time, dtype=np.uint64))
+        out = np.zeros((msgs.shape[0], self.rgsw_words), dtype=np.uint64)
+        for i in range(msgs.shape[0]):
+            base = stream(TAG_RGSW_ENC, (first_index + i) * 2 * self.digitG)
+            lib().orc_rgsw_encrypt(self.ref, _u64(sk_ntt)[1], _u64(msgs[i])[1], seed, base, _u64(out[i])[1])
+        return out
+
+    def homo_expand(self, packed, enc_sec, subs):
+        """homo_expand (app.cpp:78-97): packed [digitG][2][N] -> [N] RGSWs."""
+        packed, pp = _u64(np.asarray(packed).reshape(self.digitG, 2, self.N))
+        enc_sec, pe = _u64(enc_sec)
+        subs, pk = _u64(subs)
+        out = np.zeros((self.N, self.rgsw_words), dtype=np.uint64)
+        lib().orc_homo_expand(self.ref, pp, pe, pk, _u64(out)[1])
+        return out
+
+    def cmux_tree(self, leaves, rgsws):
+        """The tree selection's CMUX tree (test.cpp:1576-1582): 2^L leaves,
+        L RGSWs (level l selects with rgsws[l])."""
+        rgsws = np.ascontiguousarray(rgsws, dtype=np.uint64).reshape(-1, self.rgsw_words)
+        levels = rgsws.shape[0]
+        leaves = np.ascontiguousarray(leaves, dtype=np.uint64).reshape(1 << levels, 2, self.N)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        lib().orc_cmux_tree(self.ref, _u64(leaves)[1], levels, _u64(rgsws)[1], self.rgsw_words, _u64(out)[1])
+        return out
+
 
 def rng_word(seed, stream, ctr):
     return lib().orc_rng_word(seed, stream, ctr)
@@ -379,3 +435,7 @@ def rng_word(seed, stream, ctr):
 
 def stream(tag, idx):
     return (tag << 56) | idx
+
+
+# stream tags shared with the device sampler (fhe_oracle.h ORC_TAG_*)
+TAG_RLWE_ENC, TAG_RGSW_ENC = 7, 8

@@ -28,6 +28,13 @@
  *   orc_blind_rotate / orc_eval_bootstrap -> oracle/_ref (reference
  *           LWE.cpp/RLWE.cpp compiled here, explicit keys), scale_round closed
  *           form, transpose invariant, decrypted truth tables.
+ *   orc_homo_expand / orc_cmux_tree -> oracle/_ref (the reference's
+ *           application::homo_expand and RLWE::CMUX driven in the tree order of
+ *           test.cpp:1576-1582, compiled here); an expanded packing of bits
+ *           decrypts row by row to b_i BG^h; the tree selects its leaf.
+ *   orc_rlwe_encrypt / orc_rgsw_encrypt -> exact identities: iNTT(b - a s)
+ *           equals the sampled error plus the message, row by row.
+ *   orc_rlwe_decrypt -> oracle/_ref (the reference's RLWE_decrypt with t).
  *   orc_rng_word / orc_gauss / keygen -> this library's own sampler (the
  *           reference seeds from std::random_device, RNG.cpp:5-6, so it has no
  *           reproducible keys).  Pinned by decryption of fresh encryptions and
@@ -755,6 +762,95 @@ void orc_rlwe_encrypt_zero(const orc_params *p, const uint64_t *sk_ntt, uint64_t
     orc_ntt(a, N, Q, p->rou);
     orc_ntt(b, N, Q, p->rou);
     for (int k = 0; k < N; k++) b[k] = modu((u128)a[k] * sk_ntt[k] + (u128)b[k], Q);
+}
+
+/* RLWE.cpp:324-344: RLWE_encrypt of a message polynomial given in the time
+ * domain (RLWE_plaintext(m, false) transforms it, RLWE.cpp:71-84): the
+ * encryption of zero of stream `stream`, then NTT(m) added to b. */
+void orc_rlwe_encrypt(const orc_params *p, const uint64_t *sk_ntt, const uint64_t *m_time, uint64_t seed,
+                      uint64_t stream, uint64_t *out) {
+    int N = p->N;
+    uint64_t Q = p->Q;
+    uint64_t *m = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    for (int k = 0; k < N; k++) m[k] = m_time[k] % Q;
+    orc_ntt(m, N, Q, p->rou);
+    orc_rlwe_encrypt_zero(p, sk_ntt, seed, stream, out);
+    for (int k = 0; k < N; k++) out[N + k] = modu((u128)out[N + k] + m[k], Q);
+    free(m);
+}
+
+/* RLWE.cpp:362-376: RLWE_decrypt(s, c, t): b - a s in the NTT domain, to the
+ * time domain, then denoise(t) (RLWE.cpp:138-155): (x + scale/2) mod Q,
+ * divided by scale = Q / t. */
+void orc_rlwe_decrypt(const orc_params *p, const uint64_t *sk_ntt, const uint64_t *c, uint64_t t,
+                      uint64_t *m_out) {
+    int N = p->N;
+    uint64_t Q = p->Q, scale = Q / t;
+    for (int k = 0; k < N; k++) {
+        uint64_t as = modu((u128)c[k] * sk_ntt[k], Q);
+        m_out[k] = mods((s128)c[N + k] - (s128)as, Q);
+    }
+    orc_intt(m_out, N, Q, p->irou, p->iN);
+    for (int k = 0; k < N; k++) m_out[k] = modu((u128)m_out[k] + scale / 2, Q) / scale % Q;
+}
+
+/* RLWE.cpp:561-577: RGSW encryption of a message polynomial (time domain):
+ * row h, column g is an encryption of zero of stream stream_base + 2h + g
+ * with NTT(m) BG^h added to a (g = 0) or to b (g = 1). */
+void orc_rgsw_encrypt(const orc_params *p, const uint64_t *sk_ntt, const uint64_t *m_time, uint64_t seed,
+                      uint64_t stream_base, uint64_t *out) {
+    int N = p->N;
+    uint64_t Q = p->Q;
+    uint64_t *m = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    for (int k = 0; k < N; k++) m[k] = m_time[k] % Q;
+    orc_ntt(m, N, Q, p->rou);
+    for (uint32_t h = 0; h < p->digitG; h++) {
+        for (int g = 0; g < 2; g++) {
+            uint64_t *r = out + ((size_t)h * 2 + g) * 2 * N;
+            orc_rlwe_encrypt_zero(p, sk_ntt, seed, stream_base + (uint64_t)(h * 2 + g), r);
+            uint64_t *dst = r + (size_t)g * N;
+            for (int k = 0; k < N; k++) dst[k] = modu((u128)dst[k] + m[k], Q);
+        }
+        for (int k = 0; k < N; k++) m[k] = modu((u128)p->BG * m[k], Q);
+    }
+    free(m);
+}
+
+/* app.cpp:78-97: digitG expansions of the packed bits, then RGSW i has row j
+ * = (unpacked_j[i] (x) enc_sec, unpacked_j[i]).  packed: [digitG][2][N];
+ * out: [N] RGSW [digitG][2][2][N]. */
+void orc_homo_expand(const orc_params *p, const uint64_t *packed, const uint64_t *enc_sec, const uint64_t *subs,
+                     uint64_t *out) {
+    int N = p->N;
+    size_t rw = (size_t)2 * N, gw = (size_t)p->digitG * 2 * rw;
+    uint64_t *un = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)p->digitG * N * rw);
+    for (uint32_t j = 0; j < p->digitG; j++) orc_expand_rlwe(p, packed + j * rw, subs, un + (size_t)j * N * rw);
+    for (int i = 0; i < N; i++) {
+        for (uint32_t j = 0; j < p->digitG; j++) {
+            const uint64_t *u = un + ((size_t)j * N + i) * rw;
+            uint64_t *row = out + (size_t)i * gw + (size_t)j * 2 * rw;
+            orc_rlwe_mult_rgsw(p, u, enc_sec, row);
+            memcpy(row + rw, u, sizeof(uint64_t) * rw);
+        }
+    }
+    free(un);
+}
+
+/* The CMUX tree of the reference's tree selection (test.cpp:1576-1582):
+ * level l < levels halves the leaves, v[j/2] = CMUX(v[j], v[j+1], C[l]).
+ * leaves: [2^levels][2][N]; rgsw: levels RGSWs rgsw_stride words apart. */
+void orc_cmux_tree(const orc_params *p, const uint64_t *leaves, int levels, const uint64_t *rgsw,
+                   size_t rgsw_stride, uint64_t *out) {
+    size_t rw = (size_t)2 * p->N;
+    size_t m = (size_t)1 << levels;
+    uint64_t *v = (uint64_t *)malloc(sizeof(uint64_t) * m * rw);
+    memcpy(v, leaves, sizeof(uint64_t) * m * rw);
+    for (int l = 0; l < levels; l++) {
+        for (size_t j = 0; j < m; j += 2) orc_cmux(p, v + j * rw, v + (j + 1) * rw, rgsw + l * rgsw_stride, v + j / 2 * rw);
+        m >>= 1;
+    }
+    memcpy(out, v, sizeof(uint64_t) * rw);
+    free(v);
 }
 
 /* RLWE.cpp:512-526 (plaintext X^mm with sign) and 561-577 (rows) */

@@ -166,6 +166,18 @@ void orc_rlwe_ksk_gen(const orc_params *p, const uint64_t *old_sk_ntt, const uin
 void orc_subs_key_gen(const orc_params *p, const uint64_t *rlwe_sk_time, uint64_t seed, uint64_t *subs);
 void orc_lwe_encrypt(const orc_params *p, const uint64_t *sk, uint64_t m, uint64_t seed, uint64_t index,
                      uint64_t *out);
+void orc_rlwe_encrypt(const orc_params *p, const uint64_t *sk_ntt, const uint64_t *m_time, uint64_t seed,
+                      uint64_t stream, uint64_t *out /* [2][N] NTT domain */);
+void orc_rlwe_decrypt(const orc_params *p, const uint64_t *sk_ntt, const uint64_t *c, uint64_t t,
+                      uint64_t *m_out /* [N] time domain */);
+void orc_rgsw_encrypt(const orc_params *p, const uint64_t *sk_ntt, const uint64_t *m_time, uint64_t seed,
+                      uint64_t stream_base, uint64_t *out /* [digitG][2][2][N] */);
+/* application::homo_expand (app.cpp:78-97) and the tree selection's CMUX tree
+ * (test.cpp:1576-1582) */
+void orc_homo_expand(const orc_params *p, const uint64_t *packed, const uint64_t *enc_sec, const uint64_t *subs,
+                     uint64_t *out /* [N][digitG][2][2][N] */);
+void orc_cmux_tree(const orc_params *p, const uint64_t *leaves, int levels, const uint64_t *rgsw,
+                   size_t rgsw_stride, uint64_t *out /* [2][N] */);
 
 /* stream tags shared with the device key generator */
 #define ORC_TAG_LWE_SK   1ULL
@@ -175,6 +187,7 @@ void orc_lwe_encrypt(const orc_params *p, const uint64_t *sk, uint64_t m, uint64
 #define ORC_TAG_SUBS     5ULL
 #define ORC_TAG_LWE_ENC  6ULL
 #define ORC_TAG_RLWE_ENC 7ULL
+#define ORC_TAG_RGSW_ENC 8ULL
 #define ORC_STREAM(tag, idx) (((uint64_t)(tag) << 56) | (uint64_t)(idx))
 
 #ifdef __cplusplus

@@ -317,6 +317,49 @@ void ref_blind_rotate_full(const uint64_t *in, int b, const int32_t *a, int step
     dump_rlwe(RLWE::blind_rotate_full(c, b, av, C), out);
 }
 
+// application::homo_expand (app.cpp:78-97); packed [digitG][2][N], enc_sec one
+// RGSW, subs [digitN][digitKS_R][2][N]; out [N][digitG][2][2][N]
+void ref_homo_expand(const uint64_t *packed, const uint64_t *enc_sec, const uint64_t *subs, uint64_t *out) {
+    auto pk = std::make_unique<std::vector<RLWE::RLWE_ciphertext>>(digitG);
+    for (uint32_t j = 0; j < digitG; j++) fill_rlwe((*pk)[j], packed + (size_t)j * 2 * N);
+    application::substitute_key sk;
+    sk.subs_key.resize(digitN);
+    for (int i = 0; i < digitN; i++) {
+        sk.subs_key[i].RLWE_kskey.resize(digitKS_R);
+        for (uint64_t l = 0; l < digitKS_R; l++)
+            fill_rlwe(sk.subs_key[i].RLWE_kskey[l], subs + ((size_t)i * digitKS_R + l) * 2 * N);
+    }
+    auto r = application::homo_expand(pk, make_rgsw(enc_sec), sk);
+    for (int i = 0; i < N; i++)
+        for (uint32_t h = 0; h < digitG; h++)
+            for (int g = 0; g < 2; g++)
+                dump_rlwe((*r)[i].c_text[h][g], out + (((size_t)i * digitG + h) * 2 + g) * 2 * N);
+}
+
+// The tree selection's CMUX loop (test.cpp:1576-1582) over the reference's
+// RLWE::CMUX: level l halves the 2^levels leaves with rgsw[l].
+void ref_cmux_tree(const uint64_t *leaves, int levels, const uint64_t *rgsw, uint64_t *out) {
+    uint32_t size = 1u << levels;
+    std::vector<RLWE::RLWE_ciphertext> v(size);
+    for (uint32_t i = 0; i < size; i++) fill_rlwe(v[i], leaves + (size_t)i * 2 * N);
+    for (int l = 0; l < levels; l++) {
+        RLWE::RGSW_ciphertext C = make_rgsw(rgsw + (size_t)l * digitG * 4 * N);
+        for (uint32_t j = 0; j < size; j += 2) v[j / 2] = RLWE::CMUX(v[j], v[j + 1], C);
+        size >>= 1;
+    }
+    dump_rlwe(v[0], out);
+}
+
+// RLWE::RLWE_decrypt(s, c, t) (RLWE.cpp:362-376); sk in the time domain
+// (the RLWE_secretkey(vector) constructor transforms it, RLWE.cpp:281-291)
+void ref_rlwe_decrypt(const uint64_t *sk_time, const uint64_t *c, uint64_t t_rlwe, uint64_t *out) {
+    RLWE::RLWE_secretkey sk(std::vector<ciphertext_t>(sk_time, sk_time + N));
+    RLWE::RLWE_ciphertext ct;
+    fill_rlwe(ct, c);
+    RLWE::RLWE_plaintext m = RLWE::RLWE_decrypt(sk, ct, t_rlwe);
+    std::memcpy(out, m.m.data(), sizeof(uint64_t) * N);
+}
+
 uint64_t ref_root_of_unity(int order, uint64_t mod) { return root_of_unity(order, mod); }
 uint64_t ref_first_prime(int nbit, int order) { return first_prime(nbit, order); }
 uint64_t ref_previous_prime(uint64_t p, int order) { return previous_prime(p, order); }

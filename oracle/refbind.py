@@ -46,6 +46,9 @@ def lib():
             "ref_cmux": (None, [U, U, U, U]),
             "ref_blind_rotate_rlwe": (C.c_int, [U, C.c_int, U, U]),
             "ref_blind_rotate_full": (None, [U, C.c_int, I32, C.c_int, U, U]),
+            "ref_homo_expand": (None, [U, U, U, U]),
+            "ref_cmux_tree": (None, [U, C.c_int, U, U]),
+            "ref_rlwe_decrypt": (None, [U, U, C.c_uint64, U]),
             "ref_root_of_unity": (C.c_uint64, [C.c_int, C.c_uint64]),
             "ref_first_prime": (C.c_uint64, [C.c_int, C.c_int]),
             "ref_previous_prime": (C.c_uint64, [C.c_uint64, C.c_int]),
@@ -189,4 +192,23 @@ class Reference:
         out = np.zeros((2, self.N), dtype=np.uint64)
         if lib().ref_blind_rotate_rlwe(_p(rlwe), 1 if forward else 0, _p(rgsws), _p(out)) != 0:
             raise ValueError("the reference's blind_rotate needs N = 2048 (num_rotate_poly = 11)")
+        return out
+
+    def homo_expand(self, packed, enc_sec, subs):
+        packed, enc_sec, subs = (np.ascontiguousarray(x, dtype=np.uint64) for x in (packed, enc_sec, subs))
+        out = np.zeros((self.N, self.digitG * 4 * self.N), dtype=np.uint64)
+        lib().ref_homo_expand(_p(packed), _p(enc_sec), _p(subs), _p(out))
+        return out
+
+    def cmux_tree(self, leaves, rgsws):
+        leaves, rgsws = (np.ascontiguousarray(x, dtype=np.uint64) for x in (leaves, rgsws))
+        levels = rgsws.size // (self.digitG * 4 * self.N)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        lib().ref_cmux_tree(_p(leaves), levels, _p(rgsws), _p(out))
+        return out
+
+    def rlwe_decrypt(self, sk_time, c, t):
+        sk_time, c = (np.ascontiguousarray(x, dtype=np.uint64) for x in (sk_time, c))
+        out = np.zeros(self.N, dtype=np.uint64)
+        lib().ref_rlwe_decrypt(_p(sk_time), _p(c), t, _p(out))
         return out

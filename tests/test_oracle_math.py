@@ -290,3 +290,76 @@ def test_cmux_selects():
         ph = _rlwe_decrypt_phase(o, out, s_ntt)
         got = [round((v if v < o.Q // 2 else v - o.Q) / delta) % 2 for v in ph]
         assert got == list(msgs[bit]), bit
+
+
+def _phase(o, c, s_ntt):
+    return np.array(_rlwe_decrypt_phase(o, c, s_ntt), dtype=object)
+
+
+def _noise(o, seed, stream_id):
+    # the sampler's error term of an encryption of zero on stream_id
+    from oracle import rng_word
+    out = []
+    for k in range(o.N):
+        g = int(o.gauss(rng_word(seed, stream_id, o.N + k)))
+        out.append(int(np.fmod(g, 18)))
+    return np.array(out, dtype=object)
+
+
+def test_rlwe_encrypt_identity():
+    # RLWE_encrypt (RLWE.cpp:324-344): b - a s is exactly the sampled error
+    # plus the message polynomial, coefficient by coefficient
+    from oracle import TAG_RLWE_ENC, stream
+    o = Oracle(n=4, q=64, N=32, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18)
+    z = o.rlwe_secret(91)
+    rng = np.random.default_rng(91)
+    m = rng.integers(0, o.Q, (2, o.N), dtype=np.uint64)
+    c = o.rlwe_encrypt(z, m, 92, first_index=5)
+    s_ntt = o.ntt(z)
+    for i in range(2):
+        e = _noise(o, 92, stream(TAG_RLWE_ENC, 5 + i))
+        assert list(_phase(o, c[i], s_ntt)) == [int(v) % o.Q for v in (e + m[i].astype(object))], i
+
+
+def test_rgsw_encrypt_identity():
+    # RGSW encryption (RLWE.cpp:561-577): row h column 0 carries m BG^h on a,
+    # so its phase is e - m BG^h s; column 1 carries it on b, phase e + m BG^h
+    from oracle import TAG_RGSW_ENC, stream
+    o = Oracle(n=4, q=64, N=32, Q=0, nbit=27, BGbits=7)
+    z = o.rlwe_secret(93)
+    s_ntt = o.ntt(z)
+    rng = np.random.default_rng(93)
+    m = rng.integers(0, o.Q, o.N, dtype=np.uint64)
+    rg = o.rgsw_encrypt(z, m, 94, first_index=2)[0].reshape(o.digitG, 2, 2, o.N)
+    Q = o.Q
+    ms = o.intt(((o.ntt(m).astype(object) * s_ntt.astype(object)) % Q).astype(np.uint64)).astype(object)
+    for h in range(o.digitG):
+        sc = pow(2, o.BGbits * h, Q)
+        for g in range(2):
+            e = _noise(o, 94, stream(TAG_RGSW_ENC, 2 * 2 * o.digitG) + 2 * h + g)
+            want = e - sc * ms if g == 0 else e + sc * m.astype(object)
+            assert list(_phase(o, rg[h, g], s_ntt)) == [int(v) % Q for v in want], (h, g)
+
+
+def test_homo_expand_then_cmux_tree_selects():
+    # homo_expand (app.cpp:78-97) turns packed bits into RGSW(b_i); a CMUX
+    # tree over those RGSWs (test.cpp:1576-1582) selects leaf sum b_l 2^l
+    N = 64
+    o = Oracle(n=4, q=128, N=N, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Bks_R_bits=9)
+    Q = o.Q
+    z = o.rlwe_secret(95)
+    subs = o.subs_key_gen(z, 96)
+    neg_s = np.array([(Q - int(v)) % Q for v in z], dtype=np.uint64)
+    enc_sec = o.rgsw_encrypt(z, neg_s, 97)[0]
+    rng = np.random.default_rng(95)
+    bits = rng.integers(0, 2, N)
+    packed = np.zeros((o.digitG, N), dtype=np.uint64)
+    for h in range(o.digitG):
+        packed[h] = [(int(b) * pow(2, o.BGbits * h, Q) * o.iN) % Q for b in bits]
+    rg = o.homo_expand(o.rlwe_encrypt(z, packed, 98), enc_sec, subs)
+    levels = 3
+    pats = rng.integers(0, 2, (1 << levels, N))
+    leaves = o.rlwe_encrypt(z, np.array([[int(v) * (Q // 2) for v in p] for p in pats], dtype=np.uint64), 99)
+    out = o.cmux_tree(leaves, rg[:levels])
+    want = sum(int(bits[l]) << l for l in range(levels))
+    assert list(o.rlwe_decrypt(z, out, 2)[0]) == list(pats[want])

@@ -152,3 +152,42 @@ def test_blind_rotate_full(N, Q, BGbits):
     rgsw = rng.integers(0, o.Q, (a.size, o.rgsw_words), dtype=np.uint64)
     for b in (0, 9, -4, N):
         assert np.array_equal(o.blind_rotate_full(x, b, a, rgsw), r.blind_rotate_full(x, b, a, rgsw)), b
+
+
+@pytest.mark.parametrize("N,Q,BGbits,Bks_R_bits", [(64, 0x3FFFFFFFFED001, 18, 9), (128, 0, 9, 7)])
+def test_homo_expand(N, Q, BGbits, Bks_R_bits):
+    # application::homo_expand (app.cpp:78-97) on random words: every word of
+    # all N output RGSWs (expansions, then products with RGSW(-s))
+    cfg = dict(n=4, q=2 * N, N=N, Q=Q, nbit=27, BGbits=BGbits, Br=16, Bks=32)
+    o = Oracle(**cfg, Bks_R_bits=Bks_R_bits)
+    r = refbind.Reference(cfg["n"], cfg["q"], N, o.Q, BGbits, cfg["Br"], cfg["Bks"], Bks_R_bits)
+    rng = np.random.default_rng(N + 1)
+    packed = rng.integers(0, o.Q, (o.digitG, 2, N), dtype=np.uint64)
+    enc_sec = rng.integers(0, o.Q, o.rgsw_words, dtype=np.uint64)
+    logN = N.bit_length() - 1
+    subs = rng.integers(0, o.Q, logN * o.digitKS_R * 2 * N, dtype=np.uint64)
+    assert np.array_equal(o.homo_expand(packed, enc_sec, subs), r.homo_expand(packed, enc_sec, subs))
+
+
+@pytest.mark.parametrize("levels", [1, 3])
+def test_cmux_tree(levels):
+    # the tree selection's CMUX loop (test.cpp:1576-1582) over the reference's
+    # RLWE::CMUX, on random words
+    cfg = dict(n=4, q=256, N=128, Q=0x3FFFFFFFFED001, nbit=54, BGbits=18, Br=16, Bks=32)
+    o, r = pair(cfg)
+    rng = np.random.default_rng(40 + levels)
+    leaves = rng.integers(0, o.Q, (1 << levels, 2, o.N), dtype=np.uint64)
+    rgsw = rng.integers(0, o.Q, (levels, o.rgsw_words), dtype=np.uint64)
+    assert np.array_equal(o.cmux_tree(leaves, rgsw), r.cmux_tree(leaves, rgsw))
+
+
+@pytest.mark.parametrize("cfg", SETS[:3])
+def test_rlwe_decrypt(cfg):
+    # RLWE_decrypt(s, c, t) (RLWE.cpp:362-376) with its denoise(t) rounding,
+    # including words that round across Q
+    o, r = pair(cfg)
+    rng = np.random.default_rng(cfg["N"] + 3)
+    sk = rng.integers(0, o.Q, o.N, dtype=np.uint64)
+    for t in (2, 4, 1024):
+        c = rng.integers(0, o.Q, (2, o.N), dtype=np.uint64)
+        assert np.array_equal(o.rlwe_decrypt(sk, c, t)[0], r.rlwe_decrypt(sk, c, t)), t
