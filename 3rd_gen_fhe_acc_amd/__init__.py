@@ -79,6 +79,8 @@ EXPORTS = {
     "fheacc_ctx_kernel_ms": [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     "fheacc_ctx_set_tuning": [C.c_void_p, C.c_int, C.c_int],
     "fheacc_ctx_get_tuning": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "fheacc_ctx_set_schedule": [C.c_void_p, C.c_int, C.c_int],
+    "fheacc_ctx_get_schedule": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "fheacc_ctx_set_eval_mode": [C.c_void_p, C.c_int],
     "fheacc_ctx_get_eval_mode": [C.c_void_p, C.POINTER(C.c_int)],
     "fheacc_autotune": [C.c_void_p, C.c_void_p, C.c_int64, C.c_char_p, C.c_size_t],
@@ -239,6 +241,16 @@ class Context:
         t, p = C.c_int(), C.c_int()
         _check(lib().fheacc_ctx_get_tuning(self.h, C.byref(t), C.byref(p)))
         return t.value, p.value
+
+    def set_schedule(self, persist=0, sync_window=0):
+        """Blind-rotation schedule: persist workgroups per CU (0: one per
+        ciphertext), lock-step window in steps (0: free running)."""
+        _check(lib().fheacc_ctx_set_schedule(self.h, int(persist), int(sync_window)))
+
+    def get_schedule(self):
+        p, w = C.c_int(), C.c_int()
+        _check(lib().fheacc_ctx_get_schedule(self.h, C.byref(p), C.byref(w)))
+        return p.value, w.value
 
     @property
     def eval_mode(self):

@@ -238,8 +238,10 @@ int fheacc_ctx_create(const fheacc_params *params, int device, fheacc_ctx **out)
         lds_max > 0)
         c->lds_budget = std::min<size_t>((size_t)lds_max, 160 * 1024);
     int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) {
         c->tune.small_max = cus;
+        c->cus = cus;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(fail("stream"));
     c->own_stream = true;
     const HostParams &p = c->p;
@@ -950,12 +952,23 @@ static int64_t ks_part_rows(const HostParams &p, int64_t B) {
     return rows;
 }
 
+// Progress words of a paced tiled blind rotation of B ciphertexts (none unless
+// the context's schedule paces): one per ciphertext.
+static size_t pacing_words(const fheacc_ctx *c, int64_t B) {
+    const Tuning &t = c->tune;
+    if (t.br_persist <= 0 || t.br_sync_window <= 0 || B <= 0) return 0;
+    return (size_t)B;
+}
+
 // Scratch of one gate-path launch: the key-switch digits u8[B][N][digitKS],
-// the key-switch b words u64[B], and the gather key switch's partial sums.
-size_t fheacc::gate_scratch_bytes(const HostParams &p, int64_t B) {
+// the key-switch b words u64[B], the gather key switch's partial sums, and
+// the lock-step counters of a paced launch.
+size_t fheacc::gate_scratch_bytes(const fheacc_ctx *c, int64_t B) {
+    const HostParams &p = c->p;
     const size_t ksd_bytes = (size_t)B * p.N * p.digitKS;
     const size_t ksb_end = (ksd_bytes + 255) / 256 * 256 + (size_t)B * 8;
-    return (ksb_end + 255) / 256 * 256 + (size_t)ks_part_rows(p, B) * (p.n + 1) * 8;
+    const size_t part_end = (ksb_end + 255) / 256 * 256 + (size_t)ks_part_rows(p, B) * (p.n + 1) * 8;
+    return (part_end + 255) / 256 * 256 + pacing_words(c, B) * 4;
 }
 
 template <typename W>
@@ -966,7 +979,9 @@ static int gate_path_impl(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch 
     const size_t ksd_al = (ksd_bytes + 255) / 256 * 256;
     uint8_t *ksd = (uint8_t *)g.scratch;
     uint64_t *ksb = (uint64_t *)((char *)g.scratch + ksd_al);
-    uint64_t *part = (uint64_t *)((char *)g.scratch + (ksd_al + (size_t)B * 8 + 255) / 256 * 256);
+    const size_t part_off = (ksd_al + (size_t)B * 8 + 255) / 256 * 256;
+    uint64_t *part = (uint64_t *)((char *)g.scratch + part_off);
+    const size_t pacing_off = (part_off + (size_t)ks_part_rows(p, B) * (p.n + 1) * 8 + 255) / 256 * 256;
     BRArgs<W> a{};
     a.A = make_arith<W>(p);
     a.tw = make_tw<W>(c);
@@ -999,10 +1014,20 @@ static int gate_path_impl(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch 
     a.mode = c->eval_mode.load();
     const size_t lds = br_lds_bytes(p, a.pb);
     if (lds > c->lds_budget) return fail("parameter set does not fit the LDS budget");
-    const int nt = B <= c->tune.small_max && c->tune.br_threads_small ? c->tune.br_threads_small : c->tune.br_threads;
+    const bool small = B <= c->tune.small_max && c->tune.br_threads_small;
+    const int nt = small ? c->tune.br_threads_small : c->tune.br_threads;
+    // Tiles serve dense throughput batches; gathered (serving) launches may
+    // share the GPU with another launch in flight, where a workgroup of a
+    // paced tile would wait (up to its spin limit) for one not yet resident.
+    const int persist = small ? 0 : c->tune.br_persist;
+    a.sync = nullptr;
+    a.sync_groups = kPacingGroups;
+    a.sync_window = c->tune.br_sync_window;
+    a.spin_limit = c->tune.br_spin_limit;
+    if (persist > 0 && !g.key_of && pacing_words(c, B) > 0) a.sync = (uint32_t *)((char *)g.scratch + pacing_off);
     {
         Prof pr(c, 0, g.stream, g.profile);
-        HIP_TRY(launch_blind_rotate<W>(a, p.logN, nt, B, lds, g.stream));
+        HIP_TRY(launch_blind_rotate<W>(a, p.logN, nt, B, lds, g.stream, g.key_of ? 0 : persist, c->cus, nullptr));
     }
     KSArgs<W> s{};
     s.A = a.A;
@@ -1044,7 +1069,7 @@ int fheacc::launch_gate_path(fheacc_ctx *c, const fheacc_keys *k, const GateLaun
 
 static int bootstrap_dense(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
                            const uint64_t *c2, uint64_t *out, int64_t B, const fheacc_capture *cap) {
-    if (int rc = ensure_scratch(c, gate_scratch_bytes(c->p, B))) return rc;
+    if (int rc = ensure_scratch(c, gate_scratch_bytes(c, B))) return rc;
     GateLaunch g;
     g.gates = gates;
     g.c1 = c1;
@@ -1678,6 +1703,23 @@ int fheacc_ctx_set_tuning(fheacc_ctx *c, int threads, int pb) {
     return 0;
 }
 
+int fheacc_ctx_set_schedule(fheacc_ctx *c, int persist, int sync_window) {
+    if (check_ctx(c)) return -1;
+    if (persist < 0 || persist > 64) return fail("persistent workgroups per CU must be in [0, 64]");
+    if (sync_window < 0 || (sync_window > 0 && persist == 0)) return fail("a sync window needs a persistent grid");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->tune.br_persist = persist;
+    c->tune.br_sync_window = sync_window;
+    return 0;
+}
+
+int fheacc_ctx_get_schedule(const fheacc_ctx *c, int *persist, int *sync_window) {
+    if (check_ctx(c) || !persist || !sync_window) return fail("null argument");
+    *persist = c->tune.br_persist;
+    *sync_window = c->tune.br_sync_window;
+    return 0;
+}
+
 int fheacc_ctx_get_tuning(const fheacc_ctx *c, int *threads, int *pb) {
     if (check_ctx(c) || !threads || !pb) return fail("null argument");
     *threads = c->tune.br_threads;
@@ -1781,6 +1823,31 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
             }
         }
     }
+    // the schedule at the chosen shape: one workgroup per ciphertext, tiles of
+    // resident workgroups, or the persistent grid paced within a window of
+    // steps (as many workgroups per CU as fit)
+    std::string rep_sch;
+    int best_persist = saved.br_persist, best_window = saved.br_sync_window;
+    {
+        double best_sch = 1e30;
+        const int cand[][2] = {{0, 0}, {64, 0}, {64, 8}, {64, 16}, {64, 32}};
+        for (const auto &cd : cand) {
+            c->tune.br_persist = cd[0];
+            c->tune.br_sync_window = cd[1];
+            double br = 0;
+            if (int rc = time_shape(best_t, best_pb, B, &br)) return restore(rc);
+            rep_sch += std::string(rep_sch.empty() ? "" : ", ") + "{\"persist\": " + std::to_string(cd[0]) +
+                       ", \"sync_window\": " + std::to_string(cd[1]) + ", \"blind_rotate_ms\": " +
+                       std::to_string(br) + "}";
+            if (br < best_sch) {
+                best_sch = br;
+                best_persist = cd[0];
+                best_window = cd[1];
+            }
+        }
+        c->tune.br_persist = saved.br_persist;
+        c->tune.br_sync_window = saved.br_sync_window;
+    }
     // the latency shape for batches up to one workgroup per CU, at the same
     // digit batching (every shape gives the same words)
     const int64_t Bs = std::min<int64_t>(c->tune.small_max, B);
@@ -1802,9 +1869,13 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
     c->tune.br_threads = best_t;
     c->tune.br_pb = best_pb;
     c->tune.br_threads_small = Bs > 0 ? best_ts : 0;
+    c->tune.br_persist = best_persist;
+    c->tune.br_sync_window = best_window;
     HIP_TRY(hipMemset(c->status, 0, sizeof(int)));
     rep += "], \"chosen\": {\"threads\": " + std::to_string(best_t) + ", \"polys_per_batch\": " +
            std::to_string(best_pb) + ", \"blind_rotate_ms\": " + std::to_string(best) + "}" +
+           ", \"schedule\": {\"candidates\": [" + rep_sch + "], \"persist\": " + std::to_string(best_persist) +
+           ", \"sync_window\": " + std::to_string(best_window) + "}" +
            ", \"small_batch\": {\"max_batch\": " + std::to_string(c->tune.small_max) + ", \"tuned_at\": " +
            std::to_string(Bs) + ", \"candidates\": [" + rep_s + "], \"threads\": " + std::to_string(best_ts) +
            "}}";
@@ -1856,7 +1927,7 @@ static int capture_sets(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *
     }
     for (int i = 0; i < nsets; i++) {
         void *p = nullptr;
-        HIP_TRY(hipMalloc(&p, gate_scratch_bytes(ctxs[i]->p, B[i])));
+        HIP_TRY(hipMalloc(&p, gate_scratch_bytes(ctxs[i], B[i])));
         gr->scratch.push_back(p);
     }
     for (int i = 0; i < nsets; i++) HIP_TRY(hipStreamSynchronize(ctxs[i]->stream));

@@ -71,6 +71,14 @@ struct Tuning {
     // which shortens each gate (0: br_threads).  Set by the autotuner.
     int br_threads_small = 0;
     int64_t small_max = 0;
+    // Blind-rotation schedule of dense batches (fheacc_ctx_set_schedule):
+    // br_persist > 0 splits the batch into tiles of that many workgroups per
+    // CU (at most the occupancy), one launch each; with br_sync_window > 0
+    // the workgroups of a group keep within that many steps of each other,
+    // polling a counter at most br_spin_limit times.
+    int br_persist = 0;
+    int br_sync_window = 0;
+    int br_spin_limit = 256;
     // fheacc_ctx_set_tuning was called: the CMUX chains take br_threads too.
     // Otherwise they run 1024 threads from N = 1024 up (one workgroup fills
     // a CU's LDS there, and 1024 threads give it 16 waves).
@@ -108,11 +116,14 @@ struct GateLaunch {
     int nseg = 0;
     const int64_t *seg_off = nullptr;  // host
     const void *const *seg_ksk = nullptr;  // host array of device pointers
-    void *scratch = nullptr;  // gate_scratch_bytes(p, B) bytes of device memory
+    void *scratch = nullptr;  // gate_scratch_bytes(ctx, B) bytes of device memory
     hipStream_t stream = nullptr;
     bool profile = false;
 };
-size_t gate_scratch_bytes(const HostParams &p, int64_t B);
+size_t gate_scratch_bytes(const fheacc_ctx *c, int64_t B);
+// lock-step groups of a paced tile: workgroup w is in
+// group w mod 8, one group per XCD when workgroups are dealt round robin
+constexpr int kPacingGroups = 8;
 // Launches only: no locking, no allocation, no synchronisation.
 int launch_gate_path(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch &g);
 
@@ -131,6 +142,7 @@ struct fheacc_ctx {
     fheacc::DeviceTables t;
     fheacc::Tuning tune;
     int device = 0;
+    int cus = 0;                    // compute units of the device
     size_t lds_budget = 64 * 1024;  // bytes of LDS one workgroup may use
     hipStream_t stream = nullptr;
     bool own_stream = false;

@@ -40,6 +40,13 @@ struct BRArgs {
     int *ct_status;
     int mode;  // FHEACC_EVAL_*: which residue-0 steps multiply by RGSW(1)
     int64_t c_base;  // ciphertext of workgroup 0 (a batch split over several launches)
+    int64_t c_end;   // one past the last ciphertext of this launch
+    // Tiled batches: progress words, one per ciphertext (null: no pacing),
+    // cleared by the launcher before each tile; groups of workgroups w mod
+    // sync_groups; a step waits for its group's step - sync_window, at most
+    // spin_limit polls.
+    uint32_t *sync;
+    int sync_window, sync_groups, spin_limit;
 };
 
 template <typename W>
@@ -183,8 +190,12 @@ struct KskGenArgs {
     const uint64_t *cij, *cijp;
 };
 
+// persist > 0: tiles of min(persist, occupancy) workgroups per CU over `cus`
+// CUs, one launch each; 0: one launch of a workgroup per ciphertext.  *grid
+// (may be null) receives the workgroups of the first launch.
 template <typename W>
-hipError_t launch_blind_rotate(const BRArgs<W> &a, int logN, int nt, int64_t B, size_t lds, hipStream_t st);
+hipError_t launch_blind_rotate(const BRArgs<W> &a, int logN, int nt, int64_t B, size_t lds, hipStream_t st,
+                               int persist, int cus, int64_t *grid);
 template <typename W>
 hipError_t launch_lwe_keyswitch(const KSArgs<W> &a, bool gather, hipStream_t st);
 template <typename W>

@@ -2,5 +2,6 @@
 #include "kernels_impl.hpp"
 
 namespace fheacc {
-template hipError_t launch_blind_rotate<uint32_t>(const BRArgs<uint32_t> &, int, int, int64_t, size_t, hipStream_t);
+template hipError_t launch_blind_rotate<uint32_t>(const BRArgs<uint32_t> &, int, int, int64_t, size_t, hipStream_t,
+                                                  int, int, int64_t *);
 }  // namespace fheacc

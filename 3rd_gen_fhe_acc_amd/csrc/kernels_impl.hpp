@@ -417,8 +417,58 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
 // LDS: acc (2 padded polys), dig (pb padded polys), the n evaluated a words.
 // ---------------------------------------------------------------------------
 
-template <typename W, int LOGN, int NT, bool INPLACE>
-__global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
+// Lock-step pacing of a tile (one launch of resident workgroups): prog[w]
+// holds how many steps workgroup w has finished.  Workgroups are grouped by
+// w mod groups (one group per XCD when workgroups are dealt round robin); a
+// workgroup starts step t once every member of its group has finished step
+// t - window, or after a bounded spin.  Thread i reads member i's word, so a
+// check is one parallel load and one barrier; the next step's words are read
+// while this step computes.  The words only pace the work (every
+// ciphertext's words depend on its own inputs alone), so a late or missing
+// workgroup costs time, never a wrong word or a hang.
+struct StepPacer {
+    uint32_t *prog = nullptr;  // the tile's progress words (null: off)
+    int first = 0;             // this workgroup's group: members first + groups i, i < members
+    int groups = 1, members = 0, window = 0, spin_limit = 0;
+    uint32_t next = 0;         // member threadIdx.x's word as read before the previous step
+    bool have_next = false;
+    __device__ __forceinline__ uint32_t read() const {
+        return (int)threadIdx.x < members
+                   ? __hip_atomic_load(prog + first + groups * (int)threadIdx.x, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT)
+                   : ~0u;
+    }
+    // checkpoints: every max(1, window / 2) steps (a check costs a device-wide
+    // read and a barrier); between them a workgroup runs free
+    __device__ __forceinline__ bool checkpoint(uint32_t t) const { return t % (uint32_t)max(1, window >> 1) == 0; }
+    // before step t (uniform across the workgroup)
+    __device__ __forceinline__ void wait(uint32_t t) {
+        if (!prog || !checkpoint(t)) return;
+        const int64_t need = (int64_t)t - window + 1;  // steps every member must have finished
+        if (need > 0) {
+            uint32_t v = have_next ? next : read();
+            int all = __syncthreads_and(v >= (uint32_t)need);
+            for (int spins = 0; !all && spins < spin_limit; spins++) {
+                __builtin_amdgcn_s_sleep(2);
+                v = read();
+                all = __syncthreads_and(v >= (uint32_t)need);
+            }
+        }
+        next = read();
+        have_next = true;
+    }
+    // after step t: every thread has finished it (the product ends on a barrier)
+    __device__ __forceinline__ void done(uint32_t t) {
+        if (prog && threadIdx.x == 0 && checkpoint(t))
+            __hip_atomic_store(prog + blockIdx.x, (uint32_t)(t + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+// One gate bootstrap's blind rotation and key-switch preparation on the
+// workgroup's LDS: ciphertext c, thread index tid (an opaque copy in the
+// persistent kernel), steps counted from step0 for pacing.
+template <typename W, int LOGN, int NT, bool INPLACE, bool PACE>
+__device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, StepPacer &pace, uint32_t step0) {
     constexpr int N = 1 << LOGN;
     extern __shared__ __align__(16) unsigned char smem[];
     // pb >= 2d: every digit poly in one batch, product computed in place in
@@ -429,27 +479,26 @@ __global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
     uint16_t *av = reinterpret_cast<uint16_t *>(
         inplace ? acc + padded(max(2, 2 * a.d) * N) : dig + padded(a.pb * N));
     __shared__ uint32_t s_b;
-    const int64_t c = a.c_base + blockIdx.x;
+    const int n = a.n;
     const uint32_t q = a.q, qm = a.q - 1;
     // gates outside enum GATES [0, 5] are an error of the call: flagged like
-    // identical operands, and the ciphertext skips its blind rotation
+    // identical operands, and the ciphertext skips its products
     int g = a.gates[c];
     const bool bad_gate = (unsigned)g > 5u;
     if (bad_gate) g = 0;
-    const int n = bad_gate ? 0 : a.n;
     const bool xr = (g == 4 || g == 5);
     const uint64_t *x1 = a.c1 + (a.idx1 ? a.idx1[c] : c) * (n + 1);
     const uint64_t *x2 = a.c2 + (a.idx2 ? a.idx2[c] : c) * (n + 1);
     int same = 1;
-    for (int k = threadIdx.x; k <= a.n; k += NT) {
+    for (int k = tid; k <= n; k += NT) {
         uint64_t u = x1[k], v = x2[k];
         same &= (u == v);
         uint32_t r = (uint32_t)(xr ? 2 * (u - v) : u + v) & qm;
-        if (k < a.n) av[k] = (uint16_t)r;
+        if (k < n) av[k] = (uint16_t)r;
         else s_b = r;
     }
     const int all_same = __syncthreads_and(same);
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         // bit 0: identical operands (RLWE.cpp:727-730); bit 1: gate out of range
         const int flags = (all_same ? FHEACC_STATUS_SAME : 0) | (bad_gate ? FHEACC_STATUS_GATE : 0);
         if (a.ct_status) a.ct_status[c] = flags;
@@ -460,61 +509,102 @@ __global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
     const uint32_t b = s_b;
     const uint32_t b1 = (uint32_t)(a.gate_constant[g] & qm), b2 = (b1 + q / 2) & qm;
     const int ef = (2 * N) / (int)q;
-    for (int idx = threadIdx.x; idx < padded(2 * N); idx += NT) acc[idx] = 0;
+    for (int idx = tid; idx < padded(2 * N); idx += NT) acc[idx] = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < (int)(q / 2); i += NT) {
+    for (int i = tid; i < (int)(q / 2); i += NT) {
         const uint32_t tmp = (b - (uint32_t)i) & qm;
         bool neg = (b1 < b2) ? (tmp >= b1 && tmp < b2) : !(tmp >= b2 && tmp < b1);
         acc[pad(N + i * ef)] = (W)(neg ? nQ8 : Q8);
     }
     __syncthreads();
-    ntt_forward<LOGN, NT>(acc + pad(N), 1, a.A, a.tw);
+    ntt_forward<LOGN, NT>(acc + pad(N), 1, a.A, a.tw, true, tid);
     // blind rotation: i outer, j inner.  Software order (RLWE.cpp:754-767)
     // skips residue 0; the accelerator's one-ciphertext form multiplies at
     // step (0, 0) whatever the residue, its batched forms at every step
-    // (myfpga.cpp:1030-1071, 1180-1225), with btkey[0] = RGSW(1).
+    // (myfpga.cpp:1030-1071, 1180-1225), with btkey[0] = RGSW(1).  A skipped
+    // step (and every step of a flagged ciphertext) still counts for pacing.
     const W *bsk = a.key_of ? a.bsk_tab[a.key_of[c]] : a.bsk;
     const int mode = a.mode;
-    for (int i = 0; i < n; i++) {
+    for (int i = 0, t = 0; i < n; i++) {
         uint32_t nega = (q - av[i]) & qm;
-        for (int j = 0; j < a.digitR; j++) {
+        for (int j = 0; j < a.digitR; j++, t++) {
             const uint32_t r = nega % a.Br;
             nega /= a.Br;
-            if (r || mode == FHEACC_EVAL_FPGA_XN || (mode == FHEACC_EVAL_FPGA_X1 && i == 0 && j == 0)) {
+            if constexpr (PACE) pace.wait(step0 + t);
+            if (!bad_gate && (r || mode == FHEACC_EVAL_FPGA_XN || (mode == FHEACC_EVAL_FPGA_X1 && i == 0 && j == 0))) {
                 const W *key = bsk + ((size_t)(i * a.digitR + j) * a.Br + r) * a.rgsw_elems;
                 if constexpr (inplace) ext_prod_inplace<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
                 else ext_prod<W, LOGN, NT>(acc, dig, key, a.A, a.tw, a.d, a.bits, a.pb);
             }
+            if constexpr (PACE) pace.done(step0 + t);
         }
     }
     if (a.cap_acc) {
         uint64_t *dst = a.cap_acc + c * 2 * N;
-        for (int idx = threadIdx.x; idx < 2 * N; idx += NT) dst[idx] = acc[pad(idx)];
+        for (int idx = tid; idx < 2 * N; idx += NT) dst[idx] = acc[pad(idx)];
     }
     __syncthreads();
     // LWE_keyswitch preparation: NTT-domain reversal of a, inverse NTT,
     // base-Bks digits of every a coefficient, b0 + Q/8 + 1
-    for (int idx = threadIdx.x; idx < N / 2; idx += NT) {
+    for (int idx = tid; idx < N / 2; idx += NT) {
         W t = acc[pad(idx)];
         acc[pad(idx)] = acc[pad(N - 1 - idx)];
         acc[pad(N - 1 - idx)] = t;
     }
     __syncthreads();
-    ntt_inverse<LOGN, NT>(acc, 2, a.A, a.tw);
+    ntt_inverse<LOGN, NT>(acc, 2, a.A, a.tw, tid);
     const int dks = a.dks;
     const uint32_t Bks = a.Bks;
     uint8_t *dst = a.ksd + c * (size_t)N * dks;
-    for (int i = threadIdx.x; i < N; i += NT) {
+    for (int i = tid; i < N; i += NT) {
         uint64_t x = acc[pad(i)];
         for (int j = 0; j < dks; j++) {
             dst[i * dks + j] = (uint8_t)(x % Bks);
             x /= Bks;
         }
     }
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         uint64_t v = (uint64_t)acc[pad(N)] + Q8;
         a.ksb[c] = v >= a.A.Q ? v - a.A.Q : v;
     }
+}
+
+// One ciphertext per workgroup; the hardware dispatcher refills CUs as gates
+// finish.
+template <typename W, int LOGN, int NT, bool INPLACE>
+__global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
+    StepPacer none;
+    br_one<W, LOGN, NT, INPLACE, false>(a, a.c_base + blockIdx.x, threadIdx.x, none, 0);
+}
+
+// Persistent and paced: a resident grid of G workgroups, workgroup w taking
+// ciphertexts c_base + w, + w + G, ...; its steps are counted across its
+// ciphertexts, so a group stays within a window of steps from gate to gate
+// without waiting at gate boundaries.  A workgroup out of work marks itself
+// finished so that no one waits for it.
+template <typename W, int LOGN, int NT, bool INPLACE>
+__global__ __launch_bounds__(NT) void k_blind_rotate_paced(BRArgs<W> a) {
+    StepPacer pace;
+    if (a.sync) {
+        pace.prog = a.sync;
+        pace.groups = a.sync_groups;
+        pace.first = (int)(blockIdx.x % (unsigned)a.sync_groups);
+        pace.members = ((int)gridDim.x - pace.first + a.sync_groups - 1) / a.sync_groups;
+        pace.window = a.sync_window;
+        pace.spin_limit = a.spin_limit;
+    }
+    const uint32_t steps = (uint32_t)a.n * a.digitR;
+    uint32_t step0 = 0;
+    for (int64_t c = a.c_base + blockIdx.x; c < a.c_end; c += gridDim.x, step0 += steps) {
+        // an opaque thread index per gate, so that per-thread addresses are
+        // recomputed for each gate instead of held across the whole loop
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        br_one<W, LOGN, NT, INPLACE, true>(a, c, tid, pace, step0);
+        __syncthreads();  // the next gate rewrites the LDS
+    }
+    if (a.sync && threadIdx.x == 0)
+        __hip_atomic_store(a.sync + blockIdx.x, ~0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -1389,15 +1479,63 @@ constexpr int64_t max_wgs_per_launch() {
     return ((int64_t)1 << 31) / NT;
 }
 
+// Clears the lock-step counters of a tile.
+template <typename W>
+__global__ void k_clear_pacing(uint32_t *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
 template <typename W, int LOGN, int NT>
-static hipError_t launch_br_nt(const BRArgs<W> &a, int64_t B, size_t lds, hipStream_t st) {
+static hipError_t launch_br_nt(const BRArgs<W> &a, int64_t B, size_t lds, hipStream_t st, int persist, int cus,
+                               int64_t *grid) {
+    // the paced persistent kernel exists for the throughput shapes (256 and
+    // 512 threads, N >= 512)
+    constexpr bool kPaceable = LOGN >= 9 && (NT == 256 || NT == 512);
     auto k = a.pb >= 2 * a.d ? k_blind_rotate<W, LOGN, NT, true> : k_blind_rotate<W, LOGN, NT, false>;
+    if constexpr (kPaceable) {
+        if (persist > 0 && cus > 0 && a.sync) {
+            auto kp = a.pb >= 2 * a.d ? k_blind_rotate_paced<W, LOGN, NT, true> : k_blind_rotate_paced<W, LOGN, NT, false>;
+            hipError_t e = hipFuncSetAttribute((const void *)kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            int occ = 0;
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)kp, NT, lds);
+            if (e != hipSuccess) return e;
+            const int64_t G = std::min<int64_t>(B, (int64_t)std::max(1, std::min(occ, persist)) * cus);
+            // a group's members are read one per thread
+            if (G <= (int64_t)a.sync_groups * NT) {
+                BRArgs<W> part = a;
+                part.c_end = a.c_base + B;
+                // progress words, one per workgroup, cleared by a kernel (not a
+                // memset) so that the launch can be captured
+                hipLaunchKernelGGL(k_clear_pacing<W>, dim3((unsigned)std::min<int64_t>((G + 255) / 256, 1024)),
+                                   dim3(256), 0, st, part.sync, G);
+                hipLaunchKernelGGL(kp, dim3((unsigned)G), dim3(NT), lds, st, part);
+                if (grid) *grid = G;
+                return hipGetLastError();
+            }
+        }
+    }
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    for (int64_t off = 0; off < B; off += max_wgs_per_launch<NT>()) {
+    // tiles: launches of one resident wave of workgroups each (G of them, at
+    // most the occupancy per CU), back to back; otherwise one launch (split
+    // only at the 2^31 work-item limit) that the dispatcher refills as gates end
+    int64_t G = max_wgs_per_launch<NT>();
+    if (persist > 0 && cus > 0) {
+        int occ = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k, NT, lds);
+        if (e != hipSuccess) return e;
+        G = (int64_t)std::max(1, std::min(occ, persist)) * cus;
+    }
+    if (grid) *grid = std::min(B, G);
+    for (int64_t off = 0; off < B; off += G) {
         BRArgs<W> part = a;
+        const int64_t cnt = std::min(B - off, G);
         part.c_base = a.c_base + off;
-        hipLaunchKernelGGL(k, dim3((unsigned)std::min(B - off, max_wgs_per_launch<NT>())), dim3(NT), lds, st, part);
+        part.c_end = part.c_base + cnt;
+        part.sync = nullptr;
+        hipLaunchKernelGGL(k, dim3((unsigned)cnt), dim3(NT), lds, st, part);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -1405,13 +1543,14 @@ static hipError_t launch_br_nt(const BRArgs<W> &a, int64_t B, size_t lds, hipStr
 }
 
 template <typename W>
-hipError_t launch_blind_rotate(const BRArgs<W> &a, int logN, int nt, int64_t B, size_t lds, hipStream_t st) {
+hipError_t launch_blind_rotate(const BRArgs<W> &a, int logN, int nt, int64_t B, size_t lds, hipStream_t st,
+                               int persist, int cus, int64_t *grid) {
     FHEACC_LOGN_SWITCH(logN, {
-        if (nt == 128 && LOGN >= 7) return launch_br_nt<W, LOGN, 128>(a, B, lds, st);
-        if (nt == 512 && LOGN >= 9) return launch_br_nt<W, LOGN, 512>(a, B, lds, st);
+        if (nt == 128 && LOGN >= 7) return launch_br_nt<W, LOGN, 128>(a, B, lds, st, persist, cus, grid);
+        if (nt == 512 && LOGN >= 9) return launch_br_nt<W, LOGN, 512>(a, B, lds, st, persist, cus, grid);
         if constexpr (LOGN >= 10)
-            if (nt == 1024) return launch_br_nt<W, LOGN, 1024>(a, B, lds, st);
-        return launch_br_nt<W, LOGN, 256>(a, B, lds, st);
+            if (nt == 1024) return launch_br_nt<W, LOGN, 1024>(a, B, lds, st, persist, cus, grid);
+        return launch_br_nt<W, LOGN, 256>(a, B, lds, st, persist, cus, grid);
     })
     return hipSuccess;
 }

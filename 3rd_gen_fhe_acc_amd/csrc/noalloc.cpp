@@ -50,7 +50,7 @@ int fheacc_bootstrap_batch_dev_noalloc(fheacc_ctx *c, fheacc_keys *k, const int3
     if (!k->bsk || !k->ksk) return fail("bootstrap and key-switch keys must be resident");
     if (B <= 0) return B == 0 ? 0 : fail("negative batch");
     if (B > 0x7fffffff) return fail("batch too large");
-    const size_t need = gate_scratch_bytes(c->p, B);
+    const size_t need = gate_scratch_bytes(c, B);
     if (need > (size_t)FHEACC_NOALLOC_SCRATCH_BYTES)
         return fail("a batch of " + std::to_string(B) + " needs " + std::to_string(need) +
                     " bytes of scratch; the allocation-free gate path holds " +

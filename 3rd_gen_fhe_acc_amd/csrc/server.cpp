@@ -527,7 +527,7 @@ int fheacc_server_create(fheacc_ctx *ctx, const fheacc_server_config *cfg, fheac
     if ((e = hipMemset(s->key_tab_dev, 0, kMaxClients * sizeof(void *))) != hipSuccess) return bail(e, "key table");
     if ((e = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking)) != hipSuccess)
         return bail(e, "stream");
-    const size_t scratch = gate_scratch_bytes(p, cfg->max_batch);
+    const size_t scratch = gate_scratch_bytes(ctx, cfg->max_batch);
     const size_t desc = (size_t)cfg->max_batch * 36;
     for (int i = 0; i < cfg->max_inflight; i++) {
         Slot &sl = s->slots[i];

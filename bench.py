@@ -34,6 +34,32 @@ METRIC = "bootstraps/sec (N=1024, batch 4096) vs 1-per-launch; p50/p99 gate late
 PARAMS = dict(n=500, q=1024, N=1024, Q=0, nbit=27, BGbits=7, Br=32, Bks=32, Bks_R_bits=9)
 SEED = 20260101
 HBM_PEAK_GBS = 8000.0
+# kernel sources whose counters the committed profiles hold: a profile is used
+# only while the digest it records matches these files
+PROFILED_SOURCES = ("kernels_impl.hpp", "ntt_lds.hpp", "modarith.hpp")
+
+
+def kernel_sources_digest():
+    import hashlib
+    h = hashlib.sha1()
+    for f in PROFILED_SOURCES:
+        with open(os.path.join(ROOT, "3rd_gen_fhe_acc_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def load_profile(name):
+    """A committed counter summary and whether it describes the current kernels."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, {"profile": "profiles/" + name, "fresh": False, "why": "absent"}
+    with open(path) as f:
+        prof = json.load(f)
+    cur = kernel_sources_digest()
+    fresh = prof.get("kernel_sources_sha1") == cur
+    return prof, {"profile": "profiles/" + name, "kernel_sources_sha1": prof.get("kernel_sources_sha1"),
+                  "current_sources_sha1": cur, "fresh": fresh,
+                  "why": "" if fresh else "kernel sources changed since the counters were collected"}
 
 
 def parse():
@@ -47,6 +73,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", default="auto",
                     help="'auto' (run the autotuner) or THREADS,POLYS_PER_BATCH for the blind-rotation kernel")
+    ap.add_argument("--schedule", default="auto",
+                    help="blind-rotation schedule 'persist,window' (fheacc_ctx_set_schedule), or 'auto' to keep "
+                         "the autotuner's choice (with --tuning auto) / one workgroup per ciphertext")
     ap.add_argument("--no-microbench", action="store_true", help="skip the NTT / external-product microbench")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="serving-loop arrival window (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=6.0, help="serving-loop Poisson arrival rate")
@@ -492,6 +521,9 @@ def main():
         th, pb = (int(x) for x in args.tuning.split(","))
         ctx.set_tuning(th, pb)
         tune_report = {"chosen": {"threads": th, "polys_per_batch": pb}, "source": "--tuning"}
+    if args.schedule != "auto":
+        ctx.set_schedule(*(int(x) for x in args.schedule.split(",")))
+    schedule = ctx.get_schedule()
     rng = np.random.default_rng(SEED)
     m1 = rng.integers(0, 2, B).astype(np.uint8)
     m2 = rng.integers(0, 2, B).astype(np.uint8)
@@ -573,20 +605,31 @@ def main():
     ext_products = int(np.count_nonzero(nega % 32) + np.count_nonzero((nega // 32) % 32))
     d = ctx.digitG
     modmul_per_ep = 2 * d * (N // 2) * 10 + 2 * (N // 2) * 10 + 2 * N + 4 * d * N
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_blind_rotate_n1024_b4096.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    # HBM traffic per launch from the committed FETCH/WRITE summary, for the
+    # schedule this run uses, only while it describes these kernel sources
+    sched_key = "%d,%d" % tuple(schedule)
+    tprof, traffic_src = load_profile("traffic_blind_rotate_n1024_b4096.json")
+    tent = (tprof or {}).get("schedules", {}).get(sched_key)
+    traffic_src["schedule"] = sched_key
+    if tent is None and traffic_src["fresh"]:
+        traffic_src.update(fresh=False, why="no counters for schedule " + sched_key)
+    traffic = tent["hbm_bytes_per_launch"] if tent and traffic_src["fresh"] else None
+    if tent:
+        traffic_src["kernel_ms_in_counter_run"] = tent.get("avg_ms_in_counter_run")
     # integer-VALU roofline: the kernel's VALU instructions per launch (from
-    # the committed rocprofv3 SQ counter summary of this workload) over the
-    # live launch time, against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+    # the committed rocprofv3 SQ counter summary of this workload, same
+    # freshness rule) over the live launch time, against 256 CUs x 4 SIMDs x
+    # 16 lanes x 2.4 GHz
     valu_peak = 256 * 4 * 16 * 2.4e9
+    sprof, valu_src = load_profile("sq_counters_blind_rotate_n1024_b4096.json")
     valu_instr = None
-    spath = os.path.join(ROOT, "profiles", "sq_counters_blind_rotate_n1024_b4096.json")
-    if os.path.exists(spath):
-        with open(spath) as f:
-            valu_instr = json.load(f)["counters_per_launch"].get("SQ_INSTS_VALU")
+    if sprof and valu_src["fresh"] and sprof.get("schedule", "0,0") == sched_key:
+        valu_instr = sprof["counters_per_launch"].get("SQ_INSTS_VALU")
+    elif sprof and valu_src["fresh"]:
+        valu_src.update(fresh=False, why="counters are for schedule " + sprof.get("schedule", "0,0"))
+    # the key switch: its key, digits, b words and outputs
+    ks_bytes = info["ksk_device_bytes"] + B * N * dks + 8 * B + B * (n + 1) * 8
+    ks_traffic = (tprof or {}).get("keyswitch", {}).get("hbm_bytes_per_launch") if traffic_src["fresh"] else None
     rec = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -610,6 +653,13 @@ def main():
                      "algorithmic_bytes_per_launch": br_bytes,
                      "copy_ceiling_GBps": round(copy_gbs, 1) if copy_gbs else None,
                      "frac_of_copy_ceiling": round(achieved / copy_gbs, 5) if copy_gbs else None},
+        "roofline_keyswitch": {"kernel": "k_lwe_keyswitch_staged", "bound": "hbm",
+                               "achieved": round(ks_bytes / (ks_avg * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(ks_bytes / (ks_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                               "traffic": ks_traffic, "avg_launch_ms": round(ks_avg, 4),
+                               "algorithmic_bytes_per_launch": ks_bytes},
+        "traffic_source": traffic_src,
+        "valu_source": valu_src,
         "valu": {"kernel": "k_blind_rotate", "bound": "integer VALU (the kernel's roofline; no MFMA applies)",
                  "achieved": round(valu_instr * 64 / (br_avg * 1e-3) / 1e12, 3) if valu_instr else None,
                  "peak": round(valu_peak / 1e12, 3), "unit": "T lane-ops/s",
@@ -632,6 +682,7 @@ def main():
         "eager_ms_per_step": round(eager_ms, 4),
         "keygen_s": round(keygen_s, 3),
         "tuning": tune_report,
+        "schedule": {"persist": schedule[0], "sync_window": schedule[1]},
     }
     if args.serve_seconds > 0:
         rec["serving"] = serving(fhe, ctx, args)

@@ -109,6 +109,29 @@ int fheacc_ctx_kernel_ms(fheacc_ctx *ctx, double *ms4, int64_t *launches4);
  * myfpga.cpp:1533). */
 int fheacc_ctx_set_tuning(fheacc_ctx *ctx, int br_threads, int br_polys_per_batch);
 int fheacc_ctx_get_tuning(const fheacc_ctx *ctx, int *br_threads, int *br_polys_per_batch);
+/* Schedule of the blind rotation for dense batches (the serving loop's
+ * gathered launches always run one workgroup per ciphertext):
+ *   persist = 0   one workgroup per ciphertext, the hardware dispatcher
+ *                 refilling CUs as gates finish (the default);
+ *   persist > 0, sync_window = 0  tiles: launches of min(persist,
+ *                 occupancy) workgroups per CU each, back to back, so the
+ *                 resident workgroups of a tile start together;
+ *   persist > 0, sync_window > 0  one persistent launch of that many
+ *                 workgroups per CU, each taking every G-th ciphertext;
+ *                 workgroups of a group (workgroup index mod 8) keep within
+ *                 sync_window steps of each other, counted across their
+ *                 gates, on device progress words, so each step's keys are
+ *                 fetched into L2 once for the group.  A wait gives up after
+ *                 a bounded spin: pacing never changes words and never blocks.
+ *                 At the benchmark's batch the tightest pacing cuts the
+ *                 blind rotation's L2 misses 3.7x, but costs more time than
+ *                 the misses do (the kernel is VALU-bound), so the
+ *                 autotuner keeps persist = 0 there.
+ * No counterpart in the reference: the accelerator streams one key per step
+ * for its 12-14 ciphertexts in flight (myfpga.cpp:1446-1492, 1533).  The
+ * autotuner picks it. */
+int fheacc_ctx_set_schedule(fheacc_ctx *ctx, int persist, int sync_window);
+int fheacc_ctx_get_schedule(const fheacc_ctx *ctx, int *persist, int *sync_window);
 /* Autotuner: times the batched gate bootstrap on B synthetic ciphertexts
  * with `keys` for every candidate shape, keeps the fastest in the context,
  * and writes a JSON report of all candidates to report (may be NULL). */
