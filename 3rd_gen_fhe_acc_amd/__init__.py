@@ -115,6 +115,8 @@ EXPORTS = {
                                        C.c_int64, C.POINTER(C.c_void_p)],
     "fheacc_graph_capture_mixed": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)],
+    "fheacc_graph_capture_interleaved": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_void_p)],
     "fheacc_graph_launch": [C.c_void_p],
     "fheacc_graph_destroy": [C.c_void_p],
     "fheacc_ntt_dev": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int],
@@ -494,6 +496,26 @@ class MixedGraph(BootstrapGraph):
         B = (C.c_int64 * k)(*[s[3].shape[0] for s in sets])
         h = C.c_void_p()
         _check(lib().fheacc_graph_capture_mixed(k, ctxs, keys, gates, c1, c2, out, B, C.byref(h)))
+        self.h = h
+
+
+class InterleavedGraph(BootstrapGraph):
+    """A mixed batch as it arrives: rows of several parameter sets interleaved
+    in one buffer (set_of[r] names row r's set), each set's kernels gathering
+    their rows on the device.  contexts/keys: one per set; gates [total],
+    c1/c2/out [total][row_words] device tensors."""
+
+    def __init__(self, contexts, keys, set_of, gates, c1, c2, out):
+        self.ctx, self.keys = contexts[0], list(keys)
+        set_of = np.ascontiguousarray(np.asarray(set_of, dtype=np.int32).reshape(-1))
+        self._bufs = (gates, c1, c2, out, set_of)
+        k = len(contexts)
+        ctxs = (C.c_void_p * k)(*[c.h.value for c in contexts])
+        ks = (C.c_void_p * k)(*[x.h.value for x in keys])
+        h = C.c_void_p()
+        _check(lib().fheacc_graph_capture_interleaved(k, ctxs, ks, set_of.ctypes.data_as(C.c_void_p), _dptr(gates),
+                                                      _dptr(c1), _dptr(c2), _dptr(out), set_of.size, c1.shape[1],
+                                                      C.byref(h)))
         self.h = h
 
 

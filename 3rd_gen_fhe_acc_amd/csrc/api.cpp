@@ -1008,6 +1008,8 @@ static int gate_path_impl(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch 
     a.status = c->status;
     a.idx1 = g.idx1;
     a.idx2 = g.idx2;
+    a.gidx = g.gidx;
+    a.row_words = g.row_words ? g.row_words : p.n + 1;
     a.key_of = g.key_of;
     a.bsk_tab = (const W *const *)g.bsk_tab;
     a.ct_status = g.ct_status;
@@ -1051,6 +1053,7 @@ static int gate_path_impl(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch 
         s.ksd = ksd + (size_t)o0 * s.S;
         s.ksb = ksb + o0;
         s.oidx = g.oidx ? g.oidx + o0 : nullptr;
+        s.out_row_words = g.row_words ? g.row_words : p.n + 1;
         s.out = g.oidx ? g.out : g.out + o0 * (p.n + 1);
         s.cap_ks = g.cap && g.cap->ks ? g.cap->ks + o0 * (p.n + 1) : nullptr;
         s.B = o1 - o0;
@@ -1901,34 +1904,43 @@ struct fheacc_graph {
 };
 
 // Captures the gate path of every set into one graph, set after set on one
-// capture stream, each set with a scratch buffer of its own.
-static int capture_sets(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *keys, const int32_t *const *gates,
-                        const uint64_t *const *c1, const uint64_t *const *c2, uint64_t *const *out,
-                        const int64_t *B, fheacc_graph **g) {
-    if (!g || nsets < 1 || nsets > 16 || !ctxs || !keys || !gates || !c1 || !c2 || !out || !B)
-        return fail("invalid argument");
+// capture stream, each set with a scratch buffer of its own.  proto[i] holds
+// set i's buffers (and gathered rows, for an interleaved batch); `owned`
+// device buffers pass to the graph.
+static int capture_sets(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *keys,
+                        const std::vector<GateLaunch> &proto, std::vector<void *> owned, fheacc_graph **g) {
+    auto drop_owned = [&](int rc) {
+        for (void *p : owned) (void)hipFree(p);
+        return rc;
+    };
+    if (!g || nsets < 1 || nsets > 16 || !ctxs || !keys) return drop_owned(fail("invalid argument"));
     *g = nullptr;
     for (int i = 0; i < nsets; i++) {
-        if (check_ctx(ctxs[i]) || !keys[i] || !gates[i] || !c1[i] || !c2[i] || !out[i] || B[i] <= 0 ||
-            B[i] > 0x7fffffff)
-            return fail("invalid argument for set " + std::to_string(i));
-        if (keys[i]->ctx != ctxs[i]) return fail("keys of set " + std::to_string(i) + " belong to another context");
-        if (!keys[i]->bsk || !keys[i]->ksk) return fail("bootstrap and key-switch keys must be resident");
-        if (ctxs[i]->device != ctxs[0]->device) return fail("all sets must be on one device");
+        const GateLaunch &pl = proto[i];
+        if (check_ctx(ctxs[i]) || !keys[i] || !pl.gates || !pl.c1 || !pl.c2 || !pl.out || pl.B < 0 ||
+            pl.B > 0x7fffffff)
+            return drop_owned(fail("invalid argument for set " + std::to_string(i)));
+        if (keys[i]->ctx != ctxs[i])
+            return drop_owned(fail("keys of set " + std::to_string(i) + " belong to another context"));
+        if (!keys[i]->bsk || !keys[i]->ksk)
+            return drop_owned(fail("bootstrap and key-switch keys must be resident"));
+        if (ctxs[i]->device != ctxs[0]->device) return drop_owned(fail("all sets must be on one device"));
     }
-    HIP_TRY(hipSetDevice(ctxs[0]->device));
+    if (hipSetDevice(ctxs[0]->device) != hipSuccess) return drop_owned(fail("hipSetDevice failed"));
     std::unique_ptr<fheacc_graph, int (*)(fheacc_graph *)> gr(new fheacc_graph(), fheacc_graph_destroy);
     gr->ctx = ctxs[0];
+    gr->scratch = std::move(owned);
     for (int i = 0; i < nsets; i++) {
         ctx_retain(ctxs[i]);
         gr->ctxs.push_back(ctxs[i]);
         keys_retain(keys[i]);
         gr->keys.push_back(keys[i]);
     }
+    std::vector<void *> set_scratch(nsets, nullptr);
     for (int i = 0; i < nsets; i++) {
-        void *p = nullptr;
-        HIP_TRY(hipMalloc(&p, gate_scratch_bytes(ctxs[i], B[i])));
-        gr->scratch.push_back(p);
+        if (proto[i].B == 0) continue;
+        HIP_TRY(hipMalloc(&set_scratch[i], gate_scratch_bytes(ctxs[i], proto[i].B)));
+        gr->scratch.push_back(set_scratch[i]);
     }
     for (int i = 0; i < nsets; i++) HIP_TRY(hipStreamSynchronize(ctxs[i]->stream));
     // Capture on a stream of its own (the null stream cannot be captured).
@@ -1943,14 +1955,11 @@ static int capture_sets(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *
     if (!rc) {
         if (hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = fail("begin capture");
         for (int i = 0; i < nsets && !rc; i++) {
-            GateLaunch gl;
-            gl.gates = gates[i];
-            gl.c1 = c1[i];
-            gl.c2 = c2[i];
-            gl.out = out[i];
-            gl.B = B[i];
-            gl.scratch = gr->scratch[i];
+            if (proto[i].B == 0) continue;
+            GateLaunch gl = proto[i];
+            gl.scratch = set_scratch[i];
             gl.stream = root;
+            gl.profile = false;
             rc = launch_gate_path(ctxs[i], keys[i], gl);
         }
         hipError_t e = hipStreamEndCapture(root, &graph);
@@ -1966,13 +1975,66 @@ static int capture_sets(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *
 
 int fheacc_graph_capture_bootstrap(fheacc_ctx *c, fheacc_keys *k, const int32_t *gates, const uint64_t *c1,
                                    const uint64_t *c2, uint64_t *out, int64_t B, fheacc_graph **g) {
-    return capture_sets(1, &c, &k, &gates, &c1, &c2, &out, &B, g);
+    return fheacc_graph_capture_mixed(1, &c, &k, &gates, &c1, &c2, &out, &B, g);
 }
 
 int fheacc_graph_capture_mixed(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *keys,
                                const int32_t *const *gates, const uint64_t *const *c1, const uint64_t *const *c2,
                                uint64_t *const *out, const int64_t *B, fheacc_graph **g) {
-    return capture_sets(nsets, ctxs, keys, gates, c1, c2, out, B, g);
+    if (!g || nsets < 1 || nsets > 16 || !gates || !c1 || !c2 || !out || !B) return fail("invalid argument");
+    std::vector<GateLaunch> proto(nsets);
+    for (int i = 0; i < nsets; i++) {
+        if (B[i] <= 0) return fail("invalid argument for set " + std::to_string(i));
+        proto[i].gates = gates[i];
+        proto[i].c1 = c1[i];
+        proto[i].c2 = c2[i];
+        proto[i].out = out[i];
+        proto[i].B = B[i];
+    }
+    return capture_sets(nsets, ctxs, keys, proto, {}, g);
+}
+
+int fheacc_graph_capture_interleaved(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *keys,
+                                     const int32_t *set_of, const int32_t *gates, const uint64_t *c1,
+                                     const uint64_t *c2, uint64_t *out, int64_t total, int64_t row_words,
+                                     fheacc_graph **g) {
+    if (!g || nsets < 1 || nsets > 16 || !ctxs || !keys || !set_of || !gates || !c1 || !c2 || !out)
+        return fail("invalid argument");
+    if (total <= 0 || total > 0x7fffffff) return fail("total out of range");
+    std::vector<std::vector<int64_t>> rows(nsets);
+    for (int64_t r = 0; r < total; r++) {
+        if (set_of[r] < 0 || set_of[r] >= nsets) return fail("row " + std::to_string(r) + ": set out of range");
+        rows[set_of[r]].push_back(r);
+    }
+    for (int i = 0; i < nsets; i++) {
+        if (check_ctx(ctxs[i])) return -1;
+        if (row_words < ctxs[i]->p.n + 1)
+            return fail("row_words must hold the largest LWE ciphertext (n + 1 words of set " + std::to_string(i) +
+                        ")");
+    }
+    // the row indices of every set, back to back, in a device buffer the graph owns
+    HIP_TRY(hipSetDevice(ctxs[0]->device));
+    DevBuf idx;
+    HIP_TRY(hipMalloc(&idx.p, (size_t)total * 8));
+    std::vector<GateLaunch> proto(nsets);
+    int64_t off = 0;
+    for (int i = 0; i < nsets; i++) {
+        const int64_t *d = (const int64_t *)idx.p + off;
+        if (!rows[i].empty())
+            HIP_TRY(hipMemcpy((void *)d, rows[i].data(), rows[i].size() * 8, hipMemcpyHostToDevice));
+        GateLaunch &pl = proto[i];
+        pl.gates = gates;
+        pl.c1 = c1;
+        pl.c2 = c2;
+        pl.out = out;
+        pl.B = (int64_t)rows[i].size();
+        pl.idx1 = pl.idx2 = pl.oidx = pl.gidx = d;
+        pl.row_words = row_words;
+        off += pl.B;
+    }
+    void *owned = idx.p;
+    idx.p = nullptr;
+    return capture_sets(nsets, ctxs, keys, proto, {owned}, g);
 }
 
 int fheacc_graph_launch(fheacc_graph *g) {

@@ -109,6 +109,8 @@ struct GateLaunch {
     const fheacc_capture *cap = nullptr;
     // gathered form: operand / output rows and per-ciphertext key sets
     const int64_t *idx1 = nullptr, *idx2 = nullptr, *oidx = nullptr;
+    const int64_t *gidx = nullptr;  // gate of ciphertext c at gates[gidx[c]] (null: gates[c])
+    int64_t row_words = 0;          // input and output row stride in words (0: n + 1)
     const int32_t *key_of = nullptr;
     const void *const *bsk_tab = nullptr;  // device array of device key pointers
     int *ct_status = nullptr;              // device [B]: 1 where the operands are identical

@@ -35,6 +35,8 @@ struct BRArgs {
     // bsk_tab[key_of[c]], and reports identical operands in ct_status[c].
     // All null for a dense batch.
     const int64_t *idx1, *idx2;
+    const int64_t *gidx;   // gate of ciphertext c at gates[gidx[c]] (null: gates[c])
+    int64_t row_words;     // words from one input row to the next (n + 1 when dense)
     const int32_t *key_of;
     const W *const *bsk_tab;
     int *ct_status;
@@ -61,6 +63,7 @@ struct KSArgs {
     int lq1;
     uint64_t *out, *cap_ks;
     const int64_t *oidx;  // output row of ciphertext c (null: row c)
+    int64_t out_row_words;  // words from one output row to the next (n + 1 when dense)
     int64_t B;
     int CB, KB, SB;
     // gather form: the S steps of a ciphertext split into `split` ranges, one

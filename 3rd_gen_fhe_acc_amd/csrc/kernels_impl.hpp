@@ -483,12 +483,12 @@ __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, S
     const uint32_t q = a.q, qm = a.q - 1;
     // gates outside enum GATES [0, 5] are an error of the call: flagged like
     // identical operands, and the ciphertext skips its products
-    int g = a.gates[c];
+    int g = a.gates[a.gidx ? a.gidx[c] : c];
     const bool bad_gate = (unsigned)g > 5u;
     if (bad_gate) g = 0;
     const bool xr = (g == 4 || g == 5);
-    const uint64_t *x1 = a.c1 + (a.idx1 ? a.idx1[c] : c) * (n + 1);
-    const uint64_t *x2 = a.c2 + (a.idx2 ? a.idx2[c] : c) * (n + 1);
+    const uint64_t *x1 = a.c1 + (a.idx1 ? a.idx1[c] : c) * a.row_words;
+    const uint64_t *x2 = a.c2 + (a.idx2 ? a.idx2[c] : c) * a.row_words;
     int same = 1;
     for (int k = tid; k <= n; k += NT) {
         uint64_t u = x1[k], v = x2[k];
@@ -799,7 +799,7 @@ __global__ __launch_bounds__(NT) void k_lwe_keyswitch_staged(KSArgs<W> a) {
 #pragma unroll
     for (int j = 0; j < KB; j++) {
         const int col = col0 + j;
-        if (col <= a.n) a.out[orow * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s[j], a.A));
+        if (col <= a.n) a.out[orow * a.out_row_words + col] = ks_finish(a, c, col, ks_reduce<W>(s[j], a.A));
     }
 }
 
@@ -820,7 +820,7 @@ __global__ __launch_bounds__(NT) void k_lwe_keyswitch_gather(KSArgs<W> a) {
     if (col <= a.n) {
         for (int st = st0; st < st1; st++) ks_add<W>(s, a.ksk[((size_t)st * a.Bks + dg[st]) * a.n_pad + col], a.Q);
         if (a.split == 1)
-            a.out[(a.oidx ? a.oidx[c] : c) * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s, a.A));
+            a.out[(a.oidx ? a.oidx[c] : c) * a.out_row_words + col] = ks_finish(a, c, col, ks_reduce<W>(s, a.A));
         else
             a.part[(c * a.split + part) * (a.n + 1) + col] = s;
     }
@@ -843,7 +843,7 @@ __global__ __launch_bounds__(NT) void k_lwe_keyswitch_finish(KSArgs<W> a) {
             s = t >= a.Q ? t - a.Q : t;
         }
     }
-    a.out[(a.oidx ? a.oidx[c] : c) * (a.n + 1) + col] = ks_finish(a, c, col, ks_reduce<W>(s, a.A));
+    a.out[(a.oidx ? a.oidx[c] : c) * a.out_row_words + col] = ks_finish(a, c, col, ks_reduce<W>(s, a.A));
 }
 
 // ---------------------------------------------------------------------------

@@ -79,7 +79,11 @@ def parse():
     ap.add_argument("--no-microbench", action="store_true", help="skip the NTT / external-product microbench")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="serving-loop arrival window (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=6.0, help="serving-loop Poisson arrival rate")
-    ap.add_argument("--serve-budget-gb", type=float, default=8.0, help="serving-loop HBM budget for ciphertexts")
+    ap.add_argument("--serve-curve", default="3,6,10",
+                    help="offered loads (req/s) of the latency curve after the serving run ('' to skip)")
+    ap.add_argument("--serve-curve-requests", type=int, default=30, help="requests per curve point")
+    ap.add_argument("--serve-wave-cap", type=int, default=1024,
+                    help="the curve's second launch cap: one resident wave of gates (4 per CU x 256 CUs)")
     ap.add_argument("--serve-max-batch", type=int, default=4096)
     ap.add_argument("--serve-inflight", type=int, default=1,
                     help="serving launches in flight (one gives lower gate latency at this load)")
@@ -138,15 +142,8 @@ def microbench(fhe, torch, B, reps=5):
     }
 
 
-def serving(fhe, ctx, args):
-    """Serving loop: Poisson arrivals at a fixed rate of k-bit set-equality
-    jobs (k from {8, 16, 32}: XNOR of each item bit with the query bit, then
-    an AND tree per item), each for one of 8 clients with their own
-    device-resident keys (benchmark parameters).  Item counts are log-uniform
-    in [1, 1024], values uniform 32-bit from a seed.  Admission by HBM budget,
-    continuous batching across jobs and clients (fheacc_server_*)."""
-    circuits = importlib.import_module("3rd_gen_fhe_acc_amd.circuits")
-    rng = np.random.default_rng(SEED + 4)
+def serving_clients(ctx):
+    """8 clients with their own device-resident keys (benchmark parameters)."""
     clients = []
     t0 = time.perf_counter()
     for c in range(8):
@@ -156,12 +153,31 @@ def serving(fhe, ctx, args):
         keys.generate(SEED + 200 + c, s, z)
         clients.append((s, keys))
     ctx.sync()
-    keygen_s = time.perf_counter() - t0
+    return clients, time.perf_counter() - t0
+
+
+def serving(fhe, ctx, clients, qps, seconds, max_batch, inflight, min_fill=0, fair=True, seed=SEED + 4,
+            alone_check=True, n_requests=None):
+    """Serving loop: Poisson arrivals at `qps` for `seconds` of k-bit
+    set-equality jobs (k from {8, 16, 32}: XNOR of each item bit with the
+    query bit, then an AND tree per item), each for one of the 8 clients.
+    Item counts are log-uniform in [1, 1024], values uniform 32-bit from a
+    seed.  Admission by HBM budget, continuous batching across jobs and
+    clients (fheacc_server_*)."""
+    circuits = importlib.import_module("3rd_gen_fhe_acc_amd.circuits")
+    rng = np.random.default_rng(seed)
     arrivals = []
-    t = rng.exponential(1.0 / args.serve_qps)
-    while t < args.serve_seconds:
-        arrivals.append(t)
-        t += rng.exponential(1.0 / args.serve_qps)
+    if n_requests:
+        # a fixed request list: unit-rate gaps scaled by 1/qps, so that every
+        # load replays the same requests and the offered load scales with qps
+        gaps = np.random.default_rng(seed + 1).exponential(1.0, n_requests)
+        arrivals = list(np.cumsum(gaps) / qps)
+        seconds = float(arrivals[-1])
+    else:
+        t = rng.exponential(1.0 / qps)
+        while t < seconds:
+            arrivals.append(t)
+            t += rng.exponential(1.0 / qps)
     reqs = []
     cache = {}
     t0 = time.perf_counter()
@@ -180,8 +196,8 @@ def serving(fhe, ctx, args):
         want = np.array([(int(v) & mask) == (query & mask) for v in items], dtype=np.uint8)
         reqs.append(dict(t=ta, client=c, k=k, m=m, inputs=inputs, ops=ops, outs=outs, want=want))
     encrypt_s = time.perf_counter() - t0
-    srv = fhe.Server(ctx, hbm_budget=int(args.serve_budget_gb * 2 ** 30), max_batch=args.serve_max_batch,
-                     max_inflight=args.serve_inflight, min_fill=args.serve_min_fill, fair_share=not args.serve_fifo)
+    srv = fhe.Server(ctx, hbm_budget=8 << 30, max_batch=max_batch, max_inflight=inflight, min_fill=min_fill,
+                     fair_share=fair)
     ids = [srv.add_client(k) for _, k in clients]
     # warm the path once, then measure from a clean slate
     r0 = min(reqs, key=lambda r: len(r["ops"]))
@@ -199,21 +215,20 @@ def serving(fhe, ctx, args):
     st = srv.stats()
     ok = all(np.array_equal(ctx.lwe_decrypt(clients[r["client"]][0], o), r["want"]) for r, o in zip(reqs, outs))
     # the same requests alone, on an idle server: bit-identical words
-    sample = sorted(range(len(reqs)), key=lambda i: len(reqs[i]["ops"]))[:3]
-    alone_eq = True
-    for i in sample:
-        r = reqs[i]
-        o, _ = srv.wait(srv.submit(ids[r["client"]], r["inputs"], r["ops"], r["outs"]))
-        alone_eq &= bool(np.array_equal(o, outs[i]))
-    alone_eq &= bool(np.array_equal(warm, outs[reqs.index(r0)]))
+    alone_eq = bool(np.array_equal(warm, outs[reqs.index(r0)]))
+    if alone_check:
+        for i in sorted(range(len(reqs)), key=lambda i: len(reqs[i]["ops"]))[:3]:
+            r = reqs[i]
+            o, _ = srv.wait(srv.submit(ids[r["client"]], r["inputs"], r["ops"], r["outs"]))
+            alone_eq &= bool(np.array_equal(o, outs[i]))
     srv.close()
     gates = sum(len(r["ops"]) for r in reqs)
     return {
         "workload": "Poisson %.1f req/s for %.0f s: k-bit set equality (k in {8,16,32}, XNOR + AND tree), "
                     "items log-uniform in [1,1024], 8 clients with resident keys, benchmark parameters"
-                    % (args.serve_qps, args.serve_seconds),
+                    % (qps, seconds),
         "requests": len(reqs), "bootstraps": int(st["bootstraps"]),
-        "offered_bootstraps_per_s": round(gates / args.serve_seconds, 1),
+        "offered_bootstraps_per_s": round(gates / seconds, 1),
         "bootstraps_per_s": round(st["bootstraps"] / elapsed, 1),
         "gate_latency_ms_p50": round(st["gate_ms_p50"], 3), "gate_latency_ms_p99": round(st["gate_ms_p99"], 3),
         "request_latency_ms_p50": round(st["job_ms_p50"], 3),
@@ -222,11 +237,26 @@ def serving(fhe, ctx, args):
         "launches": int(st["batches"]), "mean_launch_size": round(st["bootstraps"] / max(st["batches"], 1), 1),
         "max_launch_size": int(st["max_batch_seen"]),
         "hbm_budget_bytes": int(st["pool_bytes"]), "peak_admitted_bytes": int(st["peak_bytes"]),
-        "config": {"max_batch": args.serve_max_batch, "max_inflight": args.serve_inflight,
-                   "min_fill": args.serve_min_fill, "fair_share": not args.serve_fifo},
-        "decrypt_ok": bool(ok), "alone_equals_mix": bool(alone_eq),
-        "client_keygen_s": round(keygen_s, 3), "encrypt_s": round(encrypt_s, 3),
+        "config": {"max_batch": max_batch, "max_inflight": inflight, "min_fill": min_fill, "fair_share": fair},
+        "decrypt_ok": bool(ok), "alone_equals_mix": bool(alone_eq), "encrypt_s": round(encrypt_s, 3),
     }
+
+
+def serving_curve(fhe, ctx, clients, args):
+    """Latency against offered load: the same --serve-curve-requests requests
+    arriving at each rate of --serve-curve (Poisson gaps scaled by 1/rate),
+    under the default launch cap and under a cap of one resident wave of
+    gates."""
+    runs = []
+    for qps in (float(x) for x in args.serve_curve.split(",")):
+        for cap in (args.serve_max_batch, args.serve_wave_cap):
+            r = serving(fhe, ctx, clients, qps, 0.0, cap, args.serve_inflight, seed=SEED + 40, alone_check=False,
+                        n_requests=args.serve_curve_requests)
+            runs.append({k: r[k] for k in ("requests", "bootstraps", "offered_bootstraps_per_s", "bootstraps_per_s",
+                                           "gate_latency_ms_p50", "gate_latency_ms_p99", "request_latency_ms_p50",
+                                           "request_latency_ms_p99", "gpu_busy_s", "elapsed_s",
+                                           "mean_launch_size", "decrypt_ok")} | {"qps": qps, "max_batch": cap})
+    return {"requests_per_point": args.serve_curve_requests, "runs": runs}
 
 
 MIXED = {
@@ -240,19 +270,24 @@ MIXED_SHARE = {"A": 0.6, "B": 0.3, "C": 0.1}
 def mixed(fhe, torch, main_ctx, main_keys, total=8192, reps=3):
     """Uneven mixed batch in one launch: 60 % set A (the benchmark set), 30 %
     set B (N=2048, 54-bit Q, d=3, n=630), 10 % set C (N=512, 27-bit Q, d=5),
-    8192 seeded random-bit ciphertexts uniformly interleaved, captured as one
-    HIP graph (the sets' kernels one after another), each set at its autotuned
-    launch shape."""
+    8192 seeded random-bit ciphertexts uniformly interleaved in one buffer
+    (row stride 631 words), captured as one HIP graph whose per-set kernels
+    gather their rows on the device, each set at its autotuned launch shape."""
     rng = np.random.default_rng(SEED + 5)
     names = list(MIXED)
     counts = [int(round(total * MIXED_SHARE[k])) for k in names]
     counts[0] += total - sum(counts)
-    lab = np.concatenate([np.full(c, i) for i, c in enumerate(counts)])
+    lab = np.concatenate([np.full(c, i) for i, c in enumerate(counts)]).astype(np.int32)
     rng.shuffle(lab)
+    rw = max(v["n"] for v in MIXED.values()) + 1
+    c1_all = np.zeros((total, rw), dtype=np.uint64)
+    c2_all = np.zeros_like(c1_all)
+    g_all = np.zeros(total, dtype=np.int32)
     sets = []
     t0 = time.perf_counter()
     for i, nm in enumerate(names):
-        B = int(np.count_nonzero(lab == i))
+        rows = np.flatnonzero(lab == i)
+        B = rows.size
         if nm == "A":
             ctx, keys = main_ctx, main_keys
             s = ctx.lwe_secret(SEED, binary=True)
@@ -265,14 +300,19 @@ def mixed(fhe, torch, main_ctx, main_keys, total=8192, reps=3):
         m1, m2 = rng.integers(0, 2, B), rng.integers(0, 2, B)
         gates = rng.integers(0, 6, B).astype(np.int32)
         c1, c2 = ctx.lwe_encrypt(s, m1, SEED + 30 + i, 0), ctx.lwe_encrypt(s, m2, SEED + 30 + i, B)
-        d1 = torch.from_numpy(c1.view(np.int64)).cuda()
-        d2 = torch.from_numpy(c2.view(np.int64)).cuda()
+        n1 = ctx.n + 1
+        c1_all[rows, :n1], c2_all[rows, :n1], g_all[rows] = c1, c2, gates
         truth = {0: m1 | m2, 1: m1 & m2, 2: 1 - (m1 | m2), 3: 1 - (m1 & m2), 4: m1 ^ m2, 5: 1 - (m1 ^ m2)}
         exp = np.array([truth[int(g)][t] for t, g in enumerate(gates)])
-        sets.append(dict(name=nm, ctx=ctx, keys=keys, s=s, B=B, dg=torch.from_numpy(gates).cuda(), d1=d1, d2=d2,
-                         out=torch.zeros_like(d1), exp=exp))
+        sets.append(dict(name=nm, ctx=ctx, keys=keys, s=s, B=B, rows=rows, n1=n1,
+                         dg=torch.from_numpy(gates).cuda(), d1=torch.from_numpy(c1.view(np.int64)).cuda(),
+                         d2=torch.from_numpy(c2.view(np.int64)).cuda(), exp=exp))
     setup_s = time.perf_counter() - t0
-    g = fhe.MixedGraph([(st["ctx"], st["keys"], st["dg"], st["d1"], st["d2"], st["out"]) for st in sets])
+    dg = torch.from_numpy(g_all).cuda()
+    d1 = torch.from_numpy(c1_all.view(np.int64)).cuda()
+    d2 = torch.from_numpy(c2_all.view(np.int64)).cuda()
+    dout = torch.zeros_like(d1)
+    g = fhe.InterleavedGraph([st["ctx"] for st in sets], [st["keys"] for st in sets], lab, dg, d1, d2, dout)
     g.launch()
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -282,28 +322,32 @@ def mixed(fhe, torch, main_ctx, main_keys, total=8192, reps=3):
     ev1.record()
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / reps
+    got = dout.cpu().numpy().view(np.uint64)
     ok = True
     for st in sets:
         st["ctx"].sync()
-        ok &= bool(np.array_equal(st["ctx"].lwe_decrypt(st["s"], st["out"].cpu().numpy().view(np.uint64)),
-                                  st["exp"]))
+        ok &= bool(np.array_equal(st["ctx"].lwe_decrypt(st["s"], got[st["rows"], :st["n1"]]), st["exp"]))
     # each set alone (its dense launch), back to back: what one launch saves
     per_set = {}
     dense_equal = True
     for st in sets:
-        dout = torch.zeros_like(st["d1"])
+        sout = torch.zeros_like(st["d1"])
+        st["ctx"].bootstrap_dev(st["keys"], st["dg"], st["d1"], st["d2"], sout)
+        torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        st["ctx"].bootstrap_dev(st["keys"], st["dg"], st["d1"], st["d2"], dout)
+        for _ in range(reps):
+            st["ctx"].bootstrap_dev(st["keys"], st["dg"], st["d1"], st["d2"], sout)
         e1.record()
         torch.cuda.synchronize()
-        per_set[st["name"]] = {"B": st["B"], "alone_ms": round(e0.elapsed_time(e1), 3),
-                               "tuning": st["ctx"].get_tuning()}
-        dense_equal &= bool(torch.equal(dout, st["out"]))
+        per_set[st["name"]] = {"B": int(st["B"]), "alone_ms": round(e0.elapsed_time(e1) / reps, 3),
+                               "tuning": st["ctx"].get_tuning(), "schedule": st["ctx"].get_schedule()}
+        dense_equal &= bool(np.array_equal(sout.cpu().numpy().view(np.uint64), got[st["rows"], :st["n1"]]))
     g.close()
     return {
-        "workload": "8192 ciphertexts uniformly interleaved: 60% N=1024 27-bit d=4 n=500, 30% N=2048 54-bit d=3 "
-                    "n=630, 10% N=512 27-bit d=5 n=500; one graph launch, sets back to back",
+        "workload": "8192 ciphertexts uniformly interleaved in one buffer: 60% N=1024 27-bit d=4 n=500, 30% "
+                    "N=2048 54-bit d=3 n=630, 10% N=512 27-bit d=5 n=500; one graph launch, each set's kernels "
+                    "gathering their rows on the device, sets back to back",
         "ms_per_launch": round(ms, 3), "bootstraps_per_s": round(total / (ms * 1e-3), 1),
         "sets": per_set, "sum_alone_ms": round(sum(v["alone_ms"] for v in per_set.values()), 3),
         "decrypt_ok": bool(ok), "equals_per_set_launch": bool(dense_equal), "setup_s": round(setup_s, 2),
@@ -442,52 +486,70 @@ def fpga_orders(fhe, torch, ctx, keys, dg, d1, d2, nega, s, want, reps=3):
     return res
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+# worker threads this process may run on the host: the GPU host's CPU share
+# for one GPU is 16 threads, whatever the machine has
+HOST_THREAD_SHARE = 16
+
+
 def cpu_baseline(ctx, keys, c1, c2, gates, gpu_out, sample):
-    """Time the CPU path on `sample` gates (single thread), bit-compare with the GPU."""
+    """The CPU path on the same batch: the oracle's restatement of
+    RLWE::eval_bootstrap (reference-faithful u128 '%' arithmetic) on one thread
+    and on the host cores, each compared bit for bit with the GPU; the
+    reference's own eval_bootstrap compiled from its sources (oracle/_ref),
+    when present, as a labelled cross-check."""
     import oracle
     from oracle import refbind
 
     bsk = keys.download_bsk()
     ksk = keys.download_ksk()
-    c1s, c2s, gs = c1[:sample], c2[:sample], gates[:sample]
-    if refbind.available():
-        ref = refbind.Reference(ctx.n, ctx.q, ctx.N, ctx.Q, PARAMS["BGbits"], PARAMS["Br"], PARAMS["Bks"])
-        t0 = time.perf_counter()
-        ref.load_keys(bsk, ksk)
-        load_s = time.perf_counter() - t0
-        out, secs = ref.bootstrap(c1s, c2s, gs)
-        kind = "reference"
-        what = "RLWE::eval_bootstrap (FHE_ACC_runtime/src/RLWE.cpp:726-786) compiled from the reference sources"
-    else:
-        o = oracle.Oracle(**PARAMS)
-        load_s = 0.0
-        t0 = time.perf_counter()
-        out = o.bootstrap(c1s, c2s, gs, bsk, ksk, threads=1)
-        secs = time.perf_counter() - t0
-        kind = "port"
-        what = "oracle/fhe_oracle.c restatement of eval_bootstrap"
-    # all host cores: the oracle's restatement, one ciphertext per thread
-    # (the reference's own path is single-threaded)
-    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
     o = oracle.Oracle(**PARAMS)
+    c1s, c2s, gs = c1[:sample], c2[:sample], gates[:sample]
+    t0 = time.perf_counter()
+    out = o.bootstrap(c1s, c2s, gs, bsk, ksk, threads=1)
+    secs = time.perf_counter() - t0
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(HOST_THREAD_SHARE, affinity))
     mc = min(len(c1), 4 * threads)
     t0 = time.perf_counter()
     out_mc = o.bootstrap(c1[:mc], c2[:mc], gates[:mc], bsk, ksk, threads=threads)
     mc_secs = time.perf_counter() - t0
-    multicore = {"value": round(mc / mc_secs, 3), "unit": "bootstraps/s", "cores": threads, "kind": "port",
-                 "sample": "%d NAND gate bootstraps, oracle/fhe_oracle.c restatement, %d threads" % (mc, threads),
-                 "bit_exact_vs_gpu": bool(np.array_equal(out_mc, gpu_out[:mc]))}
-    return {
-        "value": sample / secs,
+    rec = {
+        "value": round(sample / secs, 3),
         "unit": "bootstraps/s",
         "cores": 1,
-        "kind": kind,
-        "all_cores": multicore,
-        "sample": "%d NAND gate bootstraps of the benchmark batch, single thread, %s; key load %.1f s untimed"
-                  % (sample, what, load_s),
-        "seconds": secs,
+        "kind": "port",
+        "sample": "%d NAND gate bootstraps of the benchmark batch, single thread, oracle/fhe_oracle.c restatement "
+                  "of RLWE::eval_bootstrap (RLWE.cpp:726-786) with u128 '%%' arithmetic" % sample,
+        "seconds": round(secs, 3),
         "bit_exact_vs_gpu": bool(np.array_equal(out, gpu_out[:sample])),
+        "all_cores": {"value": round(mc / mc_secs, 3), "unit": "bootstraps/s", "cores": threads, "kind": "port",
+                      "sample": "%d NAND gate bootstraps, one ciphertext per thread, %d threads" % (mc, threads),
+                      "bit_exact_vs_gpu": bool(np.array_equal(out_mc, gpu_out[:mc]))},
+        "host": {"cpu_model": cpu_model(), "hardware_concurrency": os.cpu_count(), "affinity_cpus": affinity,
+                 "thread_share": HOST_THREAD_SHARE,
+                 "note": "the all-cores leg runs min(affinity, the host's 16-thread share per GPU) threads"},
     }
+    if refbind.available():
+        rs = min(sample, 10)
+        ref = refbind.Reference(ctx.n, ctx.q, ctx.N, ctx.Q, PARAMS["BGbits"], PARAMS["Br"], PARAMS["Bks"])
+        ref.load_keys(bsk, ksk)
+        rout, rsecs = ref.bootstrap(c1[:rs], c2[:rs], gates[:rs])
+        rec["reference_cross_check"] = {
+            "value": round(rs / rsecs, 3), "unit": "bootstraps/s", "cores": 1, "kind": "reference",
+            "sample": "%d gates through RLWE::eval_bootstrap compiled from the reference sources (oracle/_ref)" % rs,
+            "bit_exact_vs_gpu": bool(np.array_equal(rout, gpu_out[:rs]))}
+    return rec
 
 
 def main():
@@ -685,7 +747,13 @@ def main():
         "schedule": {"persist": schedule[0], "sync_window": schedule[1]},
     }
     if args.serve_seconds > 0:
-        rec["serving"] = serving(fhe, ctx, args)
+        clients, keygen_s_clients = serving_clients(ctx)
+        rec["serving"] = serving(fhe, ctx, clients, args.serve_qps, args.serve_seconds, args.serve_max_batch,
+                                 args.serve_inflight, args.serve_min_fill, not args.serve_fifo)
+        rec["serving"]["client_keygen_s"] = round(keygen_s_clients, 3)
+        if args.serve_curve:
+            rec["serving_curve"] = serving_curve(fhe, ctx, clients, args)
+        del clients
         rec["latency_ms"] = {"p50": rec["serving"]["gate_latency_ms_p50"],
                              "p99": rec["serving"]["gate_latency_ms_p99"],
                              "request_p50": rec["serving"]["request_latency_ms_p50"],

@@ -287,6 +287,17 @@ int fheacc_graph_capture_bootstrap(fheacc_ctx *ctx, fheacc_keys *keys, const int
 int fheacc_graph_capture_mixed(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *keys,
                                const int32_t *const *gates, const uint64_t *const *c1, const uint64_t *const *c2,
                                uint64_t *const *out, const int64_t *B, fheacc_graph **graph);
+/* A mixed batch as it arrives: total ciphertexts of several parameter sets
+ * interleaved in one buffer, row r (row_words words apart, at least the
+ * largest n + 1) belonging to set set_of[r] (a host array read at capture).
+ * Each set's kernels gather their rows (operands, gate, output) by index on
+ * the device, so nothing is sorted or copied per launch; out has the same
+ * rows.  Each row's words are those of fheacc_bootstrap_batch_dev on its
+ * set's context and keys.  Launches like fheacc_graph_capture_mixed. */
+int fheacc_graph_capture_interleaved(int nsets, fheacc_ctx *const *ctxs, fheacc_keys *const *keys,
+                                     const int32_t *set_of, const int32_t *gates, const uint64_t *c1,
+                                     const uint64_t *c2, uint64_t *out, int64_t total, int64_t row_words,
+                                     fheacc_graph **graph);
 int fheacc_graph_launch(fheacc_graph *graph);
 int fheacc_graph_destroy(fheacc_graph *graph);
 

@@ -172,3 +172,60 @@ def test_fair_share_lets_a_small_job_pass_a_large_one(serving, fair):
     assert np.array_equal(out_s, dense_eval(ctx, clients[small["client"]][1], small))
     if fair:
         assert lat_s < lat_b
+
+
+def test_poisson_load_at_the_serving_shape():
+    # the serving workload's own shape: 8 clients with resident keys, Poisson
+    # arrivals (6 req/s for 3 s), k-bit set equality with k in {8, 16, 32},
+    # item counts log-uniform in [1, 1024], values uniform 32-bit.  Every
+    # request decrypts to its equality bits, and a sample re-run alone on the
+    # idle server gives the same words as in the mix.
+    import time
+    ctx = fhe.Context(**CFG0)
+    clients = []
+    for c in range(8):
+        s, z = ctx.lwe_secret(700 + c), ctx.rlwe_secret(700 + c)
+        keys = ctx.keys()
+        keys.generate(800 + c, s, z)
+        clients.append((s, keys))
+    rng = np.random.default_rng(74)
+    arrivals, t = [], rng.exponential(1 / 6.0)
+    while t < 3.0:
+        arrivals.append(t)
+        t += rng.exponential(1 / 6.0)
+    reqs = []
+    for i, ta in enumerate(arrivals):
+        c, k = int(rng.integers(0, 8)), int(rng.choice([8, 16, 32]))
+        m = int(min(1024, max(1, np.floor(np.exp(rng.uniform(0.0, np.log(1025.0)))))))
+        items = rng.integers(0, 2 ** 32, m, dtype=np.uint64)
+        query = int(rng.integers(0, 2 ** 32, dtype=np.uint64))
+        mask = (1 << k) - 1
+        items[0] = (int(items[0]) & ~mask) | (query & mask)
+        n_in, ops, outs = circuits.equality_circuit(m, k)
+        inputs = ctx.lwe_encrypt(clients[c][0], circuits.equality_inputs(items, query, k), 75, i << 20)
+        want = np.array([(int(v) & mask) == (query & mask) for v in items], dtype=np.uint8)
+        reqs.append(dict(t=ta, client=c, inputs=inputs, ops=ops, outs=outs, want=want))
+    print("requests", len(reqs), "gates", sum(len(r["ops"]) for r in reqs), flush=True)
+    srv = fhe.Server(ctx, hbm_budget=8 << 30, max_batch=4096, max_inflight=1)
+    ids = [srv.add_client(k) for _, k in clients]
+    try:
+        jids, t0 = [], time.perf_counter()
+        for r in reqs:
+            dt = t0 + r["t"] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            jids.append(srv.submit(ids[r["client"]], r["inputs"], r["ops"], r["outs"]))
+        outs = [srv.wait(j)[0] for j in jids]
+        st = srv.stats()
+        for r, o in zip(reqs, outs):
+            assert np.array_equal(ctx.lwe_decrypt(clients[r["client"]][0], o), r["want"])
+        assert st["jobs_done"] == len(reqs) and st["jobs_failed"] == 0
+        assert st["bootstraps"] == sum(len(r["ops"]) for r in reqs)
+        assert st["gate_ms_p99"] >= st["gate_ms_p50"] > 0
+        order = sorted(range(len(reqs)), key=lambda i: len(reqs[i]["ops"]))
+        for i in order[:3] + order[-1:]:
+            r = reqs[i]
+            alone, _ = srv.wait(srv.submit(ids[r["client"]], r["inputs"], r["ops"], r["outs"]))
+            assert np.array_equal(alone, outs[i]), i
+    finally:
+        srv.close()
