@@ -134,13 +134,16 @@ int check_keys(const fheacc_ctx *c, const fheacc_keys *k) {
 }
 
 // LDS of the blind-rotation kernel with pb digit polys per batch: pb >= 2d
-// runs the product in place in 2d polys, otherwise acc (2 polys) plus a
-// pb-poly digit buffer; then the n evaluated a words.
+// runs the product in place in 2d polys, pb == d in place in d polys (two
+// batches), otherwise acc (2 polys) plus a pb-poly digit buffer; then the n
+// evaluated a words.
 size_t br_lds_bytes(const HostParams &p, int pb) {
     const size_t w = p.narrow ? 4 : 8;
-    const int d2 = 2 * (int)p.digitG;
-    const size_t polys = pb >= d2 ? (size_t)padded(std::max(2, d2) * p.N)
-                                  : (size_t)padded(2 * p.N) + (size_t)padded(pb * p.N);
+    const int d = (int)p.digitG;
+    const int form = product_form(pb, d);
+    const size_t polys = form == kInPlace ? (size_t)padded(std::max(2, 2 * d) * p.N)
+                         : form == kSplit ? (size_t)padded(std::max(2, d) * p.N)
+                                          : (size_t)padded(2 * p.N) + (size_t)padded(pb * p.N);
     return polys * w + (size_t)p.n * 2 + 16;
 }
 
@@ -1808,9 +1811,15 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
     double best = 1e30;
     int best_t = saved.br_threads, best_pb = saved.br_pb;
     bool first = true;
+    // digit batches: even sizes up to 2d (2d: the whole batch in place) and d
+    // (in place in two batches, half the LDS)
+    std::vector<int> pbs;
+    for (int pb = 2; pb <= 2 * (int)p.digitG; pb += 2) pbs.push_back(pb);
+    if (p.digitG & 1) pbs.push_back((int)p.digitG);
+    std::sort(pbs.begin(), pbs.end());
     for (int threads : {128, 256, 512, 1024}) {
         if (threads > p.N || (threads == 1024 && p.N < 1024)) continue;
-        for (int pb = 2; pb <= 2 * (int)p.digitG; pb += 2) {
+        for (int pb : pbs) {
             const size_t lds = br_lds_bytes(p, pb);
             if (lds > c->lds_budget) continue;
             double br = 0;

@@ -10,6 +10,15 @@ namespace fheacc {
 
 struct ShoupGen;
 
+// Product forms of the blind rotation (the launcher picks one from pb):
+// the whole 2d-poly digit batch in place (pb >= 2d), two in-place batches of
+// d polys, the digits of a then of b (pb == d), or an accumulator plus a
+// separate pb-poly digit buffer (any other pb).
+enum ProductForm { kSeparate = 0, kInPlace = 1, kSplit = 2 };
+__host__ __device__ __forceinline__ int product_form(int pb, int d) {
+    return pb >= 2 * d ? kInPlace : (pb == d ? kSplit : kSeparate);
+}
+
 template <typename W>
 struct BRArgs {
     Arith<W> A;

@@ -411,6 +411,156 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     __syncthreads();
 }
 
+// Two key words of row h at coefficient k: (K0a, K0b) for the digits of a
+// (half 0) or (K1a, K1b) for those of b (half 1), with companions when wide.
+template <typename W>
+struct KeyPair {
+    W v[2], p[2];
+};
+__device__ __forceinline__ KeyPair<uint32_t> load_pair(const uint32_t *key, int idx, int half) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 q = ld_global<u32x2>(key, 2 * (size_t)idx + half);
+    KeyPair<uint32_t> r;
+    r.v[0] = q.x; r.v[1] = q.y;
+    r.p[0] = r.p[1] = 0;
+    return r;
+}
+__device__ __forceinline__ KeyPair<uint64_t> load_pair(const uint64_t *key, int idx, int half) {
+    KeyPair<uint64_t> r;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const u64x2 t = ld_global<u64x2>(key, 4 * (size_t)idx + 2 * half + i);
+        r.v[i] = t.x;
+        r.p[i] = t.y;
+    }
+    return r;
+}
+
+// The in-place product in two batches of d digit polys, the digits of a and
+// then those of b, so that buf holds max(2, d) padded polys instead of 2d.
+// Each thread keeps its coefficients' a[k], b[k] in registers from the
+// inverse transform on, and sums both halves' products for its own k
+// before the new accumulator is written.  Half the LDS of ext_prod_inplace
+// for the same transforms and products: a 54-bit N=2048 d=3 gate takes 55 KB
+// instead of 110 KB, so two workgroups share a CU.
+template <typename W, int LOGN, int NT>
+__device__ void ext_prod_split(W *buf, const W *key, const Arith<W> &A, const Twiddles<W> &tw, int d, int bits) {
+    constexpr int N = 1 << LOGN;
+    constexpr int CPT = (N + NT - 1) / NT;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));  // see ext_prod
+    constexpr bool kFull = N % NT == 0;
+    // 64-bit words: 3-stage inverse passes; the forward passes, which run
+    // beside b's held words and the sums, take 2 stages to fit 128 registers
+    // (3 spill at 512 threads)
+    constexpr int MAXR = sizeof(W) == 8 ? 3 : 4;
+    constexpr int MAXRF = sizeof(W) == 8 ? 2 : 4;
+    constexpr int PS = N + N / 8;  // padded words per poly
+    ntt_inverse<LOGN, NT, MAXR>(buf, 2, A, tw, tid);
+    const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
+    W xa[CPT], xb[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int k = tid + c * NT;
+        if (kFull || k < N) {
+            xa[c] = buf[pad(k)];
+            xb[c] = buf[pad(N + k)];
+        }
+    }
+    __syncthreads();
+    MacSum<W> sa[CPT], sb[CPT];
+    // one half: its digit polys, their transforms and products (called twice,
+    // so that a's words are dead once their digits are written)
+    auto half_product = [&](const W *xs, const int half) {
+        // digit h of a (half 0) or of b (half 1) goes to poly h
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int k = tid + c * NT;
+            if (kFull || k < N) {
+                const W x = xs[c];
+                W *col = buf + pad(k);
+                if (d <= 8) {
+#pragma unroll
+                    for (int h = 0; h < 8; h++)
+                        if (h < d) col[h * PS] = (x >> (h * bits)) & mask;
+                } else {
+                    for (int h = 0; h < d; h++) col[h * PS] = (x >> (h * bits)) & mask;
+                }
+            }
+        }
+        __syncthreads();
+        ntt_forward<LOGN, NT, MAXRF>(buf, d, A, tw, MacSum<W>::kCanonicalInputs, tid);
+        // rows outer, the thread's coefficients inner; row h+1's key words are
+        // in flight while row h is summed
+        KeyPair<W> cur[CPT], nxt[CPT];
+        const W *dcol[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            dcol[c] = buf + pad(tid + c * NT);
+            if constexpr (sizeof(W) == 4)
+                if (kFull || tid + c * NT < N) cur[c] = load_pair(key, tid + c * NT, half);
+        }
+        auto row = [&](const KeyPair<W> *q) {
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                if (kFull || tid + c * NT < N) {
+                    const W dg = dcol[c][0];
+                    sa[c].add(A, dg, q[c].v[0], q[c].p[0]);
+                    sb[c].add(A, dg, q[c].v[1], q[c].p[1]);
+                }
+                dcol[c] += PS;
+            }
+        };
+        auto fetch = [&](KeyPair<W> *q, int h) {
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+                if (kFull || tid + c * NT < N) q[c] = load_pair(key, (h << LOGN) + tid + c * NT, half);
+        };
+        if constexpr (sizeof(W) == 8) {
+            // 64-bit words: each coefficient's key words are loaded just
+            // before its products (the 16 waves of two workgroups per CU
+            // cover the loads), so the products fit 128 registers
+#pragma unroll 1
+            for (int h = 0; h < d; h++) {
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    if (kFull || tid + c * NT < N) {
+                        const KeyPair<W> q = load_pair(key, (h << LOGN) + tid + c * NT, half);
+                        const W dg = dcol[c][0];
+                        sa[c].add(A, dg, q.v[0], q.p[0]);
+                        sb[c].add(A, dg, q.v[1], q.p[1]);
+                    }
+                    dcol[c] += PS;
+                }
+            }
+        } else {
+#pragma unroll 1
+            for (int h = 0; h < d; h += 2) {
+                if (h + 1 < d) fetch(nxt, h + 1);
+                row(cur);
+                if (h + 1 < d) {
+                    if (h + 2 < d) fetch(cur, h + 2);
+                    row(nxt);
+                }
+            }
+        }
+        // every digit poly has been read: the next half (or the new
+        // accumulator) overwrites it
+        __syncthreads();
+    };
+    half_product(xa, 0);
+    half_product(xb, 1);
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int k = tid + c * NT;
+        if (kFull || k < N) {
+            buf[pad(k)] = sa[c].get(A);
+            buf[pad(N + k)] = sb[c].get(A);
+        }
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // K10 + K5 + blind rotation + KS preparation, one ciphertext per workgroup
 // (RLWE.cpp:726-767, LWE.cpp:203-222,233-240, RLWE.cpp:218-226)
@@ -467,17 +617,19 @@ struct StepPacer {
 // One gate bootstrap's blind rotation and key-switch preparation on the
 // workgroup's LDS: ciphertext c, thread index tid (an opaque copy in the
 // persistent kernel), steps counted from step0 for pacing.
-template <typename W, int LOGN, int NT, bool INPLACE, bool PACE>
+template <typename W, int LOGN, int NT, int FORM, bool PACE>
 __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, StepPacer &pace, uint32_t step0) {
     constexpr int N = 1 << LOGN;
     extern __shared__ __align__(16) unsigned char smem[];
-    // pb >= 2d: every digit poly in one batch, product computed in place in
-    // 2d polys; otherwise acc (2 polys) and a separate pb-poly digit buffer
-    constexpr bool inplace = INPLACE;  // the launcher picks it from pb >= 2d
+    // kInPlace: every digit poly in one batch, product computed in place in
+    // 2d polys; kSplit: in place in d polys, digits of a then of b;
+    // kSeparate: acc (2 polys) and a separate pb-poly digit buffer (the
+    // launcher picks the form from pb)
     W *acc = reinterpret_cast<W *>(smem);
     W *dig = acc + padded(2 * N);
     uint16_t *av = reinterpret_cast<uint16_t *>(
-        inplace ? acc + padded(max(2, 2 * a.d) * N) : dig + padded(a.pb * N));
+        FORM == kInPlace ? acc + padded(max(2, 2 * a.d) * N)
+                         : (FORM == kSplit ? acc + padded(max(2, a.d) * N) : dig + padded(a.pb * N)));
     __shared__ uint32_t s_b;
     const int n = a.n;
     const uint32_t q = a.q, qm = a.q - 1;
@@ -533,7 +685,8 @@ __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, S
             if constexpr (PACE) pace.wait(step0 + t);
             if (!bad_gate && (r || mode == FHEACC_EVAL_FPGA_XN || (mode == FHEACC_EVAL_FPGA_X1 && i == 0 && j == 0))) {
                 const W *key = bsk + ((size_t)(i * a.digitR + j) * a.Br + r) * a.rgsw_elems;
-                if constexpr (inplace) ext_prod_inplace<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
+                if constexpr (FORM == kInPlace) ext_prod_inplace<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
+                else if constexpr (FORM == kSplit) ext_prod_split<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
                 else ext_prod<W, LOGN, NT>(acc, dig, key, a.A, a.tw, a.d, a.bits, a.pb);
             }
             if constexpr (PACE) pace.done(step0 + t);
@@ -571,10 +724,19 @@ __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, S
 
 // One ciphertext per workgroup; the hardware dispatcher refills CUs as gates
 // finish.
-template <typename W, int LOGN, int NT, bool INPLACE>
-__global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
+// Waves per SIMD the blind rotation and the CMUX chains are compiled for.  The
+// split product on 64-bit words at 512 threads runs two workgroups per CU
+// (55 KB of LDS each at N = 2048), 16 waves, so it must fit 128 registers;
+// every other shape keeps the register budget its workgroup size gives.
+template <typename W, int NT, int FORM>
+constexpr int product_waves_per_eu() {
+    return (FORM == kSplit && sizeof(W) == 8 && NT == 512) ? 4 : 1;
+}
+
+template <typename W, int LOGN, int NT, int FORM>
+__global__ __launch_bounds__(NT, (product_waves_per_eu<W, NT, FORM>())) void k_blind_rotate(BRArgs<W> a) {
     StepPacer none;
-    br_one<W, LOGN, NT, INPLACE, false>(a, a.c_base + blockIdx.x, threadIdx.x, none, 0);
+    br_one<W, LOGN, NT, FORM, false>(a, a.c_base + blockIdx.x, threadIdx.x, none, 0);
 }
 
 // Persistent and paced: a resident grid of G workgroups, workgroup w taking
@@ -582,8 +744,8 @@ __global__ __launch_bounds__(NT) void k_blind_rotate(BRArgs<W> a) {
 // ciphertexts, so a group stays within a window of steps from gate to gate
 // without waiting at gate boundaries.  A workgroup out of work marks itself
 // finished so that no one waits for it.
-template <typename W, int LOGN, int NT, bool INPLACE>
-__global__ __launch_bounds__(NT) void k_blind_rotate_paced(BRArgs<W> a) {
+template <typename W, int LOGN, int NT, int FORM>
+__global__ __launch_bounds__(NT, (product_waves_per_eu<W, NT, FORM>())) void k_blind_rotate_paced(BRArgs<W> a) {
     StepPacer pace;
     if (a.sync) {
         pace.prog = a.sync;
@@ -600,7 +762,7 @@ __global__ __launch_bounds__(NT) void k_blind_rotate_paced(BRArgs<W> a) {
         // recomputed for each gate instead of held across the whole loop
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        br_one<W, LOGN, NT, INPLACE, true>(a, c, tid, pace, step0);
+        br_one<W, LOGN, NT, FORM, true>(a, c, tid, pace, step0);
         __syncthreads();  // the next gate rewrites the LDS
     }
     if (a.sync && threadIdx.x == 0)
@@ -1007,8 +1169,8 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
 // difference X(acc) - acc goes through the external product in LDS.
 // ---------------------------------------------------------------------------
 
-template <typename W, int LOGN, int NT, bool INPLACE>
-__global__ __launch_bounds__(NT) void k_cmux_chain(CmuxArgs<W> a) {
+template <typename W, int LOGN, int NT, int FORM>
+__global__ __launch_bounds__(NT, (product_waves_per_eu<W, NT, FORM>())) void k_cmux_chain(CmuxArgs<W> a) {
     constexpr int N = 1 << LOGN;
     constexpr int CPT = (N + NT - 1) / NT;
     constexpr bool kFull = N % NT == 0;
@@ -1046,7 +1208,8 @@ __global__ __launch_bounds__(NT) void k_cmux_chain(CmuxArgs<W> a) {
         }
         __syncthreads();
         const W *key = a.key + c * a.key_ct_stride + st * a.key_stride;
-        if constexpr (INPLACE) ext_prod_inplace<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
+        if constexpr (FORM == kInPlace) ext_prod_inplace<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
+        else if constexpr (FORM == kSplit) ext_prod_split<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
         else ext_prod<W, LOGN, NT>(acc, dig, key, a.A, a.tw, a.d, a.bits, a.pb);
 #pragma unroll
         for (int cc = 0; cc < CPT; cc++) {
@@ -1492,10 +1655,15 @@ static hipError_t launch_br_nt(const BRArgs<W> &a, int64_t B, size_t lds, hipStr
     // the paced persistent kernel exists for the throughput shapes (256 and
     // 512 threads, N >= 512)
     constexpr bool kPaceable = LOGN >= 9 && (NT == 256 || NT == 512);
-    auto k = a.pb >= 2 * a.d ? k_blind_rotate<W, LOGN, NT, true> : k_blind_rotate<W, LOGN, NT, false>;
+    const int form = product_form(a.pb, a.d);
+    auto k = form == kInPlace ? k_blind_rotate<W, LOGN, NT, kInPlace>
+                              : (form == kSplit ? k_blind_rotate<W, LOGN, NT, kSplit>
+                                                : k_blind_rotate<W, LOGN, NT, kSeparate>);
     if constexpr (kPaceable) {
         if (persist > 0 && cus > 0 && a.sync) {
-            auto kp = a.pb >= 2 * a.d ? k_blind_rotate_paced<W, LOGN, NT, true> : k_blind_rotate_paced<W, LOGN, NT, false>;
+            auto kp = form == kInPlace ? k_blind_rotate_paced<W, LOGN, NT, kInPlace>
+                                       : (form == kSplit ? k_blind_rotate_paced<W, LOGN, NT, kSplit>
+                                                         : k_blind_rotate_paced<W, LOGN, NT, kSeparate>);
             hipError_t e = hipFuncSetAttribute((const void *)kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
             int occ = 0;
@@ -1661,7 +1829,9 @@ hipError_t launch_expand_level(const SubsArgs<W> &a, int logN, int64_t B, hipStr
 
 template <typename W, int LOGN, int NT>
 static hipError_t launch_cmux_nt(const CmuxArgs<W> &a, int64_t count, size_t lds, hipStream_t st) {
-    auto k = a.pb >= 2 * a.d ? k_cmux_chain<W, LOGN, NT, true> : k_cmux_chain<W, LOGN, NT, false>;
+    const int form = product_form(a.pb, a.d);
+    auto k = form == kInPlace ? k_cmux_chain<W, LOGN, NT, kInPlace>
+                              : (form == kSplit ? k_cmux_chain<W, LOGN, NT, kSplit> : k_cmux_chain<W, LOGN, NT, kSeparate>);
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     for (int64_t off = 0; off < count; off += max_wgs_per_launch<NT>()) {

@@ -100,6 +100,9 @@ def parse():
                     help="skip the reference's 256-bit application set (29-bit Q, N=2048)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (reference dispatch for HBM byte counters)")
+    ap.add_argument("--only-set", default="", choices=["", "A", "B", "C"],
+                    help="time one set of the mixed batch alone (its dense launch at its share of 8192; "
+                         "--tuning / --schedule apply to it) and print that set's JSON line instead")
     return ap.parse_args()
 
 
@@ -296,7 +299,9 @@ def mixed(fhe, torch, main_ctx, main_keys, total=8192, reps=3):
             s, z = ctx.lwe_secret(SEED + 10 + i, binary=True), ctx.rlwe_secret(SEED + 10 + i)
             keys = ctx.keys()
             keys.generate(SEED + 20 + i, s, z)
-            ctx.autotune(keys, min(B, 1024))
+            # tuned at the set's own batch: launch shapes that win on a full
+            # chip lose on a partial last wave and the other way round
+            ctx.autotune(keys, B)
         m1, m2 = rng.integers(0, 2, B), rng.integers(0, 2, B)
         gates = rng.integers(0, 6, B).astype(np.int32)
         c1, c2 = ctx.lwe_encrypt(s, m1, SEED + 30 + i, 0), ctx.lwe_encrypt(s, m2, SEED + 30 + i, B)
@@ -351,6 +356,61 @@ def mixed(fhe, torch, main_ctx, main_keys, total=8192, reps=3):
         "ms_per_launch": round(ms, 3), "bootstraps_per_s": round(total / (ms * 1e-3), 1),
         "sets": per_set, "sum_alone_ms": round(sum(v["alone_ms"] for v in per_set.values()), 3),
         "decrypt_ok": bool(ok), "equals_per_set_launch": bool(dense_equal), "setup_s": round(setup_s, 2),
+    }
+
+
+def one_set(fhe, torch, args, total=8192):
+    """One set of the mixed batch on its own: its share of the 8192 seeded
+    random-bit ciphertexts over all six gates, as one dense gate-path launch,
+    timed over --steps launches after --warmup, with per-kernel HIP events."""
+    nm = args.only_set
+    B = int(round(total * MIXED_SHARE[nm]))
+    ctx = fhe.Context(**MIXED[nm])
+    i = list(MIXED).index(nm)
+    s, z = ctx.lwe_secret(SEED + 10 + i, binary=True), ctx.rlwe_secret(SEED + 10 + i)
+    keys = ctx.keys()
+    keys.generate(SEED + 20 + i, s, z)
+    if args.tuning == "auto":
+        tune = ctx.autotune(keys, B)
+    else:
+        th, pb = (int(x) for x in args.tuning.split(","))
+        ctx.set_tuning(th, pb)
+        tune = {"chosen": {"threads": th, "polys_per_batch": pb}, "source": "--tuning"}
+    if args.schedule != "auto":
+        ctx.set_schedule(*(int(x) for x in args.schedule.split(",")))
+    rng = np.random.default_rng(SEED + 40 + i)
+    m1, m2 = rng.integers(0, 2, B), rng.integers(0, 2, B)
+    gates = rng.integers(0, 6, B).astype(np.int32)
+    c1, c2 = ctx.lwe_encrypt(s, m1, SEED + 30 + i, 0), ctx.lwe_encrypt(s, m2, SEED + 30 + i, B)
+    truth = {0: m1 | m2, 1: m1 & m2, 2: 1 - (m1 | m2), 3: 1 - (m1 & m2), 4: m1 ^ m2, 5: 1 - (m1 ^ m2)}
+    exp = np.array([truth[int(g)][t] for t, g in enumerate(gates)])
+    d1 = torch.from_numpy(c1.view(np.int64)).cuda()
+    d2 = torch.from_numpy(c2.view(np.int64)).cuda()
+    dg = torch.from_numpy(gates).cuda()
+    out = torch.zeros_like(d1)
+    for _ in range(max(args.warmup, 1)):
+        ctx.bootstrap_dev(keys, dg, d1, d2, out)
+    ctx.sync()
+    ctx.profile(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        ctx.bootstrap_dev(keys, dg, d1, d2, out)
+    e1.record()
+    torch.cuda.synchronize()
+    ctx.sync()
+    kms = ctx.kernel_ms()
+    ctx.profile(False)
+    ms = e0.elapsed_time(e1) / args.steps
+    ok = bool(np.array_equal(ctx.lwe_decrypt(s, out.cpu().numpy().view(np.uint64)), exp))
+    info = ctx.info
+    return {
+        "workload": "mixed-batch set %s alone: %s, %d seeded random-bit ciphertexts over all six gates, dense "
+                    "eager launches" % (nm, json.dumps(MIXED[nm]), B),
+        "B": B, "ms_per_batch": round(ms, 3), "bootstraps_per_s": round(B / (ms * 1e-3), 1),
+        "kernels_ms_avg": {k: round(v[0] / max(v[1], 1), 4) for k, v in kms.items() if v[1]},
+        "bsk_device_bytes": info["bsk_device_bytes"], "decrypt_ok": ok,
+        "tuning": tune, "schedule": ctx.get_schedule(), "steps": args.steps, "warmup": args.warmup,
     }
 
 
@@ -568,6 +628,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.only_set:
+        print(json.dumps(one_set(fhe, torch, args)))
+        return
     B = args.batch
     ctx = fhe.Context(**PARAMS)
     keys = ctx.keys()

@@ -103,7 +103,10 @@ int fheacc_ctx_kernel_ms(fheacc_ctx *ctx, double *ms4, int64_t *launches4);
 
 /* Launch shapes of the blind-rotation kernel: threads per workgroup (128,
  * 256, 512, or 1024 when N >= 1024) and digit polys per LDS batch (0: as
- * many as fit).  Once set, the CMUX chains use the same shape; until then
+ * many as fit).  A batch of 2d polys runs the product in place; d polys run
+ * it in place in two batches (the digits of a, then of b) in half the LDS;
+ * any other size uses a separate digit buffer beside the accumulator.  Every
+ * shape gives the same words.  Once set, the CMUX chains use the same shape; until then
  * they run 1024 threads from N = 1024 up.  No
  * counterpart in the reference (the FPGA's pipeline depth is fixed,
  * myfpga.cpp:1533). */

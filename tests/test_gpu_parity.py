@@ -173,7 +173,9 @@ def test_every_launch_shape_bit_exact(name, shapes):
     rg = rng.integers(0, o.Q, o.rgsw_words, dtype=np.uint64)
     cm = np.stack([o.cmux(x0[i], x1[i], rg) for i in range(3)])
     for threads in shapes:
-        for pb in sorted({1, 2, 3, 4, 2 * ctx.digitG}):  # pb = 2d: the in-place product
+        # pb = 2d: the in-place product; pb = d: in place in two batches (the
+        # digits of a, then of b); any other pb: a separate digit buffer
+        for pb in sorted({1, 2, 3, 4, ctx.digitG, 2 * ctx.digitG}):
             ctx.set_tuning(threads, pb)
             assert np.array_equal(ctx.bootstrap(keys, gates, c1, c2), exp), (threads, pb)
             out = torch.zeros((3, 2, o.N), dtype=torch.int64, device="cuda")
