@@ -944,6 +944,21 @@ static int ks_gather_split(const HostParams &p, int64_t B) {
     return (int)std::max<int64_t>(split, 1);
 }
 
+// Ciphertexts per staged key-switch workgroup: each workgroup streams its
+// column block of the whole key once, so more ciphertexts per workgroup means
+// fewer passes over the key.  On 64-bit words (keys of several GB: 3.6 GB at
+// N = 2048, n = 630) the largest of 1024, 512 that still gives every CU a
+// workgroup (the grid is column blocks x ciphertext groups): 16.2 -> 14.7 ms
+// for 2458 ciphertexts at 512.  32-bit keys stay at 256, where 512 measured
+// slower (1.37 -> 1.52 ms at the benchmark batch).
+static int ks_staged_threads(const HostParams &p, int64_t B, int cus) {
+    if (p.narrow) return 256;
+    const int64_t col_blocks = (p.n + 1 + 7) / 8;
+    for (int nt : {1024, 512})
+        if (col_blocks * ((B + nt - 1) / nt) >= std::max(cus, 1)) return nt;
+    return 256;
+}
+
 // partial-sum rows the gather key switch may need for any launch of at most
 // B ciphertexts (a serving-loop segment can be smaller than its launch)
 static int64_t ks_part_rows(const HostParams &p, int64_t B) {
@@ -1044,9 +1059,6 @@ static int gate_path_impl(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch 
     int lq = 0;
     while ((1ULL << lq) < p.q) lq++;
     s.lq1 = lq + 1;
-    s.KB = c->tune.ks_kb;
-    s.CB = c->tune.ks_cb;
-    s.SB = c->tune.ks_sb;
     // one key-switch launch per key segment (a dense batch is one segment)
     const int nseg = g.nseg > 0 ? g.nseg : 1;
     for (int sg = 0; sg < nseg; sg++) {
@@ -1060,7 +1072,9 @@ static int gate_path_impl(fheacc_ctx *c, const fheacc_keys *k, const GateLaunch 
         s.out = g.oidx ? g.out : g.out + o0 * (p.n + 1);
         s.cap_ks = g.cap && g.cap->ks ? g.cap->ks + o0 * (p.n + 1) : nullptr;
         s.B = o1 - o0;
-        const bool gather = s.B <= c->tune.ks_gather_max || ks_staged_lds<W>(s.Bks) > c->lds_budget;
+        s.nt = c->tune.ks_threads > 0 ? c->tune.ks_threads : ks_staged_threads(p, s.B, c->cus);
+        if (ks_staged_lds<W>(s.Bks, s.nt) > c->lds_budget) s.nt = 256;
+        const bool gather = s.B <= c->tune.ks_gather_max || ks_staged_lds<W>(s.Bks, s.nt) > c->lds_budget;
         s.split = gather ? ks_gather_split(p, s.B) : 1;
         s.part = part;  // segments run in order on one stream and reuse it
         Prof pr(c, 1, g.stream, g.profile);

@@ -62,9 +62,7 @@ struct DeviceTables {
 struct Tuning {
     int br_threads = 256;     // threads per blind-rotation workgroup
     int br_pb = 0;            // digit polys per NTT batch (0: as many as fit)
-    int ks_cb = 512;          // ciphertexts per key-switch workgroup
-    int ks_kb = 8;            // output columns per key-switch workgroup
-    int ks_sb = 32;           // key-switch steps staged per LDS fill
+    int ks_threads = 0;       // staged key switch: ciphertexts per workgroup (0: ks_staged_threads)
     int ks_gather_max = 32;   // batches up to this size use the gather key switch
     // Batches of at most small_max ciphertexts (one workgroup per CU) leave
     // most of the chip idle; they run at br_threads_small threads per gate,

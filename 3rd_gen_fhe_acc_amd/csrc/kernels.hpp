@@ -74,7 +74,7 @@ struct KSArgs {
     const int64_t *oidx;  // output row of ciphertext c (null: row c)
     int64_t out_row_words;  // words from one output row to the next (n + 1 when dense)
     int64_t B;
-    int CB, KB, SB;
+    int nt;  // staged form: threads per workgroup, the ciphertexts sharing one key stream (256, 512, 1024)
     // gather form: the S steps of a ciphertext split into `split` ranges, one
     // workgroup each, partial sums in part[B][split][n+1] (split > 1 only)
     int split;
@@ -211,7 +211,7 @@ hipError_t launch_blind_rotate(const BRArgs<W> &a, int logN, int nt, int64_t B, 
 template <typename W>
 hipError_t launch_lwe_keyswitch(const KSArgs<W> &a, bool gather, hipStream_t st);
 template <typename W>
-size_t ks_staged_lds(uint32_t Bks);  // LDS bytes of the staged key switch
+size_t ks_staged_lds(uint32_t Bks, int nt);  // LDS bytes of the staged key switch at nt threads
 template <typename W>
 hipError_t launch_ntt(const NTTArgs<W> &a, int logN, hipStream_t st);
 template <typename W>

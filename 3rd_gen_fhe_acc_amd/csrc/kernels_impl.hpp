@@ -1734,20 +1734,26 @@ hipError_t launch_lwe_keyswitch(const KSArgs<W> &a, bool gather, hipStream_t st)
         return hipGetLastError();
     }
     constexpr int KB = sizeof(W) == 4 ? 16 : 8;
-    dim3 grid((a.n + 1 + KB - 1) / KB, (unsigned)((a.B + 255) / 256));
-    const size_t lds = ks_staged_lds<W>(a.Bks);
-    auto k = k_lwe_keyswitch_staged<W, 256, KB>;
+    // a.nt ciphertexts share each staged key stream (one per thread); 1024
+    // threads only on 64-bit words (32-bit words need 16 columns of sums per
+    // thread, which spill at 128 registers)
+    const int nt = (a.nt == 1024 && sizeof(W) == 8) || a.nt == 512 ? a.nt : 256;
+    dim3 grid((a.n + 1 + KB - 1) / KB, (unsigned)((a.B + nt - 1) / nt));
+    const size_t lds = ks_staged_lds<W>(a.Bks, nt);
+    auto k = nt == 512 ? k_lwe_keyswitch_staged<W, 512, KB> : k_lwe_keyswitch_staged<W, 256, KB>;
+    if constexpr (sizeof(W) == 8)
+        if (nt == 1024) k = k_lwe_keyswitch_staged<W, 1024, KB>;
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, grid, dim3(256), lds, st, a);
+    hipLaunchKernelGGL(k, grid, dim3(nt), lds, st, a);
     return hipGetLastError();
 }
 
 template <typename W>
-size_t ks_staged_lds(uint32_t Bks) {
+size_t ks_staged_lds(uint32_t Bks, int nt) {
     constexpr int KB = sizeof(W) == 4 ? 16 : 8;
     const size_t fill = (size_t)KS_SB * Bks * (KB + 16 / sizeof(W)) * sizeof(W);
-    return ks_double_buffered<W, KB>(Bks, 256) ? 2 * fill : fill;
+    return ks_double_buffered<W, KB>(Bks, nt) ? 2 * fill : fill;
 }
 
 template <typename W>

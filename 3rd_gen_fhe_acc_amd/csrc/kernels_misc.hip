@@ -26,8 +26,8 @@ hipError_t launch_digest(const uint64_t *w, int64_t count, uint64_t *partials, i
     return hipGetLastError();
 }
 
-template size_t ks_staged_lds<uint32_t>(uint32_t);
-template size_t ks_staged_lds<uint64_t>(uint32_t);
+template size_t ks_staged_lds<uint32_t>(uint32_t, int);
+template size_t ks_staged_lds<uint64_t>(uint32_t, int);
 
 #define FHEACC_INSTANTIATE(W)                                                                                     \
     template hipError_t launch_lwe_keyswitch<W>(const KSArgs<W> &, bool, hipStream_t);                            \
