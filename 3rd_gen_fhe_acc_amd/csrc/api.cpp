@@ -62,7 +62,10 @@ Arith<W> make_arith(const HostParams &p) {
     A.Q2 = (W)(2 * p.Q);
     A.mu = (uint64_t)(((u128)1 << 64) / p.Q);
     if constexpr (sizeof(W) == 4) A.harvey = p.harvey;
-    else A.Q5 = 5 * p.Q;
+    else {
+        A.Q5 = 5 * p.Q;
+        A.NQ = 0 - p.Q;
+    }
     A.lzr = lazy_inverse_stages(p.Q, 8 * sizeof(W));
     return A;
 }

@@ -70,16 +70,20 @@ struct Arith<uint64_t> {
     W Q, Q2;
     uint64_t mu;  // unused for the wide path (sums are reduced from [0, 10dQ))
     W Q5;         // 5Q: the bound of mul_shoup_lazy
+    W NQ;         // 2^64 - Q: the remainder x w - qh Q is x w + qh NQ mod 2^64
     int lzr;      // as for 32-bit words: 2^(lzr + 1) Q <= 2^64
 
     __device__ __forceinline__ static W red(W x, W m) {
         W y = x - m;
         return y < x ? y : x;
     }
-    // x * w mod Q in [0, 2Q) for any 64-bit x; wp = floor(w * 2^64 / Q)
+    // x * w mod Q in [0, 2Q) for any 64-bit x; wp = floor(w * 2^64 / Q).
+    // The remainder is written x w + qh (2^64 - Q) so that the second low
+    // product accumulates into the first one's v_mad_u64_u32 (two VALU
+    // instructions fewer than a product and a 64-bit subtraction).
     __device__ __forceinline__ W mul_shoup(W x, W w, W wp) const {
         W qh = __umul64hi(x, wp);
-        return x * w - qh * Q;
+        return x * w + qh * NQ;
     }
     // x * w mod Q in [0, 5Q) for any 64-bit x.  The Shoup quotient
     // floor(x wp / 2^64) is taken from three 32-bit partial products: the
@@ -92,7 +96,7 @@ struct Arith<uint64_t> {
         const uint32_t pl = (uint32_t)wp, ph = (uint32_t)(wp >> 32);
         const uint64_t mid = (uint64_t)__umulhi(xl, ph) + __umulhi(xh, pl);
         const uint64_t qh = (uint64_t)xh * ph + mid;
-        return x * w - qh * Q;
+        return x * w + qh * NQ;
     }
     // (a, b) -> (a + wb, a - wb + 5Q), wb = b w mod Q in [0, 5Q): a word
     // entering stage s is below (5s + 1) Q
