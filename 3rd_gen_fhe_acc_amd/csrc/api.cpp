@@ -65,6 +65,7 @@ Arith<W> make_arith(const HostParams &p) {
     else {
         A.Q5 = 5 * p.Q;
         A.NQ = 0 - p.Q;
+        A.invQ = 1.0 / (double)p.Q;
     }
     A.lzr = lazy_inverse_stages(p.Q, 8 * sizeof(W));
     return A;

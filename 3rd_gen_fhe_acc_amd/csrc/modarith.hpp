@@ -71,6 +71,7 @@ struct Arith<uint64_t> {
     uint64_t mu;  // unused for the wide path (sums are reduced from [0, 10dQ))
     W Q5;         // 5Q: the bound of mul_shoup_lazy
     W NQ;         // 2^64 - Q: the remainder x w - qh Q is x w + qh NQ mod 2^64
+    double invQ;  // 1 / Q rounded to double, for reduce_small's quotient
     int lzr;      // as for 32-bit words: 2^(lzr + 1) Q <= 2^64
 
     __device__ __forceinline__ static W red(W x, W m) {
@@ -106,9 +107,14 @@ struct Arith<uint64_t> {
         a = a0 + v;
         b = a0 - v + Q5;
     }
-    // x < 2^62, any multiple of Q below that -> [0, Q)
+    // x < 2^62, any multiple of Q below that -> [0, Q).  The quotient
+    // x / Q < 2^8 comes from a double multiply by 1/Q: three roundings of
+    // relative size 2^-53 each move it by far less than one unit, so the
+    // truncated estimate is the true quotient or one off, and the remainder
+    // lies in [-Q, 2Q), which the corrections below bring to [0, Q).  (A
+    // double division instead costs a dozen VALU instructions.)
     __device__ __forceinline__ W reduce_small(uint64_t x) const {
-        uint64_t qh = (uint64_t)((double)x / (double)Q);
+        uint64_t qh = (uint64_t)((double)x * invQ);
         int64_t r = (int64_t)(x - qh * Q);
         if (r < 0) r += (int64_t)Q;
         if (r >= (int64_t)Q) r -= (int64_t)Q;
