@@ -149,8 +149,24 @@ __device__ __forceinline__ T ld_global(const void *p, size_t i) {
     return ((global_t *)p)[i];
 }
 
+// A pointer every lane holds the same value of, moved to scalar registers
+// (the compiler cannot see that a pointer read from memory is uniform).
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T *)(((uint64_t)hi << 32) | lo);
+}
+// The key base is uniform across the workgroup and the quad index fits 32
+// bits, so the load takes a scalar base and a 32-bit vector byte offset.
+template <typename T>
+__device__ __forceinline__ T ld_global_off(const void *p, uint32_t byte_off) {
+    typedef const __attribute__((address_space(1))) T global_t;
+    return *(global_t *)((const __attribute__((address_space(1))) char *)p + byte_off);
+}
 __device__ __forceinline__ KeyQuad<uint32_t> load_quad(const uint32_t *key, int idx) {
-    const u32x4 q = ld_global<u32x4>(key, (size_t)idx);
+    const u32x4 q = ld_global_off<u32x4>(key, (uint32_t)idx * 16u);
     KeyQuad<uint32_t> r;
     r.v[0] = q.x; r.v[1] = q.y; r.v[2] = q.z; r.v[3] = q.w;
     r.p[0] = r.p[1] = r.p[2] = r.p[3] = 0;
@@ -160,9 +176,42 @@ __device__ __forceinline__ KeyQuad<uint64_t> load_quad(const uint64_t *key, int 
     KeyQuad<uint64_t> r;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        const u64x2 t = ld_global<u64x2>(key, 4 * (size_t)idx + i);
+        const u64x2 t = ld_global_off<u64x2>(key, (uint32_t)idx * 64u + 16u * i);
         r.v[i] = t.x;
         r.p[i] = t.y;
+    }
+    return r;
+}
+// W words of one key quad (with companions when wide): the row h of an RGSW
+// starts at key + (h << LOGN) * kQuadWords<W>
+template <typename W>
+constexpr int kQuadWords = sizeof(W) == 4 ? 4 : 8;
+
+// Key quads through a buffer resource over one RGSW: the row's byte offset
+// goes in the scalar offset and the thread's in a loop-invariant vector
+// offset, so a load costs no VALU address arithmetic.  The range check is
+// off in effect (num_records 2^32 - 1; every offset stays inside the RGSW).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t key_rsrc(const void *key) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(key), 0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ KeyQuad<uint32_t> load_quad_rs(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff,
+                                                          const uint32_t *) {
+    typedef int i32x4 __attribute__((ext_vector_type(4)));
+    const i32x4 q = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+    KeyQuad<uint32_t> r;
+    r.v[0] = (uint32_t)q.x; r.v[1] = (uint32_t)q.y; r.v[2] = (uint32_t)q.z; r.v[3] = (uint32_t)q.w;
+    r.p[0] = r.p[1] = r.p[2] = r.p[3] = 0;
+    return r;
+}
+__device__ __forceinline__ KeyQuad<uint64_t> load_quad_rs(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff,
+                                                          const uint64_t *) {
+    typedef int i32x4 __attribute__((ext_vector_type(4)));
+    KeyQuad<uint64_t> r;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const i32x4 t = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + 16u * i, soff, 0);
+        r.v[i] = (uint64_t)(uint32_t)t.x | ((uint64_t)(uint32_t)t.y << 32);
+        r.p[i] = (uint64_t)(uint32_t)t.z | ((uint64_t)(uint32_t)t.w << 32);
     }
     return r;
 }
@@ -226,7 +275,8 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
             };
             auto fetch = [&](KeyQuad<W> *q, int hp) {
 #pragma unroll
-                for (int c = 0; c < CPT; c++) q[c] = load_quad(key, ((h0 + hp) << LOGN) + tid + c * NT);
+                for (int c = 0; c < CPT; c++)
+                    q[c] = load_quad(key + ((size_t)(h0 + hp) << LOGN) * kQuadWords<W>, tid + c * NT);
             };
             if constexpr (kLean) {
 #pragma unroll 1
@@ -357,10 +407,12 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     MacSum<W> sa[CPT], sb[CPT];
     KeyQuad<W> cur[CPT], nxt[CPT];
     const W *dcol[CPT];
+    constexpr uint32_t QB = kQuadWords<W> * sizeof(W);  // bytes per key quad
+    const __amdgpu_buffer_rsrc_t krs = key_rsrc(uniform_ptr(key));
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
         dcol[c] = buf + pad(tid + c * NT);
-        if (kFull || tid + c * NT < N) cur[c] = load_quad(key, tid + c * NT);
+        if (kFull || tid + c * NT < N) cur[c] = load_quad_rs(krs, (uint32_t)(tid + c * NT) * QB, 0u, key);
     }
     auto row = [&](const KeyQuad<W> *q) {
 #pragma unroll
@@ -378,7 +430,8 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     auto fetch = [&](KeyQuad<W> *q, int h) {
 #pragma unroll
         for (int c = 0; c < CPT; c++)
-            if (kFull || tid + c * NT < N) q[c] = load_quad(key, (h << LOGN) + tid + c * NT);
+            if (kFull || tid + c * NT < N)
+                q[c] = load_quad_rs(krs, (uint32_t)(tid + c * NT) * QB, ((uint32_t)h << LOGN) * QB, key);
     };
     if constexpr (kLean) {
         // one register set of key quads (two would not fit in 128 VGPRs);
@@ -419,7 +472,7 @@ struct KeyPair {
 };
 __device__ __forceinline__ KeyPair<uint32_t> load_pair(const uint32_t *key, int idx, int half) {
     typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-    const u32x2 q = ld_global<u32x2>(key, 2 * (size_t)idx + half);
+    const u32x2 q = ld_global_off<u32x2>(key, (uint32_t)idx * 16u + 8u * half);
     KeyPair<uint32_t> r;
     r.v[0] = q.x; r.v[1] = q.y;
     r.p[0] = r.p[1] = 0;
@@ -429,7 +482,7 @@ __device__ __forceinline__ KeyPair<uint64_t> load_pair(const uint64_t *key, int 
     KeyPair<uint64_t> r;
 #pragma unroll
     for (int i = 0; i < 2; i++) {
-        const u64x2 t = ld_global<u64x2>(key, 4 * (size_t)idx + 2 * half + i);
+        const u64x2 t = ld_global_off<u64x2>(key, (uint32_t)idx * 64u + 32u * half + 16u * i);
         r.v[i] = t.x;
         r.p[i] = t.y;
     }
@@ -514,7 +567,8 @@ __device__ void ext_prod_split(W *buf, const W *key, const Arith<W> &A, const Tw
         auto fetch = [&](KeyPair<W> *q, int h) {
 #pragma unroll
             for (int c = 0; c < CPT; c++)
-                if (kFull || tid + c * NT < N) q[c] = load_pair(key, (h << LOGN) + tid + c * NT, half);
+                if (kFull || tid + c * NT < N)
+                    q[c] = load_pair(key + ((size_t)h << LOGN) * kQuadWords<W>, tid + c * NT, half);
         };
         if constexpr (sizeof(W) == 8) {
             // 64-bit words: each coefficient's key words are loaded just
@@ -525,7 +579,7 @@ __device__ void ext_prod_split(W *buf, const W *key, const Arith<W> &A, const Tw
 #pragma unroll
                 for (int c = 0; c < CPT; c++) {
                     if (kFull || tid + c * NT < N) {
-                        const KeyPair<W> q = load_pair(key, (h << LOGN) + tid + c * NT, half);
+                        const KeyPair<W> q = load_pair(key + ((size_t)h << LOGN) * kQuadWords<W>, tid + c * NT, half);
                         const W dg = dcol[c][0];
                         sa[c].add(A, dg, q.v[0], q.p[0]);
                         sb[c].add(A, dg, q.v[1], q.p[1]);
@@ -675,10 +729,12 @@ __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, S
     // step (0, 0) whatever the residue, its batched forms at every step
     // (myfpga.cpp:1030-1071, 1180-1225), with btkey[0] = RGSW(1).  A skipped
     // step (and every step of a flagged ciphertext) still counts for pacing.
-    const W *bsk = a.key_of ? a.bsk_tab[a.key_of[c]] : a.bsk;
+    const W *bsk = uniform_ptr(a.key_of ? a.bsk_tab[a.key_of[c]] : a.bsk);
     const int mode = a.mode;
     for (int i = 0, t = 0; i < n; i++) {
-        uint32_t nega = (q - av[i]) & qm;
+        // uniform across the workgroup: the key address of each step is
+        // scalar, and its loads take a scalar base
+        uint32_t nega = __builtin_amdgcn_readfirstlane((q - av[i]) & qm);
         for (int j = 0; j < a.digitR; j++, t++) {
             const uint32_t r = nega % a.Br;
             nega /= a.Br;
