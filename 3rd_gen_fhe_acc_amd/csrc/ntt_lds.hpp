@@ -157,9 +157,10 @@ __host__ __device__ constexpr int lazy_run(int e, int k) {
     return z;
 }
 
-// LZ: the lazy form (needs 2^(R+1) Q within the word); otherwise every sum is
-// reduced to [0, 2Q) at once.
-template <int S, int R, bool LAST, bool LZ, typename W>
+// LZ = 1: the lazy form (needs 2^(R+1) Q within the word); LZ = 0: every sum
+// is reduced to [0, 2Q) at once; LZ = 2 (64-bit words): the lazy form with
+// approximate-quotient products, below.
+template <int S, int R, bool LAST, int LZ, typename W>
 __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, const Twiddles<W> &tw,
                                             const ClusterTwiddles<R, W> &t) {
     W *base = s + pad(b0);
@@ -167,7 +168,54 @@ __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, con
 #pragma unroll
     for (int e = 0; e < (1 << R); e++) x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
     int o = 0;
-    if constexpr (LZ) {
+    if constexpr (LZ == 2 && sizeof(W) == 8) {
+        // Products by mul_shoup_lazy land in [0, 5Q) (two 32-bit high
+        // products and one multiply-add for the quotient, not a full 64x64
+        // high product).  Inputs are below 5Q, so the bounds of the lazy form
+        // hold with 5Q in place of 2Q: element e after stage k is below
+        // 5Q 2^z, z = lazy_run(e, k), and so below 5Q 2^R <= 80Q, which the
+        // context's parameter check keeps within 64 bits.  The pass hands the next one
+        // words below 5Q; the transform's last stage multiplies exactly and
+        // ends canonical.
+        constexpr int KS = LAST ? R - 1 : R;
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            const int h = 1 << k;
+#pragma unroll
+            for (int e = 0; e < (1 << R); e++) {
+                if (e & h) continue;
+                const int j = o + (e >> (k + 1));
+                const W c = A.Q5 << lazy_run(e, k - 1);  // u, v < c
+                const W u = x[e], v = x[e + h];
+                x[e] = u + v;
+                x[e + h] = A.mul_shoup_lazy(u - v + c, t.w[j], t.wp[j]);
+            }
+            o += 1 << (R - k - 1);
+        }
+        if constexpr (LAST) {
+            constexpr int h = 1 << (R - 1);
+#pragma unroll
+            for (int e = 0; e < h; e++) {
+                const W c = A.Q5 << lazy_run(e, R - 2);
+                const W u = x[e], v = x[e + h];
+                x[e] = Arith<W>::red(A.mul_shoup(u + v, tw.iN, tw.iNp), A.Q);
+                x[e + h] = Arith<W>::red(A.mul_shoup(u - v + c, tw.iNw, tw.iNwp), A.Q);
+            }
+        } else {
+            // back below 5Q for the next pass: one conditional subtraction
+            // from [0, 10Q), a lazy product by 1 above that
+#pragma unroll
+            for (int e = 0; e < (1 << R); e++) {
+                const int z = lazy_run(e, R - 1);
+                if (z == 1) x[e] = Arith<W>::red(x[e], A.Q5);
+                else if (z > 1) x[e] = A.mul_shoup_lazy(x[e], (W)1, tw.onep);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
+        return;
+    }
+    if constexpr (LZ == 1) {
         constexpr int KS = LAST ? R - 1 : R;
 #pragma unroll
         for (int k = 0; k < KS; k++) {
@@ -267,7 +315,7 @@ __device__ __forceinline__ void fwd_pass(W *s, int np, bool last, const Arith<W>
 
 // Inverse pass: GS stages U0 .. U0+R-1 (half-size 2^u).  Inputs and outputs in
 // [0, 2Q).  When last, the words are scaled by N^-1 and made canonical.
-template <int LOGN, int U0, int R, int NT, bool LZ, typename W>
+template <int LOGN, int U0, int R, int NT, int LZ, typename W>
 __device__ __forceinline__ void inv_pass(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid) {
     constexpr int N = 1 << LOGN;
     constexpr int S = 1 << U0;
@@ -301,14 +349,18 @@ __device__ __forceinline__ void fwd_passes(W *s, int np, bool canonical, const A
     }
 }
 
-template <int LOGN, int ST, int NT, int MAXR, typename W>
+// LP: every pass in the lazy-product form (its 5Q bound between passes holds
+// only when all passes use it); otherwise each pass is lazy or strict by its
+// own stage count.
+template <int LOGN, int ST, int NT, int MAXR, bool LP, typename W>
 __device__ __forceinline__ void inv_passes(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid) {
     if constexpr (ST < LOGN) {
         constexpr int R = last_pass_stages<MAXR>(LOGN - ST);
-        if (R <= A.lzr) inv_pass<LOGN, ST, R, NT, true>(s, np, A, tw, tid);
-        else inv_pass<LOGN, ST, R, NT, false>(s, np, A, tw, tid);
+        if constexpr (LP) inv_pass<LOGN, ST, R, NT, 2>(s, np, A, tw, tid);
+        else if (R <= A.lzr) inv_pass<LOGN, ST, R, NT, 1>(s, np, A, tw, tid);
+        else inv_pass<LOGN, ST, R, NT, 0>(s, np, A, tw, tid);
         __syncthreads();
-        inv_passes<LOGN, ST + R, NT, MAXR>(s, np, A, tw, tid);
+        inv_passes<LOGN, ST + R, NT, MAXR, LP>(s, np, A, tw, tid);
     }
 }
 
@@ -333,7 +385,11 @@ __device__ void ntt_forward(W *s, int np, const Arith<W> &A, const Twiddles<W> &
 // canonical in [0, Q).
 template <int LOGN, int NT, int MAXR = 4, typename W>
 __device__ void ntt_inverse(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid = -1) {
-    inv_passes<LOGN, 0, NT, MAXR>(s, np, A, tw, tid < 0 ? (int)threadIdx.x : tid);
+    const int t = tid < 0 ? (int)threadIdx.x : tid;
+    // 64-bit words: every pass takes lazy products.  Words stay below
+    // 5Q 2^R <= 80Q, which the context's parameter check keeps within 64 bits.
+    static_assert(sizeof(W) == 4 || MAXR <= 4, "lazy-product inverse passes need 80Q < 2^64");
+    inv_passes<LOGN, 0, NT, MAXR, sizeof(W) == 8>(s, np, A, tw, t);
 }
 
 }  // namespace fheacc

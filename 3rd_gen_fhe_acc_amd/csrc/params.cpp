@@ -170,6 +170,9 @@ int derive_params(const fheacc_params *in, HostParams *p, std::string *err) {
         p->narrow = true;
         p->harvey = true;
     }
+    // 64-bit inverse passes of up to 4 stages take lazy products (below 5Q)
+    // and leave their sums unreduced: every word stays below 80Q
+    if (!p->narrow && 80 * (u128)Q >= two64) return bad("Q too large for lazy 64-bit inverse passes");
     return 0;
 }
 
