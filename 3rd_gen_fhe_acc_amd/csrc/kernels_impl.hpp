@@ -233,7 +233,7 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
     // keep within 128 registers
     constexpr bool kLean = sizeof(W) == 8 && NT >= 1024;
     constexpr int MAXR = kLean ? 3 : 4;
-    ntt_inverse<LOGN, NT, MAXR>(acc, 2, A, tw, tid);
+    ntt_inverse<LOGN, NT, MAXR, 2>(acc, 2, A, tw, tid);
     MacSum<W> sa[CPT], sb[CPT];
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
     const int total = 2 * d;
@@ -345,9 +345,12 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
 // registers before the digits overwrite them, and after the transforms it
 // reads all 2d digits of k before writing the new a[k], b[k] -- slots no
 // other thread touches -- so the accumulator needs no LDS of its own.
-template <typename W, int LOGN, int NT>
-__device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const Twiddles<W> &tw, int d,
+// D > 0: the digit count d is the compile-time D (the transforms, the digit
+// split and the MAC unroll over the polys); D = 0: any d.
+template <typename W, int LOGN, int NT, int D = 0>
+__device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const Twiddles<W> &tw, int d_rt,
                                  int bits) {
+    const int d = D > 0 ? D : d_rt;
     constexpr int N = 1 << LOGN;
     constexpr int CPT = (N + NT - 1) / NT;
     int tid = threadIdx.x;
@@ -359,7 +362,7 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     // twiddles) and one set of key quads keep within 128 VGPRs
     constexpr bool kLean = sizeof(W) == 8 && NT >= 1024;
     constexpr int MAXR = kLean ? 3 : 4;
-    ntt_inverse<LOGN, NT, MAXR>(buf, 2, A, tw, tid);
+    ntt_inverse<LOGN, NT, MAXR, 2>(buf, 2, A, tw, tid);
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
     W xa[CPT], xb[CPT];
 #pragma unroll
@@ -398,7 +401,7 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
         }
     }
     __syncthreads();
-    ntt_forward<LOGN, NT, MAXR>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
+    ntt_forward<LOGN, NT, MAXR, 2 * D>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
     // gadget rows outer, the thread's coefficients inner: row h+1's key quads
     // are in flight while row h is accumulated.  Rows alternate between two
     // register sets (no copies; one set when kLean), and each coefficient's
@@ -440,6 +443,16 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
         for (int h = 0; h < d; h++) {
             if (h) fetch(cur, h);
             row(cur);
+        }
+    } else if constexpr (D > 0) {
+#pragma unroll
+        for (int h = 0; h < D; h += 2) {
+            if (h + 1 < D) fetch(nxt, h + 1);
+            row(cur);
+            if (h + 1 < D) {
+                if (h + 2 < D) fetch(cur, h + 2);
+                row(nxt);
+            }
         }
     } else {
 #pragma unroll 1
@@ -509,7 +522,7 @@ __device__ void ext_prod_split(W *buf, const W *key, const Arith<W> &A, const Tw
     constexpr int MAXR = sizeof(W) == 8 ? 3 : 4;
     constexpr int MAXRF = sizeof(W) == 8 ? 2 : 4;
     constexpr int PS = N + N / 8;  // padded words per poly
-    ntt_inverse<LOGN, NT, MAXR>(buf, 2, A, tw, tid);
+    ntt_inverse<LOGN, NT, MAXR, 2>(buf, 2, A, tw, tid);
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
     W xa[CPT], xb[CPT];
 #pragma unroll
@@ -671,7 +684,7 @@ struct StepPacer {
 // One gate bootstrap's blind rotation and key-switch preparation on the
 // workgroup's LDS: ciphertext c, thread index tid (an opaque copy in the
 // persistent kernel), steps counted from step0 for pacing.
-template <typename W, int LOGN, int NT, int FORM, bool PACE>
+template <typename W, int LOGN, int NT, int FORM, bool PACE, int D = 0>
 __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, StepPacer &pace, uint32_t step0) {
     constexpr int N = 1 << LOGN;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -723,7 +736,7 @@ __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, S
         acc[pad(N + i * ef)] = (W)(neg ? nQ8 : Q8);
     }
     __syncthreads();
-    ntt_forward<LOGN, NT>(acc + pad(N), 1, a.A, a.tw, true, tid);
+    ntt_forward<LOGN, NT, 4, 1>(acc + pad(N), 1, a.A, a.tw, true, tid);
     // blind rotation: i outer, j inner.  Software order (RLWE.cpp:754-767)
     // skips residue 0; the accelerator's one-ciphertext form multiplies at
     // step (0, 0) whatever the residue, its batched forms at every step
@@ -741,7 +754,7 @@ __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, S
             if constexpr (PACE) pace.wait(step0 + t);
             if (!bad_gate && (r || mode == FHEACC_EVAL_FPGA_XN || (mode == FHEACC_EVAL_FPGA_X1 && i == 0 && j == 0))) {
                 const W *key = bsk + ((size_t)(i * a.digitR + j) * a.Br + r) * a.rgsw_elems;
-                if constexpr (FORM == kInPlace) ext_prod_inplace<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
+                if constexpr (FORM == kInPlace) ext_prod_inplace<W, LOGN, NT, D>(acc, key, a.A, a.tw, a.d, a.bits);
                 else if constexpr (FORM == kSplit) ext_prod_split<W, LOGN, NT>(acc, key, a.A, a.tw, a.d, a.bits);
                 else ext_prod<W, LOGN, NT>(acc, dig, key, a.A, a.tw, a.d, a.bits, a.pb);
             }
@@ -761,7 +774,7 @@ __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, S
         acc[pad(N - 1 - idx)] = t;
     }
     __syncthreads();
-    ntt_inverse<LOGN, NT>(acc, 2, a.A, a.tw, tid);
+    ntt_inverse<LOGN, NT, 4, 2>(acc, 2, a.A, a.tw, tid);
     const int dks = a.dks;
     const uint32_t Bks = a.Bks;
     uint8_t *dst = a.ksd + c * (size_t)N * dks;
@@ -789,10 +802,23 @@ constexpr int product_waves_per_eu() {
     return (FORM == kSplit && sizeof(W) == 8 && NT == 512) ? 4 : 1;
 }
 
-template <typename W, int LOGN, int NT, int FORM>
+// D > 0: a kernel for digit count D alone (launch_br_nt picks it when a.d == D)
+template <typename W, int LOGN, int NT, int FORM, int D = 0>
 __global__ __launch_bounds__(NT, (product_waves_per_eu<W, NT, FORM>())) void k_blind_rotate(BRArgs<W> a) {
     StepPacer none;
-    br_one<W, LOGN, NT, FORM, false>(a, a.c_base + blockIdx.x, threadIdx.x, none, 0);
+    br_one<W, LOGN, NT, FORM, false, D>(a, a.c_base + blockIdx.x, threadIdx.x, none, 0);
+}
+
+// The digit counts that get a kernel of their own at a launch shape: the
+// shapes the autotuner picks for the benchmark set (27-bit Q, N = 1024,
+// d = 4), the mixed batch's 54-bit set (N = 2048, d = 3) and its N = 512 set
+// (d = 5).  Every other (shape, d) runs the general kernel.
+template <typename W, int LOGN, int NT>
+constexpr int specialized_digits() {
+    if constexpr (sizeof(W) == 4 && LOGN == 10 && NT == 256) return 4;
+    if constexpr (sizeof(W) == 8 && LOGN == 11 && NT == 1024) return 3;
+    if constexpr (sizeof(W) == 4 && LOGN == 9 && NT == 256) return 5;
+    return 0;
 }
 
 // Persistent and paced: a resident grid of G workgroups, workgroup w taking
@@ -1715,6 +1741,9 @@ static hipError_t launch_br_nt(const BRArgs<W> &a, int64_t B, size_t lds, hipStr
     auto k = form == kInPlace ? k_blind_rotate<W, LOGN, NT, kInPlace>
                               : (form == kSplit ? k_blind_rotate<W, LOGN, NT, kSplit>
                                                 : k_blind_rotate<W, LOGN, NT, kSeparate>);
+    constexpr int kSpecD = specialized_digits<W, LOGN, NT>();
+    if constexpr (kSpecD > 0)
+        if (form == kInPlace && a.d == kSpecD) k = k_blind_rotate<W, LOGN, NT, kInPlace, kSpecD>;
     if constexpr (kPaceable) {
         if (persist > 0 && cus > 0 && a.sync) {
             auto kp = form == kInPlace ? k_blind_rotate_paced<W, LOGN, NT, kInPlace>

@@ -288,7 +288,7 @@ __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, con
 // When last, outputs are made canonical.  When NT is a multiple of the
 // clusters per poly, a thread meets the same in-poly cluster in every poly
 // it handles, and loads that cluster's twiddles once.
-template <int LOGN, int S0, int R, int NT, bool HV, typename W>
+template <int LOGN, int S0, int R, int NT, bool HV, int NP, typename W>
 __device__ __forceinline__ void fwd_pass(W *s, int np, bool last, const Arith<W> &A, const Twiddles<W> &tw,
                                          int tid) {
     constexpr int N = 1 << LOGN;
@@ -300,8 +300,23 @@ __device__ __forceinline__ void fwd_pass(W *s, int np, bool last, const Arith<W>
         const int c = tid % per_poly;
         const int blk = c >> lS, off = c & (S - 1);
         const ClusterTwiddles<R, W> t = fwd_twiddles<S0, R>(blk, tw);
-        for (int p = tid / per_poly; p < np; p += NT / per_poly)
-            fwd_cluster<S, R, HV, FINAL>(s, p * N + (blk << (lS + R)) + off, last, A, tw, t);
+        // a trip count every thread shares (np is uniform), so the loop is
+        // a scalar one; the guard only masks lanes when G does not divide np
+        constexpr int G = NT / per_poly;
+        constexpr int PSW = N + N / 8;  // padded words per poly (pad(p N + i) = p PSW + pad(i))
+        const int p0 = tid / per_poly;
+        if constexpr (NP > 0) {
+            // a poly count known at compile time: unrolled, constant offsets
+#pragma unroll
+            for (int it = 0; it < (NP + G - 1) / G; it++)
+                if (NP % G == 0 || p0 + it * G < NP)
+                    fwd_cluster<S, R, HV, FINAL>(s + (p0 + it * G) * PSW, (blk << (lS + R)) + off, last, A, tw, t);
+        } else {
+            const int iters = (np + G - 1) / G;
+            W *sp = s + p0 * PSW;
+            for (int it = 0; it < iters; it++, sp += G * PSW)
+                if (p0 + it * G < np) fwd_cluster<S, R, HV, FINAL>(sp, (blk << (lS + R)) + off, last, A, tw, t);
+        }
     } else {
         const int clusters = np * per_poly;
         for (int cg = tid; cg < clusters; cg += NT) {
@@ -315,7 +330,7 @@ __device__ __forceinline__ void fwd_pass(W *s, int np, bool last, const Arith<W>
 
 // Inverse pass: GS stages U0 .. U0+R-1 (half-size 2^u).  Inputs and outputs in
 // [0, 2Q).  When last, the words are scaled by N^-1 and made canonical.
-template <int LOGN, int U0, int R, int NT, int LZ, typename W>
+template <int LOGN, int U0, int R, int NT, int LZ, int NP, typename W>
 __device__ __forceinline__ void inv_pass(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid) {
     constexpr int N = 1 << LOGN;
     constexpr int S = 1 << U0;
@@ -325,8 +340,20 @@ __device__ __forceinline__ void inv_pass(W *s, int np, const Arith<W> &A, const 
         const int c = tid % per_poly;
         const int blk = c >> U0, off = c & (S - 1);
         const ClusterTwiddles<R, W> t = inv_twiddles<LOGN, U0, R>(blk, tw);
-        for (int p = tid / per_poly; p < np; p += NT / per_poly)
-            inv_cluster<S, R, last, LZ>(s, p * N + (blk << (U0 + R)) + off, A, tw, t);
+        constexpr int G = NT / per_poly;
+        constexpr int PSW = N + N / 8;
+        const int p0 = tid / per_poly;
+        if constexpr (NP > 0) {
+#pragma unroll
+            for (int it = 0; it < (NP + G - 1) / G; it++)
+                if (NP % G == 0 || p0 + it * G < NP)
+                    inv_cluster<S, R, last, LZ>(s + (p0 + it * G) * PSW, (blk << (U0 + R)) + off, A, tw, t);
+        } else {
+            const int iters = (np + G - 1) / G;
+            W *sp = s + p0 * PSW;
+            for (int it = 0; it < iters; it++, sp += G * PSW)
+                if (p0 + it * G < np) inv_cluster<S, R, last, LZ>(sp, (blk << (U0 + R)) + off, A, tw, t);
+        }
     } else {
         const int clusters = np * per_poly;
         for (int cg = tid; cg < clusters; cg += NT) {
@@ -338,29 +365,29 @@ __device__ __forceinline__ void inv_pass(W *s, int np, const Arith<W> &A, const 
     }
 }
 
-template <int LOGN, int ST, int NT, int MAXR, bool HV, typename W>
+template <int LOGN, int ST, int NT, int MAXR, bool HV, int NP, typename W>
 __device__ __forceinline__ void fwd_passes(W *s, int np, bool canonical, const Arith<W> &A, const Twiddles<W> &tw,
                                            int tid) {
     if constexpr (ST < LOGN) {
         constexpr int R = pass_stages<MAXR>(LOGN - ST);
-        fwd_pass<LOGN, ST, R, NT, HV>(s, np, canonical && ST + R == LOGN, A, tw, tid);
+        fwd_pass<LOGN, ST, R, NT, HV, NP>(s, np, canonical && ST + R == LOGN, A, tw, tid);
         __syncthreads();
-        fwd_passes<LOGN, ST + R, NT, MAXR, HV>(s, np, canonical, A, tw, tid);
+        fwd_passes<LOGN, ST + R, NT, MAXR, HV, NP>(s, np, canonical, A, tw, tid);
     }
 }
 
 // LP: every pass in the lazy-product form (its 5Q bound between passes holds
 // only when all passes use it); otherwise each pass is lazy or strict by its
 // own stage count.
-template <int LOGN, int ST, int NT, int MAXR, bool LP, typename W>
+template <int LOGN, int ST, int NT, int MAXR, bool LP, int NP, typename W>
 __device__ __forceinline__ void inv_passes(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid) {
     if constexpr (ST < LOGN) {
         constexpr int R = last_pass_stages<MAXR>(LOGN - ST);
-        if constexpr (LP) inv_pass<LOGN, ST, R, NT, 2>(s, np, A, tw, tid);
-        else if (R <= A.lzr) inv_pass<LOGN, ST, R, NT, 1>(s, np, A, tw, tid);
-        else inv_pass<LOGN, ST, R, NT, 0>(s, np, A, tw, tid);
+        if constexpr (LP) inv_pass<LOGN, ST, R, NT, 2, NP>(s, np, A, tw, tid);
+        else if (R <= A.lzr) inv_pass<LOGN, ST, R, NT, 1, NP>(s, np, A, tw, tid);
+        else inv_pass<LOGN, ST, R, NT, 0, NP>(s, np, A, tw, tid);
         __syncthreads();
-        inv_passes<LOGN, ST + R, NT, MAXR, LP>(s, np, A, tw, tid);
+        inv_passes<LOGN, ST + R, NT, MAXR, LP, NP>(s, np, A, tw, tid);
     }
 }
 
@@ -368,28 +395,29 @@ __device__ __forceinline__ void inv_passes(W *s, int np, const Arith<W> &A, cons
 // canonical in [0, Q), or, when canonical is false, below (2 logN + 1) Q
 // (below 2Q with Harvey's butterflies, below (5 logN + 1) Q on 64-bit words)
 // and congruent to the canonical word (for consumers that reduce anyway).
-template <int LOGN, int NT, int MAXR = 4, typename W>
+// NP > 0: np is the compile-time constant NP (the passes unroll over the polys)
+template <int LOGN, int NT, int MAXR = 4, int NP = 0, typename W>
 __device__ void ntt_forward(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, bool canonical = true,
                             int tid = -1) {
     const int t = tid < 0 ? (int)threadIdx.x : tid;
     if constexpr (sizeof(W) == 4) {
         if (A.harvey) {
-            fwd_passes<LOGN, 0, NT, MAXR, true>(s, np, canonical, A, tw, t);
+            fwd_passes<LOGN, 0, NT, MAXR, true, NP>(s, np, canonical, A, tw, t);
             return;
         }
     }
-    fwd_passes<LOGN, 0, NT, MAXR, false>(s, np, canonical, A, tw, t);
+    fwd_passes<LOGN, 0, NT, MAXR, false, NP>(s, np, canonical, A, tw, t);
 }
 
 // Inverse transform of np padded polys in LDS (inputs in [0, 2Q)); output
 // canonical in [0, Q).
-template <int LOGN, int NT, int MAXR = 4, typename W>
+template <int LOGN, int NT, int MAXR = 4, int NP = 0, typename W>
 __device__ void ntt_inverse(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid = -1) {
     const int t = tid < 0 ? (int)threadIdx.x : tid;
     // 64-bit words: every pass takes lazy products.  Words stay below
     // 5Q 2^R <= 80Q, which the context's parameter check keeps within 64 bits.
     static_assert(sizeof(W) == 4 || MAXR <= 4, "lazy-product inverse passes need 80Q < 2^64");
-    inv_passes<LOGN, 0, NT, MAXR, sizeof(W) == 8>(s, np, A, tw, t);
+    inv_passes<LOGN, 0, NT, MAXR, sizeof(W) == 8, NP>(s, np, A, tw, t);
 }
 
 }  // namespace fheacc
