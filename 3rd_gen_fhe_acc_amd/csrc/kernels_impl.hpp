@@ -811,11 +811,12 @@ __global__ __launch_bounds__(NT, (product_waves_per_eu<W, NT, FORM>())) void k_b
 
 // The digit counts that get a kernel of their own at a launch shape: the
 // shapes the autotuner picks for the benchmark set (27-bit Q, N = 1024,
-// d = 4), the mixed batch's 54-bit set (N = 2048, d = 3) and its N = 512 set
-// (d = 5).  Every other (shape, d) runs the general kernel.
+// d = 4: 256 threads for batches, 1024 for a single gate), the mixed batch's
+// 54-bit set (N = 2048, d = 3) and its N = 512 set (d = 5).  Every other
+// (shape, d) runs the general kernel.
 template <typename W, int LOGN, int NT>
 constexpr int specialized_digits() {
-    if constexpr (sizeof(W) == 4 && LOGN == 10 && NT == 256) return 4;
+    if constexpr (sizeof(W) == 4 && LOGN == 10 && (NT == 256 || NT == 1024)) return 4;
     if constexpr (sizeof(W) == 8 && LOGN == 11 && NT == 1024) return 3;
     if constexpr (sizeof(W) == 4 && LOGN == 9 && NT == 256) return 5;
     return 0;
@@ -1115,7 +1116,7 @@ __global__ __launch_bounds__(NT) void k_ntt(NTTArgs<W> a) {
 // INPLACE (pb >= 2d): the product runs in the 2d digit polys, the RLWE in
 // the first two of them, as in the blind rotation; otherwise acc (2 polys)
 // plus a pb-poly digit buffer.
-template <typename W, int LOGN, int NT, bool INPLACE>
+template <typename W, int LOGN, int NT, bool INPLACE, int D = 0>
 __global__ __launch_bounds__(NT) void k_external_product(EPArgs<W> a) {
     constexpr int N = 1 << LOGN;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1125,7 +1126,7 @@ __global__ __launch_bounds__(NT) void k_external_product(EPArgs<W> a) {
     const uint64_t *src = a.acc + c * a.in_stride;
     for (int i = threadIdx.x; i < 2 * N; i += NT) acc[pad(i)] = (W)src[i];
     __syncthreads();
-    if constexpr (INPLACE) ext_prod_inplace<W, LOGN, NT>(acc, a.key + c * a.key_stride, a.A, a.tw, a.d, a.bits);
+    if constexpr (INPLACE) ext_prod_inplace<W, LOGN, NT, D>(acc, a.key + c * a.key_stride, a.A, a.tw, a.d, a.bits);
     else ext_prod<W, LOGN, NT>(acc, dig, a.key + c * a.key_stride, a.A, a.tw, a.d, a.bits, a.pb);
     uint64_t *dst = a.out + c * a.out_stride;
     for (int i = threadIdx.x; i < 2 * N; i += NT) dst[i] = acc[pad(i)];
@@ -1854,6 +1855,10 @@ hipError_t launch_ntt(const NTTArgs<W> &a, int logN, hipStream_t st) {
 template <typename W, int LOGN, int NT, bool INPLACE>
 static hipError_t launch_ep_nt(const EPArgs<W> &a, int64_t count, size_t lds, hipStream_t st) {
     auto k = k_external_product<W, LOGN, NT, INPLACE>;
+    // the standalone product's own digit-count kernel: 54-bit N = 1024, d = 6
+    // (BG = 2^9), the external-product microbenchmark's shape
+    if constexpr (INPLACE && sizeof(W) == 8 && LOGN == 10 && NT == 1024)
+        if (a.d == 6) k = k_external_product<W, LOGN, NT, INPLACE, 6>;
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     for (int64_t off = 0; off < count; off += max_wgs_per_launch<NT>()) {
