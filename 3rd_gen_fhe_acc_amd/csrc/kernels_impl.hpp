@@ -25,6 +25,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "fheacc_internal.hpp"
 #include "kernels.hpp"
@@ -362,7 +363,13 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     // twiddles) and one set of key quads keep within 128 VGPRs
     constexpr bool kLean = sizeof(W) == 8 && NT >= 1024;
     constexpr int MAXR = kLean ? 3 : 4;
-    ntt_inverse<LOGN, NT, MAXR, 2>(buf, 2, A, tw, tid);
+    // With N = 2^F NT a thread's words tid + c NT (stride NT = N / 2^F) are
+    // the cluster of the forward transform's first F stages and of the
+    // inverse's last F: those stages run in registers, between the digit
+    // split and the LDS, when that saves an LDS pass of each transform
+    constexpr int F = !kFull ? 0 : CPT >= 8 ? 3 : CPT >= 4 ? 2 : CPT >= 2 ? 1 : 0;
+    constexpr bool kFuse = F > 0 && (1 << F) == CPT && pass_count<MAXR>(LOGN - F) < pass_count<MAXR>(LOGN);
+    ntt_inverse<LOGN, NT, MAXR, 2, kFuse ? LOGN - F : LOGN>(buf, 2, A, tw, tid);
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
     W xa[CPT], xb[CPT];
 #pragma unroll
@@ -373,35 +380,82 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
             xb[c] = buf[pad(N + k)];
         }
     }
-    __syncthreads();
     // digit h of a goes to poly 2h, of b to poly 2h + 1: a poly is a multiple
     // of 8 words, so pad(2hN + k) = pad(k) + 2h padded-poly strides, and the
-    // unrolled writes become immediate LDS offsets
+    // unrolled writes become immediate LDS offsets.  A thread writes only the
+    // columns it has just read, so no barrier separates the two.
     constexpr int PS = N + N / 8;  // padded words per poly
     constexpr int DU = 8;          // digit count unrolled (larger d loops)
+    if constexpr (kFuse) {
+        const ClusterTwiddles<F, W> ti = inv_twiddles<LOGN, LOGN - F, F>(0, tw);
+        {
+            Words<F, W> ya, yb;
 #pragma unroll
-    for (int c = 0; c < CPT; c++) {
-        const int k = tid + c * NT;
-        if (kFull || k < N) {
-            W *col = buf + pad(k);
-            if (d <= DU) {
+            for (int c = 0; c < CPT; c++) ya.x[c] = xa[c], yb.x[c] = xb[c];
+            ya = inv_tail_regs<F>(ya, A, tw, ti);
+            yb = inv_tail_regs<F>(yb, A, tw, ti);
 #pragma unroll
-                for (int h = 0; h < DU; h++) {
-                    if (h < d) {
+            for (int c = 0; c < CPT; c++) xa[c] = ya.x[c], xb[c] = yb.x[c];
+        }
+        const ClusterTwiddles<F, W> tf = fwd_twiddles<0, F>(0, tw);
+        W *col = buf + pad(tid);
+        // digit h of x through the first F stages, into poly p
+        auto put = [&](auto hv, const W (&x)[CPT], int h, int p) {
+            Words<F, W> y;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) y.x[c] = (x[c] >> (h * bits)) & mask;
+            y = fwd_regs<F, decltype(hv)::value, false>(y, false, A, tw, tf);
+#pragma unroll
+            for (int c = 0; c < CPT; c++) col[p * PS + pad(c * NT)] = y.x[c];
+        };
+        auto split = [&](auto hv) {
+            if constexpr (D > 0) {
+#pragma unroll
+                for (int h = 0; h < D; h++) {
+                    put(hv, xa, h, 2 * h);
+                    put(hv, xb, h, 2 * h + 1);
+                }
+            } else {
+#pragma unroll 1
+                for (int h = 0; h < d; h++) {
+                    put(hv, xa, h, 2 * h);
+                    put(hv, xb, h, 2 * h + 1);
+                }
+            }
+        };
+        if constexpr (sizeof(W) == 4) {
+            if (A.harvey) split(std::true_type{});
+            else split(std::false_type{});
+        } else {
+            split(std::false_type{});
+        }
+        __syncthreads();
+        ntt_forward<LOGN, NT, MAXR, 2 * D, F>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
+    } else {
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int k = tid + c * NT;
+            if (kFull || k < N) {
+                W *col = buf + pad(k);
+                if (d <= DU) {
+#pragma unroll
+                    for (int h = 0; h < DU; h++) {
+                        if (h < d) {
+                            col[2 * h * PS] = (xa[c] >> (h * bits)) & mask;
+                            col[(2 * h + 1) * PS] = (xb[c] >> (h * bits)) & mask;
+                        }
+                    }
+                } else {
+                    for (int h = 0; h < d; h++) {
                         col[2 * h * PS] = (xa[c] >> (h * bits)) & mask;
                         col[(2 * h + 1) * PS] = (xb[c] >> (h * bits)) & mask;
                     }
                 }
-            } else {
-                for (int h = 0; h < d; h++) {
-                    col[2 * h * PS] = (xa[c] >> (h * bits)) & mask;
-                    col[(2 * h + 1) * PS] = (xb[c] >> (h * bits)) & mask;
-                }
             }
         }
+        __syncthreads();
+        ntt_forward<LOGN, NT, MAXR, 2 * D>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
     }
-    __syncthreads();
-    ntt_forward<LOGN, NT, MAXR, 2 * D>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
     // gadget rows outer, the thread's coefficients inner: row h+1's key quads
     // are in flight while row h is accumulated.  Rows alternate between two
     // register sets (no copies; one set when kLean), and each coefficient's
@@ -795,16 +849,22 @@ __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, S
 // finish.
 // Waves per SIMD the blind rotation and the CMUX chains are compiled for.  The
 // split product on 64-bit words at 512 threads runs two workgroups per CU
-// (55 KB of LDS each at N = 2048), 16 waves, so it must fit 128 registers;
-// every other shape keeps the register budget its workgroup size gives.
-template <typename W, int NT, int FORM>
+// (55 KB of LDS each at N = 2048), 16 waves, so it must fit 128 registers; so
+// must the in-place product on 32-bit words at 256 threads and N = 1024 (four
+// workgroups of 8 polys per CU, 37 KB of LDS each; unbounded, the fused
+// transform stages take 155 registers, three workgroups).  Every other shape
+// keeps the register budget its workgroup size gives.
+template <typename W, int LOGN, int NT, int FORM>
 constexpr int product_waves_per_eu() {
-    return (FORM == kSplit && sizeof(W) == 8 && NT == 512) ? 4 : 1;
+    return (FORM == kSplit && sizeof(W) == 8 && NT == 512) ||
+                   (FORM == kInPlace && sizeof(W) == 4 && NT == 256 && LOGN == 10)
+               ? 4
+               : 1;
 }
 
 // D > 0: a kernel for digit count D alone (launch_br_nt picks it when a.d == D)
 template <typename W, int LOGN, int NT, int FORM, int D = 0>
-__global__ __launch_bounds__(NT, (product_waves_per_eu<W, NT, FORM>())) void k_blind_rotate(BRArgs<W> a) {
+__global__ __launch_bounds__(NT, (product_waves_per_eu<W, LOGN, NT, FORM>())) void k_blind_rotate(BRArgs<W> a) {
     StepPacer none;
     br_one<W, LOGN, NT, FORM, false, D>(a, a.c_base + blockIdx.x, threadIdx.x, none, 0);
 }
@@ -828,7 +888,7 @@ constexpr int specialized_digits() {
 // without waiting at gate boundaries.  A workgroup out of work marks itself
 // finished so that no one waits for it.
 template <typename W, int LOGN, int NT, int FORM>
-__global__ __launch_bounds__(NT, (product_waves_per_eu<W, NT, FORM>())) void k_blind_rotate_paced(BRArgs<W> a) {
+__global__ __launch_bounds__(NT, (product_waves_per_eu<W, LOGN, NT, FORM>())) void k_blind_rotate_paced(BRArgs<W> a) {
     StepPacer pace;
     if (a.sync) {
         pace.prog = a.sync;
@@ -1253,7 +1313,7 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
 // ---------------------------------------------------------------------------
 
 template <typename W, int LOGN, int NT, int FORM>
-__global__ __launch_bounds__(NT, (product_waves_per_eu<W, NT, FORM>())) void k_cmux_chain(CmuxArgs<W> a) {
+__global__ __launch_bounds__(NT, (product_waves_per_eu<W, LOGN, NT, FORM>())) void k_cmux_chain(CmuxArgs<W> a) {
     constexpr int N = 1 << LOGN;
     constexpr int CPT = (N + NT - 1) / NT;
     constexpr bool kFull = N % NT == 0;

@@ -61,6 +61,11 @@ template <int MAXR>
 __host__ __device__ constexpr int last_pass_stages(int rem) {
     return rem <= pass_stages<MAXR>(rem) ? rem : last_pass_stages<MAXR>(rem - pass_stages<MAXR>(rem));
 }
+// LDS passes of that schedule over `rem` stages
+template <int MAXR>
+__host__ __device__ constexpr int pass_count(int rem) {
+    return rem <= 0 ? 0 : 1 + pass_count<MAXR>(rem - pass_stages<MAXR>(rem));
+}
 
 
 // Physical offset of cluster element e (stride S) from the cluster's first
@@ -115,14 +120,20 @@ __device__ __forceinline__ ClusterTwiddles<R, W> inv_twiddles(int blk, const Twi
     return t;
 }
 
-// HV: Harvey's butterflies (words below 4Q); FINAL: the transform's last pass
-template <int S, int R, bool HV, bool FINAL, typename W>
-__device__ __forceinline__ void fwd_cluster(W *s, int b0, bool last, const Arith<W> &A, const Twiddles<W> &tw,
-                                            const ClusterTwiddles<R, W> &t) {
-    W *base = s + pad(b0);
+// A cluster of 2^R words in registers.  The stage functions below take and
+// return it by value: passed by reference, the 64-bit transforms compile to
+// some 250 more register moves.
+template <int R, typename W>
+struct Words {
     W x[1 << R];
-#pragma unroll
-    for (int e = 0; e < (1 << R); e++) x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
+};
+
+// R forward stages on a cluster held in registers (x[e] is word b0 + e S).
+// HV: Harvey's butterflies (words below 4Q); FINAL: the transform's last pass
+template <int R, bool HV, bool FINAL, typename W>
+__device__ __forceinline__ Words<R, W> fwd_regs(Words<R, W> cl, bool last, const Arith<W> &A,
+                                                const Twiddles<W> &tw, const ClusterTwiddles<R, W> &t) {
+    W (&x)[1 << R] = cl.x;
 #pragma unroll
     for (int k = 0; k < R; k++) {
         const int h = 1 << (R - 1 - k);
@@ -141,8 +152,19 @@ __device__ __forceinline__ void fwd_cluster(W *s, int b0, bool last, const Arith
 #pragma unroll
         for (int e = 0; e < (1 << R); e++) x[e] = Arith<W>::red(x[e], A.Q2);
     }
+    return cl;
+}
+
+template <int S, int R, bool HV, bool FINAL, typename W>
+__device__ __forceinline__ void fwd_cluster(W *s, int b0, bool last, const Arith<W> &A, const Twiddles<W> &tw,
+                                            const ClusterTwiddles<R, W> &t) {
+    W *base = s + pad(b0);
+    Words<R, W> v;
 #pragma unroll
-    for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
+    for (int e = 0; e < (1 << R); e++) v.x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
+    v = fwd_regs<R, HV, FINAL>(v, last, A, tw, t);
+#pragma unroll
+    for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = v.x[e];
 }
 
 // Lazy inverse passes leave the butterfly sums unreduced.  Within a pass the
@@ -160,13 +182,10 @@ __host__ __device__ constexpr int lazy_run(int e, int k) {
 // LZ = 1: the lazy form (needs 2^(R+1) Q within the word); LZ = 0: every sum
 // is reduced to [0, 2Q) at once; LZ = 2 (64-bit words): the lazy form with
 // approximate-quotient products, below.
-template <int S, int R, bool LAST, int LZ, typename W>
-__device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, const Twiddles<W> &tw,
-                                            const ClusterTwiddles<R, W> &t) {
-    W *base = s + pad(b0);
-    W x[1 << R];
-#pragma unroll
-    for (int e = 0; e < (1 << R); e++) x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
+template <int R, bool LAST, int LZ, typename W>
+__device__ __forceinline__ Words<R, W> inv_regs(Words<R, W> cl, const Arith<W> &A, const Twiddles<W> &tw,
+                                                const ClusterTwiddles<R, W> &t) {
+    W (&x)[1 << R] = cl.x;
     int o = 0;
     if constexpr (LZ == 2 && sizeof(W) == 8) {
         // Products by mul_shoup_lazy land in [0, 5Q) (two 32-bit high
@@ -211,9 +230,7 @@ __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, con
                 else if (z > 1) x[e] = A.mul_shoup_lazy(x[e], (W)1, tw.onep);
             }
         }
-#pragma unroll
-        for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
-        return;
+        return cl;
     }
     if constexpr (LZ == 1) {
         constexpr int KS = LAST ? R - 1 : R;
@@ -249,9 +266,7 @@ __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, con
                 else if (z > 1) x[e] = A.mul_shoup(x[e], (W)1, tw.onep);
             }
         }
-#pragma unroll
-        for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
-        return;
+        return cl;
     }
     // the transform's last stage (one twiddle, irou[1]) carries the N^-1
     // scaling: its sum goes straight into the product by N^-1 (a Shoup
@@ -280,8 +295,30 @@ __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, con
             x[e + h] = Arith<W>::red(A.mul_shoup(u - v + A.Q2, tw.iNw, tw.iNwp), A.Q);
         }
     }
+    return cl;
+}
+
+// The transform's last F stages (with the N^-1 scaling) on words a thread
+// holds in registers: x[e] is word b0 + e N / 2^F.  Inputs below 2Q (5Q on
+// 64-bit words), outputs canonical.
+template <int F, typename W>
+__device__ __forceinline__ Words<F, W> inv_tail_regs(Words<F, W> x, const Arith<W> &A, const Twiddles<W> &tw,
+                                                     const ClusterTwiddles<F, W> &t) {
+    if constexpr (sizeof(W) == 8) return inv_regs<F, true, 2>(x, A, tw, t);
+    else if (F <= A.lzr) return inv_regs<F, true, 1>(x, A, tw, t);
+    else return inv_regs<F, true, 0>(x, A, tw, t);
+}
+
+template <int S, int R, bool LAST, int LZ, typename W>
+__device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, const Twiddles<W> &tw,
+                                            const ClusterTwiddles<R, W> &t) {
+    W *base = s + pad(b0);
+    Words<R, W> v;
 #pragma unroll
-    for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = x[e];
+    for (int e = 0; e < (1 << R); e++) v.x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
+    v = inv_regs<R, LAST, LZ>(v, A, tw, t);
+#pragma unroll
+    for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = v.x[e];
 }
 
 // Forward pass: stages S0 .. S0+R-1 over np polys of 2^LOGN words each.
@@ -376,18 +413,18 @@ __device__ __forceinline__ void fwd_passes(W *s, int np, bool canonical, const A
     }
 }
 
-// LP: every pass in the lazy-product form (its 5Q bound between passes holds
-// only when all passes use it); otherwise each pass is lazy or strict by its
-// own stage count.
-template <int LOGN, int ST, int NT, int MAXR, bool LP, int NP, typename W>
+// Stages ST .. END-1.  LP: every pass in the lazy-product form (its 5Q bound
+// between passes holds only when all passes use it); otherwise each pass is
+// lazy or strict by its own stage count.
+template <int LOGN, int ST, int END, int NT, int MAXR, bool LP, int NP, typename W>
 __device__ __forceinline__ void inv_passes(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid) {
-    if constexpr (ST < LOGN) {
-        constexpr int R = last_pass_stages<MAXR>(LOGN - ST);
+    if constexpr (ST < END) {
+        constexpr int R = last_pass_stages<MAXR>(END - ST);
         if constexpr (LP) inv_pass<LOGN, ST, R, NT, 2, NP>(s, np, A, tw, tid);
         else if (R <= A.lzr) inv_pass<LOGN, ST, R, NT, 1, NP>(s, np, A, tw, tid);
         else inv_pass<LOGN, ST, R, NT, 0, NP>(s, np, A, tw, tid);
         __syncthreads();
-        inv_passes<LOGN, ST + R, NT, MAXR, LP, NP>(s, np, A, tw, tid);
+        inv_passes<LOGN, ST + R, END, NT, MAXR, LP, NP>(s, np, A, tw, tid);
     }
 }
 
@@ -395,29 +432,32 @@ __device__ __forceinline__ void inv_passes(W *s, int np, const Arith<W> &A, cons
 // canonical in [0, Q), or, when canonical is false, below (2 logN + 1) Q
 // (below 2Q with Harvey's butterflies, below (5 logN + 1) Q on 64-bit words)
 // and congruent to the canonical word (for consumers that reduce anyway).
-// NP > 0: np is the compile-time constant NP (the passes unroll over the polys)
-template <int LOGN, int NT, int MAXR = 4, int NP = 0, typename W>
+// NP > 0: np is the compile-time constant NP (the passes unroll over the polys).
+// ST0 > 0: stages 0 .. ST0-1 have already run (in registers, fwd_regs, with
+// the same butterflies as this transform's: Harvey's when A.harvey).
+template <int LOGN, int NT, int MAXR = 4, int NP = 0, int ST0 = 0, typename W>
 __device__ void ntt_forward(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, bool canonical = true,
                             int tid = -1) {
     const int t = tid < 0 ? (int)threadIdx.x : tid;
     if constexpr (sizeof(W) == 4) {
         if (A.harvey) {
-            fwd_passes<LOGN, 0, NT, MAXR, true, NP>(s, np, canonical, A, tw, t);
+            fwd_passes<LOGN, ST0, NT, MAXR, true, NP>(s, np, canonical, A, tw, t);
             return;
         }
     }
-    fwd_passes<LOGN, 0, NT, MAXR, false, NP>(s, np, canonical, A, tw, t);
+    fwd_passes<LOGN, ST0, NT, MAXR, false, NP>(s, np, canonical, A, tw, t);
 }
 
 // Inverse transform of np padded polys in LDS (inputs in [0, 2Q)); output
-// canonical in [0, Q).
-template <int LOGN, int NT, int MAXR = 4, int NP = 0, typename W>
+// canonical in [0, Q).  END < LOGN: only stages 0 .. END-1, leaving words
+// below 2Q (5Q on 64-bit words) for inv_tail_regs to finish in registers.
+template <int LOGN, int NT, int MAXR = 4, int NP = 0, int END = LOGN, typename W>
 __device__ void ntt_inverse(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, int tid = -1) {
     const int t = tid < 0 ? (int)threadIdx.x : tid;
     // 64-bit words: every pass takes lazy products.  Words stay below
     // 5Q 2^R <= 80Q, which the context's parameter check keeps within 64 bits.
     static_assert(sizeof(W) == 4 || MAXR <= 4, "lazy-product inverse passes need 80Q < 2^64");
-    inv_passes<LOGN, 0, NT, MAXR, sizeof(W) == 8, NP>(s, np, A, tw, t);
+    inv_passes<LOGN, 0, END, NT, MAXR, sizeof(W) == 8, NP>(s, np, A, tw, t);
 }
 
 }  // namespace fheacc
