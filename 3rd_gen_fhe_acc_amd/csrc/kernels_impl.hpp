@@ -369,7 +369,17 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     // split and the LDS, when that saves an LDS pass of each transform
     constexpr int F = !kFull ? 0 : CPT >= 8 ? 3 : CPT >= 4 ? 2 : CPT >= 2 ? 1 : 0;
     constexpr bool kFuse = F > 0 && (1 << F) == CPT && pass_count<MAXR>(LOGN - F) < pass_count<MAXR>(LOGN);
-    ntt_inverse<LOGN, NT, MAXR, 2, kFuse ? LOGN - F : LOGN>(buf, 2, A, tw, tid);
+    // The forward side too, except at N = 1024 on 32-bit words at 256
+    // threads: there the unfused schedule 4,3,3 runs as fast as [2],4,4 and
+    // its LDS sees fewer bank conflicts (16 % of cycles against 19 %); at
+    // N = 512 the fused [1],4,4 is 2.7 % faster than 4,3,2.
+    constexpr bool kFuseFwd = kFuse && !(sizeof(W) == 4 && LOGN == 10 && NT == 256);
+    // The inverse runs on 2 polys only: with 16-word clusters half the
+    // threads would idle at every pass.  On 32-bit words, where 8-word
+    // clusters give every thread one (2N / 8 = NT), 3-stage passes are
+    // faster despite the extra pass (N = 1024 at 256 threads: -1.6 %).
+    constexpr int IMAXR = (sizeof(W) == 4 && MAXR > 3 && 2 * N / 8 == NT) ? 3 : MAXR;
+    ntt_inverse<LOGN, NT, IMAXR, 2, kFuse ? LOGN - F : LOGN>(buf, 2, A, tw, tid);
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
     W xa[CPT], xb[CPT];
 #pragma unroll
@@ -388,15 +398,15 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     constexpr int DU = 8;          // digit count unrolled (larger d loops)
     if constexpr (kFuse) {
         const ClusterTwiddles<F, W> ti = inv_twiddles<LOGN, LOGN - F, F>(0, tw);
-        {
-            Words<F, W> ya, yb;
+        Words<F, W> ya, yb;
 #pragma unroll
-            for (int c = 0; c < CPT; c++) ya.x[c] = xa[c], yb.x[c] = xb[c];
-            ya = inv_tail_regs<F>(ya, A, tw, ti);
-            yb = inv_tail_regs<F>(yb, A, tw, ti);
+        for (int c = 0; c < CPT; c++) ya.x[c] = xa[c], yb.x[c] = xb[c];
+        ya = inv_tail_regs<F>(ya, A, tw, ti);
+        yb = inv_tail_regs<F>(yb, A, tw, ti);
 #pragma unroll
-            for (int c = 0; c < CPT; c++) xa[c] = ya.x[c], xb[c] = yb.x[c];
-        }
+        for (int c = 0; c < CPT; c++) xa[c] = ya.x[c], xb[c] = yb.x[c];
+    }
+    if constexpr (kFuseFwd) {
         const ClusterTwiddles<F, W> tf = fwd_twiddles<0, F>(0, tw);
         W *col = buf + pad(tid);
         // digit h of x through the first F stages, into poly p
@@ -429,8 +439,6 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
         } else {
             split(std::false_type{});
         }
-        __syncthreads();
-        ntt_forward<LOGN, NT, MAXR, 2 * D, F>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
     } else {
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
@@ -453,9 +461,9 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
                 }
             }
         }
-        __syncthreads();
-        ntt_forward<LOGN, NT, MAXR, 2 * D>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
     }
+    __syncthreads();
+    ntt_forward<LOGN, NT, MAXR, 2 * D, kFuseFwd ? F : 0>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
     // gadget rows outer, the thread's coefficients inner: row h+1's key quads
     // are in flight while row h is accumulated.  Rows alternate between two
     // register sets (no copies; one set when kLean), and each coefficient's
@@ -851,9 +859,8 @@ __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, S
 // split product on 64-bit words at 512 threads runs two workgroups per CU
 // (55 KB of LDS each at N = 2048), 16 waves, so it must fit 128 registers; so
 // must the in-place product on 32-bit words at 256 threads and N = 1024 (four
-// workgroups of 8 polys per CU, 37 KB of LDS each; unbounded, the fused
-// transform stages take 155 registers, three workgroups).  Every other shape
-// keeps the register budget its workgroup size gives.
+// workgroups of 8 polys per CU, 37 KB of LDS each).  Every other shape keeps
+// the register budget its workgroup size gives.
 template <typename W, int LOGN, int NT, int FORM>
 constexpr int product_waves_per_eu() {
     return (FORM == kSplit && sizeof(W) == 8 && NT == 512) ||
