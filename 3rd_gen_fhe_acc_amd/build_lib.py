@@ -58,6 +58,7 @@ def build(verbose=False, jobs=8):
     for o in objs:
         os.remove(o)
     build_host()
+    build_diag()
     return LIB
 
 
@@ -74,6 +75,13 @@ def build_host():
         subprocess.check_call(["g++", "-O2", "-std=c++17", "-Wall", "-o", demo + ".tmp",
                                os.path.join(host, name + ".cpp"), "-lfheacc_host", "-lfheacc"] + link)
         os.replace(demo + ".tmp", demo)
+
+
+def build_diag():
+    """valu_rate: the issue rates of the integer VALU instructions the kernels use."""
+    exe = os.path.join(LIBDIR, "valu_rate")
+    subprocess.check_call([HIPCC] + FLAGS + ["-x", "hip", os.path.join(CSRC, "valu_rate.hip"), "-o", exe + ".tmp"])
+    os.replace(exe + ".tmp", exe)
 
 
 if __name__ == "__main__":
