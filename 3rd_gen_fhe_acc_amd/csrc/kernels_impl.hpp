@@ -462,6 +462,19 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
             }
         }
     }
+    constexpr uint32_t QB = kQuadWords<W> * sizeof(W);  // bytes per key quad
+    const __amdgpu_buffer_rsrc_t krs = key_rsrc(uniform_ptr(key));
+    KeyQuad<W> cur[CPT], nxt[CPT];
+    // With one register set of key quads (kLean) each row's loads would be
+    // waited for in full; row 0's go out before the forward transform, which
+    // hides them, where a compile-time digit count leaves the registers for
+    // it (the 54-bit d = 3 gate: -2.1 %; d = 6 external product: -1.7 %)
+    constexpr bool kEarlyRow0 = kLean && D > 0;
+    if constexpr (kEarlyRow0) {
+#pragma unroll
+        for (int c = 0; c < CPT; c++)
+            if (kFull || tid + c * NT < N) cur[c] = load_quad_rs(krs, (uint32_t)(tid + c * NT) * QB, 0u, key);
+    }
     __syncthreads();
     ntt_forward<LOGN, NT, MAXR, 2 * D, kFuseFwd ? F : 0>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
     // gadget rows outer, the thread's coefficients inner: row h+1's key quads
@@ -470,14 +483,12 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     // digit words are read through a pointer that moves two padded polys per
     // row, so the a and b digits are immediate offsets from it.
     MacSum<W> sa[CPT], sb[CPT];
-    KeyQuad<W> cur[CPT], nxt[CPT];
     const W *dcol[CPT];
-    constexpr uint32_t QB = kQuadWords<W> * sizeof(W);  // bytes per key quad
-    const __amdgpu_buffer_rsrc_t krs = key_rsrc(uniform_ptr(key));
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
         dcol[c] = buf + pad(tid + c * NT);
-        if (kFull || tid + c * NT < N) cur[c] = load_quad_rs(krs, (uint32_t)(tid + c * NT) * QB, 0u, key);
+        if (!kEarlyRow0 && (kFull || tid + c * NT < N))
+            cur[c] = load_quad_rs(krs, (uint32_t)(tid + c * NT) * QB, 0u, key);
     }
     auto row = [&](const KeyQuad<W> *q) {
 #pragma unroll
@@ -499,8 +510,8 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
                 q[c] = load_quad_rs(krs, (uint32_t)(tid + c * NT) * QB, ((uint32_t)h << LOGN) * QB, key);
     };
     if constexpr (kLean) {
-        // one register set of key quads (two would not fit in 128 VGPRs);
-        // the 16 waves per CU hide the loads
+        // one register set of key quads: two would not fit in 128 VGPRs, and
+        // at 128 (the d = 3 kernel) the gate ran 12 % slower
 #pragma unroll 1
         for (int h = 0; h < d; h++) {
             if (h) fetch(cur, h);
