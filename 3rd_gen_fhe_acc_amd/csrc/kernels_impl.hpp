@@ -890,13 +890,15 @@ __global__ __launch_bounds__(NT, (product_waves_per_eu<W, LOGN, NT, FORM>())) vo
 // The digit counts that get a kernel of their own at a launch shape: the
 // shapes the autotuner picks for the benchmark set (27-bit Q, N = 1024,
 // d = 4: 256 threads for batches, 1024 for a single gate), the mixed batch's
-// 54-bit set (N = 2048, d = 3) and its N = 512 set (d = 5).  Every other
-// (shape, d) runs the general kernel.
+// 54-bit set (N = 2048, d = 3) and its N = 512 set (d = 5), and the
+// reference's 256-bit application set (N = 2048, 29-bit Q, d = 3, 256
+// threads).  Every other (shape, d) runs the general kernel.
 template <typename W, int LOGN, int NT>
 constexpr int specialized_digits() {
     if constexpr (sizeof(W) == 4 && LOGN == 10 && (NT == 256 || NT == 1024)) return 4;
     if constexpr (sizeof(W) == 8 && LOGN == 11 && NT == 1024) return 3;
     if constexpr (sizeof(W) == 4 && LOGN == 9 && NT == 256) return 5;
+    if constexpr (sizeof(W) == 4 && LOGN == 11 && NT == 256) return 3;
     return 0;
 }
 
