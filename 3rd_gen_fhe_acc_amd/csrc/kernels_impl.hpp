@@ -376,9 +376,11 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     constexpr bool kFuseFwd = kFuse && !(sizeof(W) == 4 && LOGN == 10 && NT == 256);
     // The inverse runs on 2 polys only: with 16-word clusters half the
     // threads would idle at every pass.  On 32-bit words, where 8-word
-    // clusters give every thread one (2N / 8 = NT), 3-stage passes are
-    // faster despite the extra pass (N = 1024 at 256 threads: -1.6 %).
-    constexpr int IMAXR = (sizeof(W) == 4 && MAXR > 3 && 2 * N / 8 == NT) ? 3 : MAXR;
+    // clusters give every thread one (2N / 8 = NT) and the last stages run
+    // in registers, 3-stage passes are faster despite the extra pass
+    // (N = 1024 at 256 threads: -1.6 %; at N = 512, where they would leave
+    // half the threads idle too, +2.5 %).
+    constexpr int IMAXR = (sizeof(W) == 4 && MAXR > 3 && 2 * N / 8 == NT && kFuse) ? 3 : MAXR;
     ntt_inverse<LOGN, NT, IMAXR, 2, kFuse ? LOGN - F : LOGN>(buf, 2, A, tw, tid);
     const W mask = bits >= (int)(8 * sizeof(W)) ? ~(W)0 : (((W)1 << bits) - 1);
     W xa[CPT], xb[CPT];
