@@ -478,6 +478,57 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
             if (kFull || tid + c * NT < N) cur[c] = load_quad_rs(krs, (uint32_t)(tid + c * NT) * QB, 0u, key);
     }
     __syncthreads();
+    // At N = 1024 on 32-bit words at 256 threads the forward transform's last
+    // two stages run in the multiply-accumulate instead, on groups of 4
+    // consecutive coefficients: a thread reads words 4 tid .. 4 tid + 3 of
+    // each digit poly after stage 7, finishes stages 8 and 9 in registers
+    // and accumulates those 4 coefficients.  Their key quads are adjacent, 64
+    // bytes per lane.  One LDS round trip of the 2d polys fewer (-2.2 %).
+    constexpr bool kMacTail = sizeof(W) == 4 && LOGN == 10 && NT == 256 && kFull && CPT == 4;
+    if constexpr (kMacTail) {
+        constexpr int T = 2;  // forward stages in the multiply-accumulate
+        ntt_forward<LOGN, NT, MAXR, 2 * D, kFuseFwd ? F : 0, LOGN - T>(buf, 2 * d, A, tw, false, tid);
+        const ClusterTwiddles<T, W> tt = fwd_twiddles<LOGN - T, T>(tid, tw);
+        MacSum<W> sa[CPT], sb[CPT];
+        W *g = buf + pad(CPT * tid);  // 4 consecutive words: one 8-word padding block
+        auto mac_row = [&](auto hv, int h) {
+            KeyQuad<W> q[CPT];
+#pragma unroll
+            for (int c = 0; c < CPT; c++)
+                q[c] = load_quad_rs(krs, (uint32_t)(CPT * tid + c) * QB, ((uint32_t)h << LOGN) * QB, key);
+            Words<T, W> ya, yb;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) ya.x[c] = g[2 * h * PS + c], yb.x[c] = g[(2 * h + 1) * PS + c];
+            ya = fwd_regs<T, decltype(hv)::value, true>(ya, false, A, tw, tt);
+            yb = fwd_regs<T, decltype(hv)::value, true>(yb, false, A, tw, tt);
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                sa[c].add(A, ya.x[c], q[c].v[0], q[c].p[0]);
+                sb[c].add(A, ya.x[c], q[c].v[1], q[c].p[1]);
+                sa[c].add(A, yb.x[c], q[c].v[2], q[c].p[2]);
+                sb[c].add(A, yb.x[c], q[c].v[3], q[c].p[3]);
+            }
+        };
+        auto mac = [&](auto hv) {
+            if constexpr (D > 0) {
+#pragma unroll
+                for (int h = 0; h < D; h++) mac_row(hv, h);
+            } else {
+#pragma unroll 1
+                for (int h = 0; h < d; h++) mac_row(hv, h);
+            }
+        };
+        if (A.harvey) mac(std::true_type{});
+        else mac(std::false_type{});
+        // every digit word of these 4 columns has been read by this thread
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            g[c] = sa[c].get(A);
+            g[PS + c] = sb[c].get(A);
+        }
+        __syncthreads();
+        return;
+    }
     ntt_forward<LOGN, NT, MAXR, 2 * D, kFuseFwd ? F : 0>(buf, 2 * d, A, tw, MacSum<W>::kCanonicalInputs, tid);
     // gadget rows outer, the thread's coefficients inner: row h+1's key quads
     // are in flight while row h is accumulated.  Rows alternate between two

@@ -402,14 +402,14 @@ __device__ __forceinline__ void inv_pass(W *s, int np, const Arith<W> &A, const 
     }
 }
 
-template <int LOGN, int ST, int NT, int MAXR, bool HV, int NP, typename W>
+template <int LOGN, int ST, int NT, int MAXR, bool HV, int NP, int END = LOGN, typename W>
 __device__ __forceinline__ void fwd_passes(W *s, int np, bool canonical, const Arith<W> &A, const Twiddles<W> &tw,
                                            int tid) {
-    if constexpr (ST < LOGN) {
-        constexpr int R = pass_stages<MAXR>(LOGN - ST);
+    if constexpr (ST < END) {
+        constexpr int R = pass_stages<MAXR>(END - ST);
         fwd_pass<LOGN, ST, R, NT, HV, NP>(s, np, canonical && ST + R == LOGN, A, tw, tid);
         __syncthreads();
-        fwd_passes<LOGN, ST + R, NT, MAXR, HV, NP>(s, np, canonical, A, tw, tid);
+        fwd_passes<LOGN, ST + R, NT, MAXR, HV, NP, END>(s, np, canonical, A, tw, tid);
     }
 }
 
@@ -435,17 +435,18 @@ __device__ __forceinline__ void inv_passes(W *s, int np, const Arith<W> &A, cons
 // NP > 0: np is the compile-time constant NP (the passes unroll over the polys).
 // ST0 > 0: stages 0 .. ST0-1 have already run (in registers, fwd_regs, with
 // the same butterflies as this transform's: Harvey's when A.harvey).
-template <int LOGN, int NT, int MAXR = 4, int NP = 0, int ST0 = 0, typename W>
+// END < LOGN: only stages up to END-1; the caller runs the rest in registers.
+template <int LOGN, int NT, int MAXR = 4, int NP = 0, int ST0 = 0, int END = LOGN, typename W>
 __device__ void ntt_forward(W *s, int np, const Arith<W> &A, const Twiddles<W> &tw, bool canonical = true,
                             int tid = -1) {
     const int t = tid < 0 ? (int)threadIdx.x : tid;
     if constexpr (sizeof(W) == 4) {
         if (A.harvey) {
-            fwd_passes<LOGN, ST0, NT, MAXR, true, NP>(s, np, canonical, A, tw, t);
+            fwd_passes<LOGN, ST0, NT, MAXR, true, NP, END>(s, np, canonical, A, tw, t);
             return;
         }
     }
-    fwd_passes<LOGN, ST0, NT, MAXR, false, NP>(s, np, canonical, A, tw, t);
+    fwd_passes<LOGN, ST0, NT, MAXR, false, NP, END>(s, np, canonical, A, tw, t);
 }
 
 // Inverse transform of np padded polys in LDS (inputs in [0, 2Q)); output
