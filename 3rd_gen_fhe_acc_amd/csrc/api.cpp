@@ -1855,11 +1855,13 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
     }
     // the schedule at the chosen shape: one workgroup per ciphertext, tiles of
     // resident workgroups, or the persistent grid paced within a window of
-    // steps (as many workgroups per CU as fit)
+    // steps (as many workgroups per CU as fit).  Another schedule replaces
+    // one workgroup per ciphertext only when it is at least 3 % faster:
+    // repeated timings of one shape differ by 1-2 %.
     std::string rep_sch;
     int best_persist = saved.br_persist, best_window = saved.br_sync_window;
     {
-        double best_sch = 1e30;
+        double best_sch = 1e30, plain_ms = 0;
         const int cand[][2] = {{0, 0}, {64, 0}, {64, 8}, {64, 16}, {64, 32}};
         for (const auto &cd : cand) {
             c->tune.br_persist = cd[0];
@@ -1869,7 +1871,8 @@ int fheacc_autotune(fheacc_ctx *c, fheacc_keys *k, int64_t B, char *report, size
             rep_sch += std::string(rep_sch.empty() ? "" : ", ") + "{\"persist\": " + std::to_string(cd[0]) +
                        ", \"sync_window\": " + std::to_string(cd[1]) + ", \"blind_rotate_ms\": " +
                        std::to_string(br) + "}";
-            if (br < best_sch) {
+            if (cd[0] == 0 && cd[1] == 0) plain_ms = br;
+            if (br < best_sch && (cd[0] == 0 || br < 0.97 * plain_ms)) {
                 best_sch = br;
                 best_persist = cd[0];
                 best_window = cd[1];

@@ -137,7 +137,10 @@ int fheacc_ctx_set_schedule(fheacc_ctx *ctx, int persist, int sync_window);
 int fheacc_ctx_get_schedule(const fheacc_ctx *ctx, int *persist, int *sync_window);
 /* Autotuner: times the batched gate bootstrap on B synthetic ciphertexts
  * with `keys` for every candidate shape, keeps the fastest in the context,
- * and writes a JSON report of all candidates to report (may be NULL). */
+ * and writes a JSON report of all candidates to report (may be NULL).  A
+ * schedule other than persist = 0 is kept only when it is at least 3 %
+ * faster than persist = 0 at the chosen shape (timings of one shape repeat
+ * within 1-2 %). */
 int fheacc_autotune(fheacc_ctx *ctx, fheacc_keys *keys, int64_t B, char *report, size_t report_len);
 
 /* Evaluation order of the blind rotation for every later gate bootstrap on
