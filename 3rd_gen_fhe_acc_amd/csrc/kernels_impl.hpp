@@ -345,7 +345,10 @@ __device__ void ext_prod(W *acc, W *dig, const W *key, const Arith<W> &A, const 
 // thread owns coefficients k = tid + c NT: it reads its a[k], b[k] into
 // registers before the digits overwrite them, and after the transforms it
 // reads all 2d digits of k before writing the new a[k], b[k] -- slots no
-// other thread touches -- so the accumulator needs no LDS of its own.
+// other thread touches -- so the accumulator needs no LDS of its own.  (At
+// the shape whose multiply-accumulate takes the forward's last stages, the
+// thread's coefficients there are CPT tid + c instead: still columns only
+// that thread touches between the barriers.)
 // D > 0: the digit count d is the compile-time D (the transforms, the digit
 // split and the MAC unroll over the polys); D = 0: any d.
 template <typename W, int LOGN, int NT, int D = 0>
@@ -371,8 +374,9 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     constexpr bool kFuse = F > 0 && (1 << F) == CPT && pass_count<MAXR>(LOGN - F) < pass_count<MAXR>(LOGN);
     // The forward side too, except at N = 1024 on 32-bit words at 256
     // threads: there the unfused schedule 4,3,3 runs as fast as [2],4,4 and
-    // its LDS sees fewer bank conflicts (16 % of cycles against 19 %); at
-    // N = 512 the fused [1],4,4 is 2.7 % faster than 4,3,2.
+    // its LDS sees fewer bank conflicts (16 % of cycles against 19 %), and
+    // the forward's last stages run in the multiply-accumulate instead
+    // (kMacTail); at N = 512 the fused [1],4,4 is 2.7 % faster than 4,3,2.
     constexpr bool kFuseFwd = kFuse && !(sizeof(W) == 4 && LOGN == 10 && NT == 256);
     // The inverse runs on 2 polys only: with 16-word clusters half the
     // threads would idle at every pass.  On 32-bit words, where 8-word
