@@ -11,7 +11,10 @@
 // Stages are grouped into passes of up to four (a compile-time schedule per
 // log2 N): each thread loads a cluster of 2^R words at stride S, runs R
 // butterfly stages in registers and writes the cluster back, so a
-// length-1024 transform is three LDS round trips (4+3+3 stages).
+// length-1024 transform is three LDS round trips (4+3+3 stages).  A caller
+// whose threads already hold a cluster in registers can run its stages there
+// (fwd_regs, inv_regs, inv_tail_regs) and have the LDS passes start or end at
+// another stage (ntt_forward's ST0 and END, ntt_inverse's END).
 //
 // LDS layout: logical word i of a run of polynomials lives at i + i/8.  One
 // pad word per eight spreads the strided cluster accesses (strides 1..8
