@@ -1232,19 +1232,56 @@ __global__ __launch_bounds__(NT) void k_lwe_keyswitch_finish(KSArgs<W> a) {
 // batched NTT / external product / subs + RLWE key switch
 // ---------------------------------------------------------------------------
 
-template <typename W, int LOGN, int NT>
+// NP > 0: every workgroup transforms NP polys (a compile-time count): the
+// words move as 16-byte vectors, each thread's loads all in flight at once,
+// and the passes unroll over the polys.  NP = 0: per_wg polys, the last
+// workgroup fewer.
+template <typename W, int LOGN, int NT, int NP = 0>
 __global__ __launch_bounds__(NT) void k_ntt(NTTArgs<W> a) {
     constexpr int N = 1 << LOGN;
     extern __shared__ __align__(16) unsigned char smem[];
     W *s = reinterpret_cast<W *>(smem);
-    const int64_t p0 = (int64_t)blockIdx.x * a.per_wg;
-    const int np = (int)min((int64_t)a.per_wg, a.count - p0);
+    const int64_t p0 = (int64_t)blockIdx.x * (NP > 0 ? NP : a.per_wg);
+    const int np = NP > 0 ? NP : (int)min((int64_t)a.per_wg, a.count - p0);
     uint64_t *g = a.polys + p0 * N;
-    for (int i = threadIdx.x; i < np * N; i += NT) s[pad(i)] = (W)g[i];
-    __syncthreads();
     // 3-stage passes for 64-bit words here (fewer registers, more resident
     // waves; measured 15 % faster at N=1024); the fused kernels keep 4
     constexpr int MAXR = sizeof(W) == 8 ? 3 : 4;
+    if constexpr (NP > 0) {
+        constexpr int VEC = NP * N / 2;  // 16-byte vectors of two words
+        constexpr int IT = (VEC + NT - 1) / NT;
+        u64x2 *gv = reinterpret_cast<u64x2 *>(g);
+        u64x2 v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int e = threadIdx.x + it * NT;
+            if (VEC % NT == 0 || e < VEC) v[it] = gv[e];
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int e = threadIdx.x + it * NT;
+            if (VEC % NT == 0 || e < VEC) {
+                s[pad(2 * e)] = (W)v[it].x;  // words 2e, 2e + 1 share an 8-word padding block
+                s[pad(2 * e) + 1] = (W)v[it].y;
+            }
+        }
+        __syncthreads();
+        if (a.inverse) ntt_inverse<LOGN, NT, MAXR, NP>(s, NP, a.A, a.tw);
+        else ntt_forward<LOGN, NT, MAXR, NP>(s, NP, a.A, a.tw);
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int e = threadIdx.x + it * NT;
+            if (VEC % NT == 0 || e < VEC) {
+                u64x2 o;
+                o.x = s[pad(2 * e)];
+                o.y = s[pad(2 * e) + 1];
+                gv[e] = o;
+            }
+        }
+        return;
+    }
+    for (int i = threadIdx.x; i < np * N; i += NT) s[pad(i)] = (W)g[i];
+    __syncthreads();
     if (a.inverse) ntt_inverse<LOGN, NT, MAXR>(s, np, a.A, a.tw);
     else ntt_forward<LOGN, NT, MAXR>(s, np, a.A, a.tw);
     for (int i = threadIdx.x; i < np * N; i += NT) g[i] = s[pad(i)];
@@ -1984,6 +2021,14 @@ hipError_t launch_ntt(const NTTArgs<W> &a, int logN, hipStream_t st) {
     const int64_t wgs = (a.count + a.per_wg - 1) / a.per_wg;
     const size_t lds = (size_t)padded(a.per_wg << logN) * sizeof(W);
     FHEACC_LOGN_SWITCH(logN, {
+        // N >= 1024: two polys per workgroup; a compile-time count when
+        // every workgroup has both
+        if constexpr (LOGN >= 10) {
+            if (a.per_wg == 2 && a.count % 2 == 0) {
+                hipLaunchKernelGGL((k_ntt<W, LOGN, 256, 2>), dim3((unsigned)wgs), dim3(256), lds, st, a);
+                return hipGetLastError();
+            }
+        }
         hipLaunchKernelGGL((k_ntt<W, LOGN, 256>), dim3((unsigned)wgs), dim3(256), lds, st, a);
     })
     return hipGetLastError();

@@ -46,6 +46,14 @@ def test_ntt_matches_oracle(N, nbit, Q):
     t2 = dev(x)
     ctx.ntt_dev(t2, inverse=True)
     assert np.array_equal(host(t2), o.intt(x))
+    # an even count: from N = 1024 every workgroup then takes two polys
+    # (the compile-time-count kernel)
+    x4 = x[:36]
+    t4 = dev(x4)
+    ctx.ntt_dev(t4)
+    assert np.array_equal(host(t4), o.ntt(x4))
+    ctx.ntt_dev(t4, inverse=True)
+    assert np.array_equal(host(t4), x4)
 
 
 @pytest.mark.parametrize("name,narrow,harvey,lazy_inv", [("n12_N1024_q27", 1, 0, 4), ("n8_N1024_q28", 1, 0, 3),
