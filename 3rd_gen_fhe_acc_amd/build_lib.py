@@ -1,8 +1,10 @@
 """Build libfheacc.so (HIP, gfx950) in-tree.
 
 hipcc compiles the kernels and the C ABI into
-3rd_gen_fhe_acc_amd/lib/libfheacc.so.  Called by __graft_entry__.build() and
-usable directly: python 3rd_gen_fhe_acc_amd/build_lib.py
+3rd_gen_fhe_acc_amd/lib/libfheacc.so, and the same sources with the
+device-side bounds checks (-DFHEACC_DEVICE_CHECKS=1) into
+lib/libfheacc_checked.so.  Called by __graft_entry__.build() and usable
+directly: python 3rd_gen_fhe_acc_amd/build_lib.py
 """
 import os
 import subprocess
@@ -12,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfheacc.so")
+CHECKED_LIB = os.path.join(LIBDIR, "libfheacc_checked.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["kernels_br_u32.hip", "kernels_br_u64.hip", "kernels_cmux_u32.hip", "kernels_cmux_u64.hip",
@@ -25,12 +28,22 @@ FLAGS = [
 
 
 def build(verbose=False, jobs=8):
+    compile_lib(LIB, [], verbose, jobs)
+    compile_lib(CHECKED_LIB, ["-DFHEACC_DEVICE_CHECKS=1"], verbose, jobs)
+    build_host()
+    build_diag()
+    return LIB
+
+
+def compile_lib(lib, defines, verbose=False, jobs=8):
+    """Compile SOURCES with `defines` and link them into the shared library `lib`."""
     os.makedirs(LIBDIR, exist_ok=True)
+    tag = os.path.splitext(os.path.basename(lib))[0]
     objs = []
     procs = []
     for src in SOURCES:
-        obj = os.path.join(LIBDIR, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        obj = os.path.join(LIBDIR, "%s.%s.o" % (tag, os.path.splitext(src)[0]))
+        cmd = [HIPCC] + FLAGS + defines + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -51,15 +64,13 @@ def build(verbose=False, jobs=8):
         elif verbose and errs:
             sys.stderr.write(out)
     if failed:
-        raise RuntimeError("libfheacc build failed")
-    tmp = LIB + ".tmp"
+        raise RuntimeError("%s build failed" % tag)
+    tmp = lib + ".tmp"
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     for o in objs:
         os.remove(o)
-    build_host()
-    build_diag()
-    return LIB
+    return lib
 
 
 def build_host():

@@ -219,6 +219,40 @@ int fheacc_ctx_kernel_ms(fheacc_ctx *c, double *ms, int64_t *n) {
 
 const char *fheacc_last_error(void) { return g_err.c_str(); }
 
+namespace {
+// readers of the checked translation units' violation words (none in the
+// default build); registered while the library loads
+std::vector<unsigned (*)()> &check_readers() {
+    static std::vector<unsigned (*)()> r;
+    return r;
+}
+}  // namespace
+
+static_assert(kCheckLds == FHEACC_DEVCHECK_LDS && kCheckCiphertext == FHEACC_DEVCHECK_CIPHERTEXT,
+              "device check bits");
+
+void fheacc::register_check_reader(unsigned (*reader)()) { check_readers().push_back(reader); }
+
+int fheacc_device_checks_enabled(void) { return check_readers().empty() ? 0 : 1; }
+
+int fheacc_device_check_read(int device, uint32_t *bits) {
+    if (!bits) return fail("null output");
+    *bits = 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return fail("no such device");
+    int prev = 0;
+    HIP_TRY(hipGetDevice(&prev));
+    HIP_TRY(hipSetDevice(device));
+    const hipError_t e = hipDeviceSynchronize();
+    unsigned v = 0;
+    for (auto r : check_readers()) v |= r();
+    HIP_TRY(hipSetDevice(prev));
+    if (e != hipSuccess) return fail(std::string("device synchronize: ") + hipGetErrorString(e));
+    if (v == ~0u) return fail("reading the device check words failed");
+    *bits = v;
+    return 0;
+}
+
 int fheacc_ctx_create(const fheacc_params *params, int device, fheacc_ctx **out) {
     if (!out) return fail("null output handle");
     *out = nullptr;

@@ -392,6 +392,7 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
     for (int c = 0; c < CPT; c++) {
         const int k = tid + c * NT;
         if (kFull || k < N) {
+            FHEACC_CHECK_LDS(&buf[pad(N + k)]);
             xa[c] = buf[pad(k)];
             xb[c] = buf[pad(N + k)];
         }
@@ -422,7 +423,10 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
             for (int c = 0; c < CPT; c++) y.x[c] = (x[c] >> (h * bits)) & mask;
             y = fwd_regs<F, decltype(hv)::value, false>(y, false, A, tw, tf);
 #pragma unroll
-            for (int c = 0; c < CPT; c++) col[p * PS + pad(c * NT)] = y.x[c];
+            for (int c = 0; c < CPT; c++) {
+                FHEACC_CHECK_LDS(&col[p * PS + pad(c * NT)]);
+                col[p * PS + pad(c * NT)] = y.x[c];
+            }
         };
         auto split = [&](auto hv) {
             if constexpr (D > 0) {
@@ -455,12 +459,14 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
 #pragma unroll
                     for (int h = 0; h < DU; h++) {
                         if (h < d) {
+                            FHEACC_CHECK_LDS(&col[(2 * h + 1) * PS]);
                             col[2 * h * PS] = (xa[c] >> (h * bits)) & mask;
                             col[(2 * h + 1) * PS] = (xb[c] >> (h * bits)) & mask;
                         }
                     }
                 } else {
                     for (int h = 0; h < d; h++) {
+                        FHEACC_CHECK_LDS(&col[(2 * h + 1) * PS]);
                         col[2 * h * PS] = (xa[c] >> (h * bits)) & mask;
                         col[(2 * h + 1) * PS] = (xb[c] >> (h * bits)) & mask;
                     }
@@ -502,7 +508,10 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
                 q[c] = load_quad_rs(krs, (uint32_t)(CPT * tid + c) * QB, ((uint32_t)h << LOGN) * QB, key);
             Words<T, W> ya, yb;
 #pragma unroll
-            for (int c = 0; c < CPT; c++) ya.x[c] = g[2 * h * PS + c], yb.x[c] = g[(2 * h + 1) * PS + c];
+            for (int c = 0; c < CPT; c++) {
+                FHEACC_CHECK_LDS(&g[(2 * h + 1) * PS + c]);
+                ya.x[c] = g[2 * h * PS + c], yb.x[c] = g[(2 * h + 1) * PS + c];
+            }
             ya = fwd_regs<T, decltype(hv)::value, true>(ya, false, A, tw, tt);
             yb = fwd_regs<T, decltype(hv)::value, true>(yb, false, A, tw, tt);
 #pragma unroll
@@ -551,6 +560,7 @@ __device__ void ext_prod_inplace(W *buf, const W *key, const Arith<W> &A, const 
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             if (kFull || tid + c * NT < N) {
+                FHEACC_CHECK_LDS(&dcol[c][PS]);
                 const W da = dcol[c][0], db = dcol[c][PS];
                 sa[c].add(A, da, q[c].v[0], q[c].p[0]);
                 sb[c].add(A, da, q[c].v[1], q[c].p[1]);
@@ -659,6 +669,7 @@ __device__ void ext_prod_split(W *buf, const W *key, const Arith<W> &A, const Tw
     for (int c = 0; c < CPT; c++) {
         const int k = tid + c * NT;
         if (kFull || k < N) {
+            FHEACC_CHECK_LDS(&buf[pad(N + k)]);
             xa[c] = buf[pad(k)];
             xb[c] = buf[pad(N + k)];
         }
@@ -830,6 +841,8 @@ __device__ __forceinline__ void br_one(const BRArgs<W> &a, int64_t c, int tid, S
     __shared__ uint32_t s_b;
     const int n = a.n;
     const uint32_t q = a.q, qm = a.q - 1;
+    FHEACC_CHECK(c >= a.c_base && c < a.c_end, kCheckCiphertext);
+    FHEACC_CHECK_LDS(&av[n - 1]);
     // gates outside enum GATES [0, 5] are an error of the call: flagged like
     // identical operands, and the ciphertext skips its products
     int g = a.gates[a.gidx ? a.gidx[c] : c];

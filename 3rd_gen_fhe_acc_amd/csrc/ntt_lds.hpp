@@ -29,6 +29,7 @@
 // stays below 4Q, and the last pass leaves its words below 2Q.  The last
 // pass makes the words canonical when asked.
 #pragma once
+#include "device_checks.hpp"
 #include "modarith.hpp"
 
 namespace fheacc {
@@ -164,7 +165,10 @@ __device__ __forceinline__ void fwd_cluster(W *s, int b0, bool last, const Arith
     W *base = s + pad(b0);
     Words<R, W> v;
 #pragma unroll
-    for (int e = 0; e < (1 << R); e++) v.x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
+    for (int e = 0; e < (1 << R); e++) {
+        FHEACC_CHECK_LDS(&base[ClusterOffsets<S, R>::at(b0, e)]);
+        v.x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
+    }
     v = fwd_regs<R, HV, FINAL>(v, last, A, tw, t);
 #pragma unroll
     for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = v.x[e];
@@ -318,7 +322,10 @@ __device__ __forceinline__ void inv_cluster(W *s, int b0, const Arith<W> &A, con
     W *base = s + pad(b0);
     Words<R, W> v;
 #pragma unroll
-    for (int e = 0; e < (1 << R); e++) v.x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
+    for (int e = 0; e < (1 << R); e++) {
+        FHEACC_CHECK_LDS(&base[ClusterOffsets<S, R>::at(b0, e)]);
+        v.x[e] = base[ClusterOffsets<S, R>::at(b0, e)];
+    }
     v = inv_regs<R, LAST, LZ>(v, A, tw, t);
 #pragma unroll
     for (int e = 0; e < (1 << R); e++) base[ClusterOffsets<S, R>::at(b0, e)] = v.x[e];

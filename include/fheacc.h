@@ -92,6 +92,22 @@ int fheacc_ctx_info(const fheacc_ctx *ctx, fheacc_info *out);
 int fheacc_ctx_set_stream(fheacc_ctx *ctx, void *hip_stream);
 int fheacc_sync(fheacc_ctx *ctx);
 const char *fheacc_last_error(void);
+/* Device-side bounds checks.  The checked library, lib/libfheacc_checked.so
+ * (every source built with -DFHEACC_DEVICE_CHECKS=1), checks each LDS access
+ * of the transforms and external products against the workgroup's LDS
+ * allocation, and each gate workgroup's ciphertext against its launch's
+ * range.  A failed check sets a bit and the kernel goes on: it never traps.
+ * No counterpart in the reference, whose only checks are host compiler
+ * warnings (FHE_ACC_runtime/CMakeLists.txt:24,42) and the DEBUG define's rejection bounds
+ * (LWE.cpp:40-51).
+ * fheacc_device_checks_enabled: 1 in the checked library, 0 otherwise.
+ * fheacc_device_check_read: synchronizes the device, then stores in *bits the
+ * FHEACC_DEVCHECK_* bits set since the last read and clears them (always 0
+ * in the default library). */
+#define FHEACC_DEVCHECK_LDS 1
+#define FHEACC_DEVCHECK_CIPHERTEXT 2
+int fheacc_device_checks_enabled(void);
+int fheacc_device_check_read(int device, uint32_t *bits);
 /* Per-kernel timing with HIP events on the launch stream (the role of the
  * reference's chrono prints around each phase, myfpga.cpp:1431-1525).
  * enable != 0 starts recording and clears the totals; fheacc_ctx_kernel_ms
