@@ -1110,6 +1110,7 @@ __global__ __launch_bounds__(NT) void k_lwe_keyswitch_staged(KSArgs<W> a) {
                     if (st < nsb) {
                         const int d = (int)((w4[st >> 2] >> (8 * (st & 3))) & 0xFF);
                         const W *row = kt + (st * (int)Bks + d) * RS;
+                        FHEACC_CHECK_LDS(row + KB - 1);
 #pragma unroll
                         for (int vv = 0; vv < KB / V; vv++) {
                             uint4 q = *reinterpret_cast<const uint4 *>(row + vv * V);
@@ -1130,6 +1131,7 @@ __global__ __launch_bounds__(NT) void k_lwe_keyswitch_staged(KSArgs<W> a) {
             if (st < nsb) {
                 const int d = (int)((w4[st >> 2] >> (8 * (st & 3))) & 0xFF);
                 const W *row = kt + (st * (int)Bks + d) * RS;
+                FHEACC_CHECK_LDS(row + KB - 1);
 #pragma unroll
                 for (int vv = 0; vv < KB / V; vv++) {
                     uint4 q = *reinterpret_cast<const uint4 *>(row + vv * V);
@@ -1274,6 +1276,7 @@ __global__ __launch_bounds__(NT) void k_ntt(NTTArgs<W> a) {
         for (int it = 0; it < IT; it++) {
             const int e = threadIdx.x + it * NT;
             if (VEC % NT == 0 || e < VEC) {
+                FHEACC_CHECK_LDS(&s[pad(2 * e) + 1]);
                 s[pad(2 * e)] = (W)v[it].x;  // words 2e, 2e + 1 share an 8-word padding block
                 s[pad(2 * e) + 1] = (W)v[it].y;
             }
@@ -1353,6 +1356,7 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
         const int64_t ip = (int64_t)i * power;
         const int io = (int)(ip & (N - 1));
         const W v = x[pad(idx)];
+        FHEACC_CHECK_LDS(&y[pad((half << LOGN) + io)]);
         y[pad((half << LOGN) + io)] = ((ip >> LOGN) & 1) ? (v ? (W)a.A.Q - v : (W)0) : v;
     }
     __syncthreads();
@@ -1368,7 +1372,10 @@ __global__ __launch_bounds__(NT) void k_subs_keyswitch(SubsArgs<W> a) {
         const int nd = min(dch, d - h0);
         for (int k = threadIdx.x; k < N; k += NT) {
             const W v = y[pad(k)];
-            for (int hh = 0; hh < nd; hh++) dig[pad((hh << LOGN) + k)] = (v >> ((h0 + hh) * bits)) & mask;
+            for (int hh = 0; hh < nd; hh++) {
+                FHEACC_CHECK_LDS(&dig[pad((hh << LOGN) + k)]);
+                dig[pad((hh << LOGN) + k)] = (v >> ((h0 + hh) * bits)) & mask;
+            }
         }
         __syncthreads();
         ntt_forward<LOGN, NT, MAXR>(dig, nd, a.A, a.tw, MacSum<W>::kCanonicalInputs);

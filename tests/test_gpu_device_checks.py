@@ -85,6 +85,24 @@ def run_cases():
         ctx.sync()
         out["xprod_%d_%d" % (N, nbit)] = _host(o)
         ctx.close()
+    # the leveled operations: RLWE expansion (the substitution permutes LDS
+    # words by a caller-given factor), one substitution key switch, a CMUX
+    for kw in (dict(n=4, q=512, N=256, Q=0, nbit=27, BGbits=7, Br=16, Bks=32, Bks_R_bits=7),
+               dict(n=512, q=512, N=2048, Q=0x3FFFFFFFFED001, nbit=54, BGbits=9, Br=23, Bks=25, Bks_R_bits=9)):
+        N = kw["N"]
+        print("device checks: expansion / subs / cmux", N, kw["nbit"], flush=True)
+        ctx = fhe.Context(**kw)
+        keys = ctx.keys()
+        keys.generate_subs(33, ctx.rlwe_secret(32))
+        rng = np.random.default_rng(N)
+        x = rng.integers(0, ctx.Q, (1, 2, N), dtype=np.uint64)
+        out["expand_%d" % N] = ctx.expand_rlwe(keys, x)
+        out["subs_%d" % N] = ctx.subs_keyswitch(keys, x, 3)
+        c0, c1 = (rng.integers(0, ctx.Q, (2, 2, N), dtype=np.uint64) for _ in range(2))
+        rg = rng.integers(0, ctx.Q, ctx.digitG * 4 * N, dtype=np.uint64)
+        out["cmux_%d" % N] = ctx.cmux(c0, c1, rg)
+        keys.close()
+        ctx.close()
     return out
 
 
