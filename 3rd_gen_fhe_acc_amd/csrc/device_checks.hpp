@@ -5,11 +5,10 @@
 // external products is checked against the workgroup's LDS allocation (the
 // group segment size of the dispatch packet), and each gate workgroup's
 // ciphertext against its launch's range.  A failed check never traps: it
-// ORs a bit into this translation unit's violation word and the kernel goes
-// on (the hardware drops LDS writes outside the allocation and reads zero
-// there).  fheacc_device_check_read() synchronizes the device, then returns
-// and clears the OR of every translation unit's word.  Without the flag the
-// checks compile to nothing.
+// ORs a bit into this translation unit's violation word, and the access it
+// checked runs as it would without the check.  fheacc_device_check_read()
+// synchronizes the device, then returns and clears the OR of every
+// translation unit's word.  Without the flag the checks compile to nothing.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
