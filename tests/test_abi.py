@@ -53,3 +53,18 @@ def test_no_cpu_fallback(monkeypatch, tmp_path):
     monkeypatch.setattr(fhe, "_lib", None)
     with pytest.raises(fhe.FheAccError):
         fhe.lib()
+
+
+def test_checked_library_loads_with_the_same_symbols():
+    # the device-side bounds-check build exports the same C ABI, and says it
+    # is checked: each kernel translation unit registers its violation word
+    # while the library loads (no device call)
+    lib_dir = os.path.join(ROOT, "3rd_gen_fhe_acc_amd", "lib")
+    plain = C.CDLL(os.path.join(lib_dir, "libfheacc.so"))
+    checked = C.CDLL(os.path.join(lib_dir, "libfheacc_checked.so"))
+    missing = [s for s in declared_symbols() if not hasattr(checked, s)]
+    assert not missing, missing
+    assert plain.fheacc_device_checks_enabled() == 0
+    assert checked.fheacc_device_checks_enabled() == 1
+    # a null output is refused before any device is touched
+    assert checked.fheacc_device_check_read(0, None) == -1
