@@ -417,8 +417,9 @@ def one_set(fhe, torch, args, total=8192):
 def leveled(fhe, torch, B=8, reps=3):
     """RLWE expansion (expand_RLWE) at the reference's expansion app size --
     N=2048, Q=0x3FFFFFFFFED001, Bks_R=2^9 (digitKS_R=6) -- B inputs per call,
-    N-1 substitution + key switch steps each; and the leveled blind_rotate
-    (11 CMUXes, BG=2^9 d=6) over 256 RLWEs.  Seeded random words and keys."""
+    N-1 substitution + key switch steps each; the leveled blind_rotate
+    (11 CMUXes, BG=2^9 d=6) over 256 RLWEs; homo_expand of one packed query
+    and CMUX-tree selection at the same size.  Seeded random words and keys."""
     kw = dict(n=512, q=512, N=2048, Q=0x3FFFFFFFFED001, nbit=54, BGbits=9, Br=23, Bks=25, Bks_R_bits=9)
     ctx = fhe.Context(**kw)
     z = ctx.rlwe_secret(SEED + 6)
@@ -450,12 +451,47 @@ def leveled(fhe, torch, B=8, reps=3):
     e1.record()
     torch.cuda.synchronize()
     ms_br = e0.elapsed_time(e1) / reps
+    del rl, ro
+    # homo_expand (app.cpp:78-97) of one packed query: digitG expansions, then
+    # N * digitG external products by RGSW(-s) into N RGSWs (805 MB of output)
+    d = ctx.digitG
+    packed = torch.from_numpy(rng.integers(0, ctx.Q, (1, d, 2, N), dtype=np.uint64).view(np.int64)).cuda()
+    enc_sec = torch.from_numpy(rng.integers(0, ctx.Q, d * 4 * N, dtype=np.uint64).view(np.int64)).cuda()
+    he = torch.zeros((1, N, d * 4 * N), dtype=torch.int64, device="cuda")
+    ctx.homo_expand_dev(keys, packed, enc_sec, he)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        ctx.homo_expand_dev(keys, packed, enc_sec, he)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_he = e0.elapsed_time(e1) / reps
+    del he
+    # tree selection (test.cpp:1576-1582): T CMUX trees over 2^7 leaves, the
+    # reference app's 18 - log2(N) levels
+    T, L = 64, 7
+    leaves = torch.from_numpy(rng.integers(0, ctx.Q, (T, 1 << L, 2, N), dtype=np.uint64).view(np.int64)).cuda()
+    rgt = torch.from_numpy(rng.integers(0, ctx.Q, (L, d * 4 * N), dtype=np.uint64).view(np.int64)).cuda()
+    to = torch.zeros((T, 2, N), dtype=torch.int64, device="cuda")
+    ctx.cmux_tree_dev(leaves, rgt, to, L)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        ctx.cmux_tree_dev(leaves, rgt, to, L)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_tree = e0.elapsed_time(e1) / reps
     return {
         "workload": "expand_RLWE, N=2048 54-bit Q, Bks_R=2^9 (6 digits), %d inputs per call; blind_rotate (11 CMUX, "
-                    "BG=2^9 d=6) over %d RLWEs" % (B, R),
+                    "BG=2^9 d=6) over %d RLWEs; homo_expand of one packed query; %d CMUX trees of %d levels"
+                    % (B, R, T, L),
         "expansion_ms_per_call": round(ms, 3), "expansions_per_s": round(B / (ms * 1e-3), 1),
         "subs_keyswitch_per_s": round(B * (N - 1) / (ms * 1e-3), 1),
         "blind_rotate_ms_per_call": round(ms_br, 3), "cmux_per_s": round(R * 11 / (ms_br * 1e-3), 1),
+        "homo_expand_ms_per_query": round(ms_he, 3),
+        "homo_expand_external_products_per_s": round(N * d / (ms_he * 1e-3), 1),
+        "cmux_tree_ms_per_call": round(ms_tree, 3),
+        "tree_cmux_per_s": round(T * ((1 << L) - 1) / (ms_tree * 1e-3), 1),
     }
 
 
