@@ -976,8 +976,9 @@ constexpr int specialized_digits() {
 // ciphertexts c_base + w, + w + G, ...; its steps are counted across its
 // ciphertexts, so a group stays within a window of steps from gate to gate
 // without waiting at gate boundaries.  A workgroup out of work marks itself
-// finished so that no one waits for it.
-template <typename W, int LOGN, int NT, int FORM>
+// finished so that no one waits for it.  D > 0: the kernel for digit count D
+// alone, as for k_blind_rotate.
+template <typename W, int LOGN, int NT, int FORM, int D = 0>
 __global__ __launch_bounds__(NT, (product_waves_per_eu<W, LOGN, NT, FORM>())) void k_blind_rotate_paced(BRArgs<W> a) {
     StepPacer pace;
     if (a.sync) {
@@ -995,7 +996,7 @@ __global__ __launch_bounds__(NT, (product_waves_per_eu<W, LOGN, NT, FORM>())) vo
         // recomputed for each gate instead of held across the whole loop
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        br_one<W, LOGN, NT, FORM, true>(a, c, tid, pace, step0);
+        br_one<W, LOGN, NT, FORM, true, D>(a, c, tid, pace, step0);
         __syncthreads();  // the next gate rewrites the LDS
     }
     if (a.sync && threadIdx.x == 0)
@@ -1944,6 +1945,8 @@ static hipError_t launch_br_nt(const BRArgs<W> &a, int64_t B, size_t lds, hipStr
             auto kp = form == kInPlace ? k_blind_rotate_paced<W, LOGN, NT, kInPlace>
                                        : (form == kSplit ? k_blind_rotate_paced<W, LOGN, NT, kSplit>
                                                          : k_blind_rotate_paced<W, LOGN, NT, kSeparate>);
+            if constexpr (kSpecD > 0)
+                if (form == kInPlace && a.d == kSpecD) kp = k_blind_rotate_paced<W, LOGN, NT, kInPlace, kSpecD>;
             hipError_t e = hipFuncSetAttribute((const void *)kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
             int occ = 0;

@@ -756,14 +756,19 @@ def main():
     ks_ms, ks_n = kms["lwe_keyswitch"]
     br_avg = br_ms / max(br_n, 1)
     ks_avg = ks_ms / max(ks_n, 1)
+    # blind-rotation launches per batch: 1, or the tiles of a tiled schedule
+    # (P,0 runs the batch as launches of one resident wave of workgroups);
+    # the per-batch bytes and products below are divided by it, so that they
+    # are per launch like the launch time and the per-dispatch counters
+    br_per_batch = max(br_n, 1) / args.steps
     # the key bytes a batch touches: every nonzero residue (residue 0 is
     # stored but the software order never reads it)
     Br = PARAMS["Br"]
     touched_bsk = info["bsk_device_bytes"] // Br * (Br - 1)
-    br_bytes = touched_bsk + 2 * B * (n + 1) * 8 + B * 4 + B * N * dks + B * 8
+    br_bytes = int(round((touched_bsk + 2 * B * (n + 1) * 8 + B * 4 + B * N * dks + B * 8) / br_per_batch))
     achieved = br_bytes / (br_avg * 1e-3) / 1e9
     nega = (PARAMS["q"] - ((c1[:, :n] + c2[:, :n]) % PARAMS["q"])) % PARAMS["q"]
-    ext_products = int(np.count_nonzero(nega % 32) + np.count_nonzero((nega // 32) % 32))
+    ext_products = int(round((np.count_nonzero(nega % 32) + np.count_nonzero((nega // 32) % 32)) / br_per_batch))
     d = ctx.digitG
     modmul_per_ep = 2 * d * (N // 2) * 10 + 2 * (N // 2) * 10 + 2 * N + 4 * d * N
     # HBM traffic per launch from the committed FETCH/WRITE summary, for the
@@ -784,10 +789,19 @@ def main():
     valu_peak = 256 * 4 * 16 * 2.4e9
     sprof, valu_src = load_profile("sq_counters_blind_rotate_n1024_b4096.json")
     valu_instr = None
-    if sprof and valu_src["fresh"] and sprof.get("schedule", "0,0") == sched_key:
-        valu_instr = sprof["counters_per_launch"].get("SQ_INSTS_VALU")
-    elif sprof and valu_src["fresh"]:
-        valu_src.update(fresh=False, why="counters are for schedule " + sprof.get("schedule", "0,0"))
+    sent = None
+    if sprof:
+        # the full counter set is for the default schedule; the other
+        # schedules the autotuner can pick have their VALU counts beside it
+        if sprof.get("schedule", "0,0") == sched_key:
+            sent = sprof["counters_per_launch"]
+        else:
+            sent = sprof.get("schedules", {}).get(sched_key)
+    valu_src["schedule"] = sched_key
+    if sent is None and valu_src["fresh"]:
+        valu_src.update(fresh=False, why="no counters for schedule " + sched_key)
+    if sent and valu_src["fresh"]:
+        valu_instr = sent.get("SQ_INSTS_VALU")
     # the key switch: its key, digits, b words and outputs
     ks_bytes = info["ksk_device_bytes"] + B * N * dks + 8 * B + B * (n + 1) * 8
     ks_traffic = (tprof or {}).get("keyswitch", {}).get("hbm_bytes_per_launch") if traffic_src["fresh"] else None
@@ -811,7 +825,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_blind_rotate", "avg_launch_ms": round(br_avg, 4),
-                     "algorithmic_bytes_per_launch": br_bytes,
+                     "algorithmic_bytes_per_launch": br_bytes, "launches_per_batch": round(br_per_batch, 3),
                      "copy_ceiling_GBps": round(copy_gbs, 1) if copy_gbs else None,
                      "frac_of_copy_ceiling": round(achieved / copy_gbs, 5) if copy_gbs else None},
         "roofline_keyswitch": {"kernel": "k_lwe_keyswitch_staged", "bound": "hbm",
